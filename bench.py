@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Throughput of the WaterGAP 2 day loop on one MI355X: cell-days/s.
+
+Workload: the synthetic global 0.5-degree grid of 67,420 land cells (seed 1),
+FP64 storages, 100 elevation sub-cells per cell.  One "step" is one simulated
+year (365 days: calcNewDay for every cell, routing in upstream-to-downstream
+order, land-area update) run as hipGraph-captured days; the default times 10
+steps = 10 years.  Forcing for all timed days is resident in HBM before the
+timed region starts.
+
+Besides the JSON line's value, the run reports
+  * roofline: algorithmic bytes per launch of the dominant kernel divided by its
+    average duration from HIP events on the model's stream (eager launches of
+    the same kernels over a profiling window after the timed region), against
+    the 8 TB/s HBM peak, with the HBM traffic per launch when a rocprof counter
+    summary is present under profiles/;
+  * cpu_baseline: the CPU oracle (single thread) on a bounded sample of the same
+    workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cell_kernel_bytes(n_cells, n_members, level0_cells):
+    """Algorithmic HBM bytes of one cell-kernel launch (each array element the
+    kernel must read or write once; statics once per launch, shared by members)."""
+    per_member_cell = (
+        16            # forcing T, P, SW, LW (float)
+        + 2 * 800     # 100 elevation sub-cell snow storages, read + write
+        + 2 * 8 * 5   # canopy, soil, G_snow, storage transfer, LAI precip sum (r+w)
+        + 2 * 4 * 2   # LAI day counter and growing status (r+w)
+        + 2 * 8 * 2   # groundwater (r+w) -- humid cells write here
+        + 2 * 8 * 2   # local lake and wetland storage (r+w)
+        + 2 * 8 * 2   # their reduction factors (r+w)
+        + 8 * 4       # global lake, wetland, reservoir, river storage (read)
+        + 2 * 8 * 4   # reduction factors global lake/wetland/reservoir/river (r+w)
+        + 2 * 8 * 2   # river area fraction and its change (r+w)
+        + 2 * 8 * 3   # fswb fraction, land area fraction, previous land area fraction
+        + 8 * 2 + 4   # gamma, CFA, Smax
+    )
+    chain_record = 14 * 8  # written for cells with upstream inflow
+    statics = (202         # elevations int16 x 101
+               + 8 * 36    # FP64 statics (geometry, parameters, exp terms)
+               + 4 * 3     # float statics
+               + 2 * 2 + 1 * 4 + 4 * 2 + 4)  # short/char statics, CSR offsets, station slot
+    n_chain = n_cells - level0_cells
+    return (n_members * (n_cells * per_member_cell + n_chain * chain_record + level0_cells * 8)
+            + n_cells * statics)
+
+
+def traffic_from_profiles():
+    """HBM bytes per cell-kernel launch from the committed rocprof counter summary."""
+    path = os.path.join(ROOT, "profiles", "cell_kernel_hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="timed simulated years")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed simulated years")
+    ap.add_argument("--days-per-step", type=int, default=365)
+    ap.add_argument("--profile-days", type=int, default=60)
+    ap.add_argument("--cpu-days", type=int, default=240,
+                    help="days of the CPU-oracle sample (bounded CPU baseline)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; the path shards by basin with no "
+                          "per-day exchange (see DESIGN.md), but only --gpus 1 is measured"}))
+        sys.exit(2)
+
+    import numpy as np
+    import watergap2_amd as wg
+
+    t0 = time.time()
+    world = wg.World(wg.global_spec(seed=1))
+    n = world.n
+    dps = args.days_per_step
+    n_days = (args.warmup + args.steps) * dps
+    forcing = world.forcing(seed=1, day0=0, n_days=n_days)
+    model = wg.WaterBalanceModel(world)
+    model.set_stations(world.basin_outlet)
+    model.load_forcing(forcing)
+    info = model.schedule_info()
+    print(f"# setup {time.time() - t0:.1f}s: {n} cells, {world.n_basins} basins, "
+          f"{info['groups']} routing groups, depth {info['max_depth']} levels, "
+          f"{info['level0_cells']} headwater cells", file=sys.stderr, flush=True)
+
+    # warmup (also builds the graphs)
+    for s in range(args.warmup):
+        model.step_days(s * dps, 1, dps)
+    model.synchronize()
+    # timed region: K steps bracketed by device synchronisation
+    model.synchronize()
+    t_start = time.perf_counter()
+    for s in range(args.warmup, args.warmup + args.steps):
+        model.step_days(s * dps, 1, dps)
+    model.synchronize()
+    elapsed = time.perf_counter() - t_start
+    timed_days = args.steps * dps
+    value = n * timed_days / elapsed
+    ms_per_step = 1000.0 * elapsed / args.steps
+    print(f"# timed {timed_days} days in {elapsed:.3f}s -> {value:.4g} cell-days/s",
+          file=sys.stderr, flush=True)
+
+    # per-kernel durations: eager launches with HIP events on the model's stream
+    pd = min(args.profile_days, n_days)
+    model.step_days(0, 1, pd, profiled=True)
+    ms, launches = model.timing()
+    cell_ms = ms[0] / max(launches[0], 1)
+    chain_ms = ms[1] / max(launches[1], 1)
+    cell_bytes = cell_kernel_bytes(n, 1, info["level0_cells"])
+    cell_gbs = cell_bytes / (cell_ms * 1e-3) / 1e9
+    print(f"# cell kernel {cell_ms * 1e3:.1f} us/launch ({cell_gbs:.0f} GB/s algorithmic), "
+          f"chain kernel {chain_ms * 1e3:.1f} us/launch", file=sys.stderr, flush=True)
+    traffic = traffic_from_profiles()
+    roofline = {
+        "kernel": "wg_cell_kernel",
+        "bound": "hbm",
+        "achieved": round(cell_gbs, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(cell_gbs / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "algorithmic_bytes_per_launch": cell_bytes,
+        "avg_launch_us": round(cell_ms * 1e3, 2),
+        "chain_kernel_avg_launch_us": round(chain_ms * 1e3, 2),
+        "chain_kernel_note": "latency-bound level walk; share of day time %.0f%%" %
+                             (100.0 * chain_ms / (cell_ms + chain_ms)),
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import wg_oracle
+        cd = min(args.cpu_days, n_days)
+        o = wg_oracle.Oracle(world)
+        o.set_stations(world.basin_outlet)
+        t1 = time.perf_counter()
+        o.run(forcing[:cd], 1)
+        cpu_s = time.perf_counter() - t1
+        cpu = {"value": round(n * cd / cpu_s, 1), "unit": "cell-days/s", "cores": 1, "kind": "port",
+               "sample": f"CPU oracle, 1 thread, {n} cells x {cd} days of the same forcing ({cpu_s:.1f}s)"}
+        print(f"# cpu oracle {n * cd / cpu_s:.4g} cell-days/s on 1 core", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "cell-days/s at global 0.5deg (FP64)",
+        "value": round(value, 1),
+        "unit": "cell-days/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "global 0.5deg synthetic land mask, 67,420 cells, seed 1, 100 elevation "
+                               "sub-cells, 10 years of daily steps, hipGraph-captured days",
+                   "cells": n, "days_per_step": dps, "members": 1, "basins": world.n_basins,
+                   "routing_levels": info["max_depth"]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,267 @@
+/*
+ * watergap2_amd.h -- C ABI of the MI355X implementation of the WaterGAP 2 daily
+ * grid-cell water balance and lateral routing.
+ *
+ * Every entry point has C linkage, takes plain pointers and sizes, returns 0 on
+ * success or a negative WG_E_* code, and never lets an exception cross the
+ * boundary.  wg_last_error() gives the text of the last failure.
+ *
+ * Cell arrays are in the reference's cell order n = 0..N-1 (the order of the
+ * reference's Grid<> containers, grid.h:58-136).  Per-cell channel arrays are
+ * cell-major, channel-contiguous like VariableChannelGrid (grid.h:471-474):
+ * elevation[n*101 + ch].  The library permutes to its own device layout
+ * internally; callers never see it.
+ *
+ * Reference interfaces replaced (file:line in HydrologyFrankfurt/WaterGAP2
+ * source/):
+ *   wg_rout_order            rout_prepare.cpp:834-886  rout_order()
+ *   wg_inflow_outflow        rout_prepare.cpp:338-439, 619-670  inflow_cells(), outflow_cell()
+ *   wg_create                routing.cpp:131-742 routingClass::init, daily.cpp:1544-1682
+ *                            dailyWaterBalanceClass::init, lai.cpp:40-108 laiClass::init,
+ *                            routing.cpp:745-765 initFractionStatus, routing.cpp:5621-5720
+ *                            initLake/WetlDepthActive + setLakeWetlToMaximum,
+ *                            routing.cpp:979-1495 annualInit (first year)
+ *   wg_get_state/wg_set_state daily.cpp:1896-1924 setStorages, routing.cpp:851-882 setStorages
+ *   wg_load_forcing          climateYear.cpp:57-79 (daily .365 forcing grids)
+ *   wg_step_days             integrateWGHM.cpp:755-798: per day
+ *                            dailyWaterBalanceClass::calcNewDay (daily.cpp:94-1279) for all n,
+ *                            routingClass::routing (routing.cpp:1629-5244),
+ *                            routingClass::updateLandAreaFrac (routing.cpp:5343-5352)
+ *   wg_get_station_discharge routing.cpp:4232-4238 dailyRiverDischarge, 7232-7239 getAnnualDischarge
+ *   wg_destroy               routingClass/dailyWaterBalanceClass destructors
+ */
+#ifndef WATERGAP2_AMD_H
+#define WATERGAP2_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WG_ABI_VERSION 1
+
+/* error codes */
+#define WG_OK 0
+#define WG_E_ARG (-1)       /* invalid argument (null pointer, size out of range) */
+#define WG_E_CYCLE (-2)     /* flow-direction graph is cyclic or dangling (rout_prepare.cpp:876-881) */
+#define WG_E_HIP (-3)       /* HIP runtime error */
+#define WG_E_NOMEM (-4)     /* allocation failed */
+#define WG_E_INPUT (-5)     /* invalid input value (e.g. arid index not 0/1, daily.cpp:345-347) */
+#define WG_E_STATE (-6)     /* call out of sequence */
+
+#define WG_NSUB 100          /* elevation sub-cells per cell (daily.cpp:920) */
+#define WG_NELEV 101         /* channels of G_Elevation / G_SnowInElevation (daily.h:49,122) */
+#define WG_NLCT 18           /* land cover classes (def.h:20) */
+
+/* Calibration parameters, in the order of eCalibParam (calib_param.h:72-98). */
+enum wg_param {
+  WG_P_GAMRUN_C = 0, WG_P_CFA, WG_P_CFS, WG_M_ROOT_D, WG_M_RIVRGH_C, WG_P_LAK_D, WG_P_WET_D,
+  WG_P_SWOUTF_C, WG_M_EVAREDEX, WG_M_NETRAD, WG_P_PTC_HUM, WG_P_PTC_ARI, WG_P_PET_MXDY, WG_P_MCWH,
+  WG_M_LAI, WG_P_T_SNOWFZ, WG_P_T_SNOWMT, WG_M_DEGDAY_F, WG_P_T_GRADNT, WG_M_GW_F, WG_M_RG_MAX,
+  WG_P_PCRITGWA, WG_P_GWOUTF_C, WG_M_NETABSSW, WG_M_NETABSGW, WG_M_PREC, WG_NPARAM
+};
+
+/* Land cover tables: LCT_22.DAT (daily.cpp:1925-1959) and LAI_22.DAT (lai.cpp:110-149). */
+typedef struct wg_lct_tables {
+  float rooting_depth[WG_NLCT];
+  double albedo[WG_NLCT];
+  double albedo_snow[WG_NLCT];
+  double ddf[WG_NLCT];
+  double emissivity[WG_NLCT];
+  float lai[WG_NLCT];
+  float decid_frac[WG_NLCT];
+  float evergreen_red[WG_NLCT];
+  int16_t initial_days[WG_NLCT];
+  double kc_min[WG_NLCT];
+  double kc_max[WG_NLCT];
+} wg_lct_tables;
+
+/*
+ * Static per-cell inputs with the reference's own element types.  All arrays
+ * have N entries unless stated.  Raw inputs, as the reference reads them from
+ * its .UNF files; wg_create derives the routing geometry from them the way
+ * routingClass::init does (routing.cpp:476-507).
+ */
+typedef struct wg_static_fields {
+  /* geometry: geo.cpp:7-81 */
+  const int16_t *row;          /* GR: 1-based row, row 1 = 89.75 N */
+  const double *cell_area;     /* GAREA by row, km2 (geo.h:39-41) */
+  const double *contfreq;      /* GCONTFREQ, % continental area */
+  /* vertical balance: daily.cpp, lai.cpp, gw_frac.cpp, s_max.cpp, land.h */
+  const int16_t *aindex;       /* G_aindex: 1 arid, 0 humid */
+  const int8_t *ldd;           /* G_LDD: 1-9 D8 (keypad), 5 sink/outlet, -1 inland sink */
+  const int8_t *land_cover;    /* 1..18 */
+  const float *built_up;       /* land.G_built_up fraction */
+  const float *smax;           /* maxSoilWaterCap.G_Smax, mm */
+  const float *gw_factor;      /* GW.getgwFactor */
+  const int16_t *rgmax;        /* GW.getRgmax, mm/d * 100 */
+  const int8_t *texture;       /* GW.G_texture */
+  const int16_t *elevation;    /* [N][101] G_Elevation, m; channel 0 = cell mean */
+  const double *params;        /* [WG_NPARAM][N] calibration parameters */
+  /* routing: routing.cpp:131-507 */
+  const double *glo_lake;      /* G_GLOLAK % */
+  const double *loc_lake;      /* G_LOCLAK % (local reservoirs already added) */
+  const double *glo_wetland;   /* G_GLOWET % */
+  const double *loc_wetland;   /* G_LOCWET % */
+  const double *lake_area;     /* G_LAKAREA km2 */
+  const double *reservoir_area;/* G_RESAREA km2 of operating reservoirs */
+  const double *glo_res;       /* G_RES % */
+  const double *stor_cap;      /* G_STORAGE_CAPACITY km3 */
+  const double *mean_outflow;  /* G_MEAN_OUTFLOW converted to m3/s (routing.cpp:424-425) */
+  const int8_t *res_type;      /* 1 irrigation, 2 other */
+  const int8_t *start_month;   /* 1..12, start of operational year */
+  const float *river_length;   /* G_RIVER_LENGTH km, before the contfreq scaling */
+  const float *river_slope;    /* G_RIVERSLOPE m/m */
+  const float *roughness;      /* G_ROUGHNESS */
+  const float *bankfull_flow;  /* G_BANKFULL m3/s */
+  const int32_t *downstream;   /* G_OUTFLC: 1-based downstream cell, 0 = none */
+  const int32_t *rout_order;   /* G_ROUT_ORDER: 1-based position of each cell in the routing loop */
+  const wg_lct_tables *lct;
+} wg_static_fields;
+
+/* Option profile.  Only the values below are implemented; wg_create rejects others. */
+typedef struct wg_options_profile {
+  int32_t time_series;          /* 1: .365 forcing */
+  int32_t cloud;                /* 1: downward LW + emissivity */
+  int32_t intercept;            /* 1 */
+  int32_t calc_albedo;          /* 0 */
+  int32_t pet_opt;              /* 0: Priestley-Taylor */
+  int32_t use_kc;               /* 0 */
+  int32_t river_velo_opt;       /* 2: storage-based Manning velocity */
+  int32_t subtract_use;         /* 0 */
+  int32_t res_opt;              /* 1 */
+  int32_t statcorr_opt;         /* 0 */
+  int32_t arid_area_opt;        /* 1 */
+  int32_t fractional_routing_opt; /* 1 */
+  int32_t river_evapo_opt;      /* 1 */
+  int32_t aggr_nus_glolakres_opt; /* 0 or 1; identical results without water use */
+} wg_options_profile;
+
+/* Fills the parity option profile. */
+void wg_default_options(wg_options_profile *opt);
+
+/*
+ * Prognostic state of one member, reference cell order.  Storages of the
+ * vertical balance in mm, routing storages in km3 (as the reference keeps them).
+ * Arrays of N entries, snow_elev of N*101.
+ */
+typedef struct wg_state {
+  double *canopy;        /* G_canopyWaterContent */
+  double *soil;          /* G_soilWaterContent */
+  double *snow;          /* G_snow */
+  double *snow_elev;     /* [N][101] G_SnowInElevation (channel 0 unused) */
+  double *storage_transfer; /* G_dailyStorageTransfer */
+  int32_t *lai_days;     /* G_days_since_start */
+  int32_t *lai_status;   /* G_GrowingStatus */
+  double *lai_precsum;   /* G_PrecSum */
+  double *groundwater, *loc_lake, *loc_wetl, *glo_lake, *glo_wetl, *reservoir, *river;
+  double *red_loc_lake;  /* G_locLakeAreaReductionFactor */
+  double *red_loc_wetl;  /* G_locWetlAreaReductionFactor */
+  double *red_glo_lake;  /* G_gloLakeEvapoReductionFactor */
+  double *red_glo_wetl;  /* G_gloWetlAreaReductionFactor */
+  double *red_res;       /* G_gloResEvapoReductionFactor */
+  double *red_river;     /* G_riverAreaReductionFactor */
+  double *river_frac_next; /* G_riverAreaFracNextTimestep_Frac */
+  double *fswb_frac_next;  /* G_fswbLandAreaFracNextTimestep */
+  double *laf;           /* G_landAreaFrac */
+  double *laf_prev;      /* G_landAreaFracPrevTimestep */
+  double *k_release;     /* K_release */
+  int32_t started;       /* statusStarted_landAreaFracNextTimestep (1 after the first routed day) */
+} wg_state;
+
+/* Per-cell daily diagnostics of the last simulated day (optional). */
+typedef struct wg_day_outputs {
+  double *discharge;     /* transportedVolume km3/d (routing.cpp:3955-3958) */
+  double *surface_runoff;/* G_dailyLocalSurfaceRunoff mm */
+  double *gw_recharge;   /* G_dailyGwRecharge mm */
+  double *open_water_pet;/* G_openWaterPET mm */
+  double *cell_aet_uncorr; /* dailyCellLandAET_uncorr mm */
+  double *prec;          /* precipitation after M_PREC, mm */
+} wg_day_outputs;
+
+typedef struct wg_handle wg_handle;
+
+/* ---- routing schedule (host) ---- */
+
+/* Inflow cells (9 channels, 1-based, 0 = none) and outflow cell (1-based, 0 =
+ * none) from the LDD on a lattice.  cell_at[(row-1)*720 + (col-1)] gives the
+ * 1-based cell at each 0.5-degree position (0 = no land), rows 1..360, columns
+ * 1..720 with wrap in longitude.  Replaces rout_prepare.cpp:338-439, 619-670. */
+int wg_inflow_outflow(int32_t n_cells, const int8_t *ldd, const int16_t *row, const int16_t *col,
+                      const int32_t *cell_at, int32_t *inflow9, int32_t *outflow);
+
+/* Routing order (1-based) and Kahn sweep (level, 0-based) from the 9-channel
+ * inflow cells and the outflow cell.  Returns WG_E_CYCLE when some cell cannot
+ * be ordered.  Replaces rout_prepare.cpp:834-886; level may be NULL. */
+int wg_rout_order(int32_t n_cells, const int32_t *inflow9, const int32_t *outflow,
+                  int32_t *rout_order, int32_t *level);
+
+/* ---- model ---- */
+
+/* Creates a model of n_members members sharing the static fields.  Reservoirs
+ * with reservoir_area > 0 are in operation from the first day (resYearOpt = 1,
+ * routing.cpp:5647-5720). */
+int wg_create(int32_t n_cells, const wg_static_fields *st, const wg_options_profile *opt,
+              int32_t n_members, wg_handle **out);
+int wg_destroy(wg_handle *h);
+const char *wg_last_error(const wg_handle *h);
+
+/* State of one member (member -1 in wg_set_state: every member).  NULL array
+ * pointers in the struct are skipped. */
+int wg_get_state(wg_handle *h, int32_t member, wg_state *out);
+int wg_set_state(wg_handle *h, int32_t member, const wg_state *in);
+
+/* Per-member values of the runoff coefficient gamma, the cell correction factor
+ * CFA and the soil capacity Smax (ensemble members and calibration candidates).
+ * Arrays are [n_members][N] in reference cell order; NULL keeps the current values. */
+int wg_set_member_params(wg_handle *h, const double *gamma, const double *cfa, const float *smax);
+
+/* Per-member forcing perturbation of the ensemble generator: seed per member,
+ * precipitation lognormal sigma and temperature normal sigma.  sigma 0 disables. */
+int wg_set_forcing_perturbation(wg_handle *h, const uint64_t *member_seed, double prec_sigma,
+                                double temp_sigma);
+
+/* Copies n_days of daily forcing [n_days][4][N] (temperature degC, precipitation
+ * mm/d, shortwave W/m2, longwave W/m2; reference cell order) into device memory. */
+int wg_load_forcing(wg_handle *h, const float *forcing, int32_t n_days);
+
+/* Station (outlet) cells whose daily discharge is accumulated (0-based cells). */
+int wg_set_stations(wg_handle *h, int32_t n_stations, const int32_t *cells);
+
+/* Runs n_days days: forcing day forcing_day0+i, julian day doy (1..365)
+ * starting at first_doy and wrapping after 365. */
+int wg_step_days(wg_handle *h, int32_t forcing_day0, int32_t first_doy, int32_t n_days);
+
+/* Station discharge summed over the days since the last reset [n_members][n_stations]
+ * (km3), and daily series of the last wg_step_days call [n_members][n_days][n_stations]
+ * when daily is not NULL. */
+int wg_get_station_discharge(wg_handle *h, double *sum, double *daily, int32_t max_days);
+int wg_reset_station_discharge(wg_handle *h);
+
+/* Diagnostics of the last simulated day for one member. */
+int wg_get_day_outputs(wg_handle *h, int32_t member, wg_day_outputs *out);
+
+/* Synchronises the handle's stream. */
+int wg_synchronize(wg_handle *h);
+
+/* Device id and stream (hipStream_t as void*) the handle launches on. */
+void *wg_stream(wg_handle *h);
+
+/* As wg_step_days, launched eagerly with a HIP event pair around every kernel
+ * on the handle's stream, so that wg_last_timing reports per-kernel durations. */
+int wg_step_days_profiled(wg_handle *h, int32_t forcing_day0, int32_t first_doy, int32_t n_days);
+
+/* Timing of the last wg_step_days / wg_step_days_profiled call (ms, HIP events on
+ * the handle's stream): [0] cell kernels, [1] routing-chain kernels, [2] whole
+ * call (only [2] for wg_step_days).  n_launches receives the launch counts. */
+int wg_last_timing(wg_handle *h, double *ms, int64_t *n_launches);
+
+/* Schedule facts: [0] routing groups, [1] deepest group (levels), [2] level
+ * slots, [3] cells routed by the cell kernel (no upstream cell). */
+int wg_schedule_info(wg_handle *h, int32_t *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

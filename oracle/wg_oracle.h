@@ -1,0 +1,72 @@
+/*
+ * wg_oracle.h -- CPU oracle for the WaterGAP 2 daily water balance and routing.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the HIP implementation is
+ * compared against.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it; the product library never links or calls it.
+ *
+ * It is a plain-C restatement of the reference arithmetic for the parity option
+ * profile, cell by cell in the reference's own order: calcNewDay for every cell
+ * (daily.cpp:94-1279, lai.cpp:152-306), then the serial routing loop in routing
+ * order (routing.cpp:1835-4519), then the land-area-fraction update
+ * (routing.cpp:5034-5188, 5343-5352).  State is kept in the reference layout
+ * (cell-major, channel-contiguous).  Each function cites the reference lines it
+ * follows.
+ */
+#ifndef WG_ORACLE_H
+#define WG_ORACLE_H
+
+#include <stdint.h>
+#include "../include/watergap2_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wgo_model wgo_model;
+
+/* rout_prepare.cpp:338-439, 619-670 */
+int wgo_inflow_outflow(int32_t n, const int8_t *ldd, const int16_t *row, const int16_t *col,
+                       const int32_t *cell_at, int32_t *inflow9, int32_t *outflow);
+/* rout_prepare.cpp:834-886; level = Kahn sweep index (0-based) */
+int wgo_rout_order(int32_t n, const int32_t *inflow9, const int32_t *outflow, int32_t *order,
+                   int32_t *level);
+
+/* Initialisation as routingClass::init + initFractionStatus + setStoragesToZero +
+ * setLakeWetlToMaximum + annualInit (first year) + laiClass::init. */
+wgo_model *wgo_create(int32_t n, const wg_static_fields *st);
+void wgo_destroy(wgo_model *m);
+
+/* Per-cell overrides of gamma, CFA and Smax (NULL = keep). */
+void wgo_set_params(wgo_model *m, const double *gamma, const double *cfa, const float *smax);
+
+/* One simulated day: day = julian day 1..365, month 0..11 (integrateWGHM.cpp:755-798).
+ * Forcing arrays hold N floats each. */
+void wgo_step_day(wgo_model *m, const float *temp, const float *prec, const float *sw,
+                  const float *lw, int day, int month);
+
+/* Runs n_days from a [D][4][N] forcing block starting at doy first_doy. */
+void wgo_run(wgo_model *m, const float *forcing, int32_t n_days, int32_t first_doy);
+
+int wgo_get_state(const wgo_model *m, wg_state *out);
+int wgo_set_state(wgo_model *m, const wg_state *in);
+/* Last day's per-cell diagnostics. */
+int wgo_get_day_outputs(const wgo_model *m, wg_day_outputs *out);
+
+/* Station discharge accumulation (routing.cpp:4232-4238). */
+void wgo_set_stations(wgo_model *m, int32_t n_st, const int32_t *cells);
+void wgo_station_discharge(const wgo_model *m, double *sum);
+void wgo_reset_station_discharge(wgo_model *m);
+
+/* Member forcing perturbation shared with the ensemble generator. */
+void wgo_perturb_forcing(uint64_t seed, int32_t day_index, int32_t n, double prec_sigma,
+                         double temp_sigma, const float *temp_in, const float *prec_in,
+                         float *temp_out, float *prec_out);
+
+/* Month (0..11) of julian day 1..365 (integrateWGHM.cpp:100-102 calendar). */
+int wgo_month_of_day(int day);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

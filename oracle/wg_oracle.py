@@ -1,0 +1,168 @@
+"""ctypes wrapper of the CPU oracle (oracle/libwg_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from watergap2_amd import _abi
+from watergap2_amd._abi import NELEV, STATE_FIELDS, DAY_OUTPUT_FIELDS
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwg_oracle.so")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_LIB = None
+
+
+def load():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    i32 = C.c_int32
+    P_i32, P_i16, P_i8, P_f, P_d = _abi.P_i32, _abi.P_i16, _abi.P_i8, _abi.P_f, _abi.P_d
+    lib.wgo_inflow_outflow.argtypes = [i32, P_i8, P_i16, P_i16, P_i32, P_i32, P_i32]
+    lib.wgo_rout_order.argtypes = [i32, P_i32, P_i32, P_i32, P_i32]
+    lib.wgo_create.argtypes = [i32, C.POINTER(_abi.StaticFields)]
+    lib.wgo_create.restype = C.c_void_p
+    lib.wgo_destroy.argtypes = [C.c_void_p]
+    lib.wgo_destroy.restype = None
+    lib.wgo_set_params.argtypes = [C.c_void_p, P_d, P_d, P_f]
+    lib.wgo_set_params.restype = None
+    lib.wgo_step_day.argtypes = [C.c_void_p, P_f, P_f, P_f, P_f, C.c_int, C.c_int]
+    lib.wgo_step_day.restype = None
+    lib.wgo_run.argtypes = [C.c_void_p, P_f, i32, i32]
+    lib.wgo_run.restype = None
+    lib.wgo_get_state.argtypes = [C.c_void_p, C.POINTER(_abi.StateStruct)]
+    lib.wgo_set_state.argtypes = [C.c_void_p, C.POINTER(_abi.StateStruct)]
+    lib.wgo_get_day_outputs.argtypes = [C.c_void_p, C.POINTER(_abi.DayOutputs)]
+    lib.wgo_set_stations.argtypes = [C.c_void_p, i32, P_i32]
+    lib.wgo_set_stations.restype = None
+    lib.wgo_station_discharge.argtypes = [C.c_void_p, P_d]
+    lib.wgo_station_discharge.restype = None
+    lib.wgo_reset_station_discharge.argtypes = [C.c_void_p]
+    lib.wgo_reset_station_discharge.restype = None
+    lib.wgo_perturb_forcing.argtypes = [C.c_uint64, i32, i32, C.c_double, C.c_double, P_f, P_f, P_f, P_f]
+    lib.wgo_perturb_forcing.restype = None
+    lib.wgo_month_of_day.argtypes = [C.c_int]
+    _LIB = lib
+    return lib
+
+
+def rout_order(inflow9, outflow):
+    lib = load()
+    inflow9 = np.ascontiguousarray(inflow9, dtype=np.int32)
+    outflow = np.ascontiguousarray(outflow, dtype=np.int32)
+    n = len(outflow)
+    order = np.zeros(n, dtype=np.int32)
+    level = np.zeros(n, dtype=np.int32)
+    rc = lib.wgo_rout_order(n, _abi.ptr(inflow9, _abi.P_i32), _abi.ptr(outflow, _abi.P_i32),
+                            _abi.ptr(order, _abi.P_i32), _abi.ptr(level, _abi.P_i32))
+    return rc, order, level
+
+
+def inflow_outflow(world):
+    lib = load()
+    n = world.n
+    inflow9 = np.zeros((n, 9), dtype=np.int32)
+    outflow = np.zeros(n, dtype=np.int32)
+    lib.wgo_inflow_outflow(n, _abi.ptr(world.ldd, _abi.P_i8), _abi.ptr(world.row, _abi.P_i16),
+                           _abi.ptr(world.col, _abi.P_i16), _abi.ptr(world.cell_at, _abi.P_i32),
+                           _abi.ptr(inflow9, _abi.P_i32), _abi.ptr(outflow, _abi.P_i32))
+    return inflow9, outflow
+
+
+def perturb_forcing(seed, day_index, temp, prec, prec_sigma, temp_sigma):
+    lib = load()
+    temp = np.ascontiguousarray(temp, dtype=np.float32)
+    prec = np.ascontiguousarray(prec, dtype=np.float32)
+    t_out = np.empty_like(temp)
+    p_out = np.empty_like(prec)
+    lib.wgo_perturb_forcing(int(seed), int(day_index), len(temp), prec_sigma, temp_sigma,
+                            _abi.ptr(temp, _abi.P_f), _abi.ptr(prec, _abi.P_f),
+                            _abi.ptr(t_out, _abi.P_f), _abi.ptr(p_out, _abi.P_f))
+    return t_out, p_out
+
+
+class Oracle:
+    """One CPU model: calcNewDay for every cell, serial routing, land-area update."""
+
+    def __init__(self, world):
+        self._lib = load()
+        self.world = world
+        self.n = world.n
+        self._static = world.static_fields()
+        self._m = self._lib.wgo_create(self.n, C.byref(self._static))
+        self.n_stations = 0
+
+    def __del__(self):
+        try:
+            if self._m:
+                self._lib.wgo_destroy(self._m)
+                self._m = None
+        except Exception:
+            pass
+
+    def set_params(self, gamma=None, cfa=None, smax=None):
+        g = None if gamma is None else np.ascontiguousarray(gamma, dtype=np.float64)
+        c = None if cfa is None else np.ascontiguousarray(cfa, dtype=np.float64)
+        s = None if smax is None else np.ascontiguousarray(smax, dtype=np.float32)
+        self._keep = (g, c, s)
+        self._lib.wgo_set_params(self._m, _abi.ptr(g, _abi.P_d), _abi.ptr(c, _abi.P_d),
+                                 _abi.ptr(s, _abi.P_f))
+
+    def run(self, forcing, first_doy=1):
+        forcing = np.ascontiguousarray(forcing, dtype=np.float32)
+        self._lib.wgo_run(self._m, _abi.ptr(forcing, _abi.P_f), forcing.shape[0], first_doy)
+
+    def step_day(self, temp, prec, sw, lw, doy):
+        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (temp, prec, sw, lw)]
+        month = self._lib.wgo_month_of_day(doy)
+        self._lib.wgo_step_day(self._m, *[_abi.ptr(a, _abi.P_f) for a in arrs], doy, month)
+
+    def get_state(self):
+        from watergap2_amd import new_state, state_struct
+        st = new_state(self.n)
+        s = state_struct(st)
+        self._lib.wgo_get_state(self._m, C.byref(s))
+        st["started"] = int(s.started)
+        return st
+
+    def set_state(self, st):
+        from watergap2_amd import new_state, state_struct
+        full = new_state(self.n)
+        for k in full:
+            if k in st and k != "started":
+                full[k] = np.ascontiguousarray(st[k], dtype=full[k].dtype).reshape(full[k].shape)
+        full["started"] = st.get("started", 0)
+        self._lib.wgo_set_state(self._m, C.byref(state_struct(full)))
+
+    def day_outputs(self):
+        out = {k: np.zeros(self.n) for k in DAY_OUTPUT_FIELDS}
+        s = _abi.DayOutputs(*[_abi.ptr(out[k], _abi.P_d) for k in DAY_OUTPUT_FIELDS])
+        self._lib.wgo_get_day_outputs(self._m, C.byref(s))
+        return out
+
+    def set_stations(self, cells):
+        cells = np.ascontiguousarray(cells, dtype=np.int32)
+        self.n_stations = len(cells)
+        self._lib.wgo_set_stations(self._m, len(cells), _abi.ptr(cells, _abi.P_i32))
+
+    def station_discharge(self):
+        out = np.zeros(self.n_stations)
+        self._lib.wgo_station_discharge(self._m, _abi.ptr(out, _abi.P_d))
+        return out
+
+    def reset_station_discharge(self):
+        self._lib.wgo_reset_station_discharge(self._m)

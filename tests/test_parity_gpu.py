@@ -1,0 +1,192 @@
+"""GPU (HIP kernels through the C ABI) against the CPU oracle on the same inputs.
+
+Every prognostic field after D days, the last day's per-cell discharge and the
+daily station discharge, FP64 at 1e-10 relative (tests/parity.py), integers
+bit-exact.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from parity import assert_close, compare
+
+pytestmark = pytest.mark.gpu
+
+
+def run_pair(wg, oracle, world, n_days, seed=0, first_doy=1, init_state=None, stations=None,
+             profiled=False):
+    f = world.forcing(seed=seed, day0=0, n_days=n_days)
+    m = wg.WaterBalanceModel(world)
+    o = oracle.Oracle(world)
+    if init_state is not None:
+        m.set_state(init_state)
+        o.set_state(init_state)
+    st = world.basin_outlet if stations is None else stations
+    m.set_stations(st)
+    o.set_stations(st)
+    m.load_forcing(f)
+    m.step_days(0, first_doy, n_days, profiled=profiled)
+    o.run(f, first_doy)
+    return m, o
+
+
+def check_pair(m, o, n_days, label=""):
+    assert_close(m.get_state(), o.get_state(), label=label)
+    assert_close(m.day_outputs(), o.day_outputs(), label=label + "day:")
+    tot, _ = m.station_discharge()
+    assert_close({"discharge": tot[0]}, {"discharge": o.station_discharge()}, label=label + "station:")
+
+
+def test_regional_one_year(wg, oracle, regional_world):
+    print("case: regional 2000 cells, 365 days", flush=True)
+    m, o = run_pair(wg, oracle, regional_world, 365)
+    check_pair(m, o, 365)
+    st = m.get_state()
+    assert st["started"] == 1
+    for k in ("soil", "canopy", "snow", "river", "groundwater"):
+        assert np.isfinite(st[k]).all()
+
+
+def test_regional_daily_station_series(wg, oracle, regional_world):
+    print("case: regional daily station series, 40 days", flush=True)
+    w = regional_world
+    n_days = 40
+    f = w.forcing(seed=2, day0=0, n_days=n_days)
+    m = wg.WaterBalanceModel(w)
+    o = oracle.Oracle(w)
+    cells = np.array([w.basin_outlet[0], 5, 77, 1500], dtype=np.int32)
+    m.set_stations(cells)
+    o.set_stations(cells)
+    m.load_forcing(f)
+    m.step_days(0, 100, n_days)
+    _, daily = m.station_discharge(max_days=n_days)
+    prev = np.zeros(len(cells))
+    for d in range(n_days):
+        o.step_day(f[d, 0], f[d, 1], f[d, 2], f[d, 3], (100 - 1 + d) % 365 + 1)
+        tot = o.station_discharge()
+        assert_close({"discharge": daily[0, d]}, {"discharge": tot - prev}, label=f"day{d}:")
+        prev = tot
+
+
+def test_edge_cases(wg, oracle, regional_edge_world):
+    print("case: edge-case grid 1200 cells, 120 days from day 300", flush=True)
+    m, o = run_pair(wg, oracle, regional_edge_world, 120, seed=4, first_doy=300)
+    check_pair(m, o, 120)
+    assert (regional_edge_world.smax < 0).any()
+
+
+def test_random_initial_state(wg, oracle, regional_edge_world):
+    """Random storages drive rarely taken branches: deep snow (threshold elevation),
+    empty and negative lakes, saturated soils, land area fraction zero."""
+    print("case: random initial state, 30 days", flush=True)
+    w = regional_edge_world
+    rng = np.random.default_rng(11)
+    o0 = oracle.Oracle(w)
+    st = o0.get_state()
+    n = w.n
+    st["canopy"] = rng.uniform(0, 1.5, n)
+    st["soil"] = w.smax.astype(np.float64) * rng.uniform(0, 1.3, n)
+    snow = np.zeros((n, 101))
+    deep = rng.random(n) < 0.3
+    snow[:, 1:] = rng.uniform(0, 50, (n, 100))
+    snow[deep, 1:] = rng.uniform(500, 2500, (deep.sum(), 100))
+    st["snow_elev"] = snow
+    st["snow"] = snow[:, 1:].sum(axis=1) / 100.0
+    for k in ("loc_lake", "glo_lake"):
+        st[k] = st[k] * rng.uniform(-0.9, 1.2, n)
+    for k in ("loc_wetl", "glo_wetl", "reservoir", "river", "groundwater"):
+        st[k] = st[k] * rng.uniform(0, 1.2, n) + rng.uniform(0, 1e-3, n)
+    st["groundwater"] = rng.uniform(0, 0.05, n)
+    for k in ("red_loc_lake", "red_loc_wetl", "red_glo_lake", "red_glo_wetl", "red_res"):
+        st[k] = rng.uniform(0, 1, n)
+    st["lai_days"] = rng.integers(0, 45, n).astype(np.int32)
+    st["lai_status"] = rng.integers(0, 2, n).astype(np.int32)
+    st["lai_precsum"] = rng.uniform(0, 80, n)
+    zero = rng.random(n) < 0.03
+    st["laf"][zero] = 0.0
+    st["laf_prev"] = st["laf"] * rng.uniform(0.9, 1.1, n)
+    st["storage_transfer"] = rng.uniform(0, 5, n)
+    st["started"] = 1
+    m, o = run_pair(wg, oracle, w, 30, seed=8, first_doy=20, init_state=st)
+    check_pair(m, o, 30)
+
+
+def test_eager_equals_graph(wg, oracle, regional_world):
+    print("case: eager profiled path vs graph path", flush=True)
+    w = regional_world
+    f = w.forcing(seed=1, day0=0, n_days=20)
+    a = wg.WaterBalanceModel(w)
+    b = wg.WaterBalanceModel(w)
+    for mdl in (a, b):
+        mdl.load_forcing(f)
+    a.step_days(0, 1, 20)
+    b.step_days(0, 1, 20, profiled=True)
+    sa, sb = a.get_state(), b.get_state()
+    for k in sa:
+        if k != "started":
+            assert np.array_equal(sa[k], sb[k]), k
+    ms, n = b.timing()
+    assert ms[0] > 0 and ms[1] > 0 and n[0] == 20
+
+
+def test_level_schedule_invariance(wg, regional_world):
+    """Results do not depend on how basins are packed into routing groups or on the
+    chain workgroup size (bit-identical)."""
+    print("case: schedule invariance", flush=True)
+    w = regional_world
+    f = w.forcing(seed=3, day0=0, n_days=30)
+    states = []
+    for cells, threads in ((1024, 256), (1, 64), (100000, 1024)):
+        os.environ["WG_GROUP_CELLS"] = str(cells)
+        os.environ["WG_CHAIN_THREADS"] = str(threads)
+        try:
+            m = wg.WaterBalanceModel(w)
+        finally:
+            del os.environ["WG_GROUP_CELLS"]
+            del os.environ["WG_CHAIN_THREADS"]
+        m.load_forcing(f)
+        m.step_days(0, 50, 30)
+        states.append(m.get_state())
+    for s in states[1:]:
+        for k in s:
+            if k != "started":
+                assert np.array_equal(s[k], states[0][k]), k
+
+
+def test_members_params_and_perturbation(wg, oracle, regional_world):
+    """Members with their own gamma/CFA/Smax and perturbed forcing each match a
+    separate oracle run with the same inputs."""
+    print("case: 3 members, perturbed forcing and parameters, 45 days", flush=True)
+    w = regional_world
+    n, M, n_days = w.n, 3, 45
+    rng = np.random.default_rng(5)
+    gamma = np.tile(w.params[0], (M, 1)) * rng.uniform(0.8, 1.2, (M, n))
+    cfa = rng.uniform(0.9, 1.1, (M, n))
+    smax = (np.tile(w.smax, (M, 1)) * rng.uniform(0.8, 1.2, (M, n))).astype(np.float32)
+    seeds = np.array([11, 12, 13], dtype=np.uint64)
+    f = w.forcing(seed=0, day0=0, n_days=n_days)
+    m = wg.WaterBalanceModel(w, n_members=M)
+    m.set_member_params(gamma, cfa, smax)
+    m.set_forcing_perturbation(seeds, 0.2, 1.0)
+    m.set_stations(w.basin_outlet)
+    m.load_forcing(f)
+    m.step_days(0, 1, n_days)
+    tot, _ = m.station_discharge()
+    for k in range(M):
+        o = oracle.Oracle(w)
+        o.set_params(gamma[k], cfa[k], smax[k])
+        o.set_stations(w.basin_outlet)
+        for d in range(n_days):
+            t, p = oracle.perturb_forcing(int(seeds[k]), d, f[d, 0], f[d, 1], 0.2, 1.0)
+            o.step_day(t, p, f[d, 2], f[d, 3], d + 1)
+        assert_close(m.get_state(k), o.get_state(), label=f"m{k}:")
+        assert_close({"discharge": tot[k]}, {"discharge": o.station_discharge()}, label=f"m{k}:")
+
+
+def test_global_grid_20_days(wg, oracle, global_world):
+    print("case: global 67420 cells, 20 days", flush=True)
+    m, o = run_pair(wg, oracle, global_world, 20, seed=1, first_doy=160)
+    check_pair(m, o, 20)
+    info = m.schedule_info()
+    assert info["groups"] >= 1 and info["max_depth"] == global_world.max_level + 1

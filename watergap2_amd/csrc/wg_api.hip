@@ -1,0 +1,1002 @@
+// wg_api.hip -- host side of the C ABI (include/watergap2_amd.h).
+//
+// wg_create validates the inputs, derives the routing geometry and the initial
+// state the way the reference initialises a run (routingClass::init
+// routing.cpp:131-742, initFractionStatus 745-765, setStoragesToZero 789-818,
+// setLakeWetlToMaximum 5647-5720, annualInit 979-1495, laiClass::init
+// lai.cpp:40-108), builds the level schedule, permutes every array into the
+// device order and uploads it.  wg_step_days replays hipGraphs that hold one
+// block of days each (two kernels per day plus a closing pass).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/watergap2_amd.h"
+#include "wg_device.h"
+// The kernels are compiled in this translation unit so that their launches need
+// no relocatable device code.
+#include "wg_kernels.hip"
+
+namespace {
+
+constexpr int kNsub = WG_DEV_NSUB;
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct wg_handle {
+  int32_t n = 0, m = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<DevBuf> bufs;
+  // schedule
+  std::vector<int32_t> perm;      // device index -> reference index
+  std::vector<int32_t> inv;       // reference index -> device index
+  std::vector<int32_t> level_off; // global level slots -> start p (plus sentinel)
+  std::vector<int32_t> group_first_level;
+  int32_t n_groups = 0, max_depth = 0;
+  int32_t *d_level_off = nullptr, *d_group_first = nullptr;
+  // host copies needed at the boundary
+  std::vector<double> snow_ch0;   // [M][N] channel 0 of G_SnowInElevation (never read by calcNewDay)
+  WgStatic S{};
+  WgState X{};
+  WgTables T{};
+  WgClock *d_clock = nullptr;
+  bool started = false;
+  // forcing
+  float *d_forcing = nullptr;
+  int64_t n_forcing_days = 0;
+  // stations
+  int32_t n_stations = 0;
+  std::vector<int32_t> station_cells;
+  int32_t last_daily_days = 0;
+  // graphs keyed by block length
+  std::map<int, hipGraphExec_t> graphs;
+  int block_threads_cell = 256, block_threads_chain = 256;
+  // timing
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms[3] = {0, 0, 0};
+  int64_t last_launches[3] = {0, 0, 0};
+  uint64_t *d_seeds = nullptr;
+};
+
+namespace {
+
+int fail(wg_handle *h, int code, const char *fmt, ...) {
+  if (h) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    h->err = buf;
+  }
+  return code;
+}
+
+#define HIP_CHECK(h, call)                                                                  \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail((h), WG_E_HIP, "%s failed: %s", #call, hipGetErrorString(e_));            \
+  } while (0)
+
+template <class T>
+int dev_alloc(wg_handle *h, T **out, size_t count) {
+  void *p = nullptr;
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) return fail(h, WG_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  e = hipMemsetAsync(p, 0, bytes, h->stream);
+  if (e != hipSuccess) return fail(h, WG_E_HIP, "hipMemsetAsync failed: %s", hipGetErrorString(e));
+  h->bufs.push_back({p, bytes});
+  *out = static_cast<T *>(p);
+  return WG_OK;
+}
+
+template <class T>
+int upload(wg_handle *h, T **out, const std::vector<T> &host) {
+  int rc = dev_alloc(h, out, host.size());
+  if (rc) return rc;
+  if (!host.empty()) {
+    hipError_t e = hipMemcpy(*out, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(h, WG_E_HIP, "hipMemcpy failed: %s", hipGetErrorString(e));
+  }
+  return WG_OK;
+}
+
+// host array in reference order -> device order
+template <class T>
+std::vector<T> permute(const wg_handle *h, const T *ref) {
+  std::vector<T> out(h->n);
+  for (int32_t p = 0; p < h->n; p++) out[p] = ref[h->perm[p]];
+  return out;
+}
+
+// per-member replication of a device-order vector
+template <class T>
+std::vector<T> replicate(const std::vector<T> &v, int m) {
+  std::vector<T> out;
+  out.reserve(v.size() * m);
+  for (int k = 0; k < m; k++) out.insert(out.end(), v.begin(), v.end());
+  return out;
+}
+
+bool options_supported(const wg_options_profile *o) {
+  return o->time_series == 1 && o->cloud == 1 && o->intercept == 1 && o->calc_albedo == 0 &&
+         o->pet_opt == 0 && o->use_kc == 0 && o->river_velo_opt == 2 && o->subtract_use == 0 &&
+         o->res_opt == 1 && o->statcorr_opt == 0 && o->arid_area_opt == 1 && o->fractional_routing_opt == 1 &&
+         o->river_evapo_opt == 1 && (o->aggr_nus_glolakres_opt == 0 || o->aggr_nus_glolakres_opt == 1);
+}
+
+// The per-member state arrays in the order of the wg_state struct.
+struct StateMap {
+  const char *name;
+  double *wg_state::*ref_d;
+  int32_t *wg_state::*ref_i;
+  double *WgState::*dev_d;
+  int32_t *WgState::*dev_i;
+};
+
+const StateMap kStateMap[] = {
+    {"canopy", &wg_state::canopy, nullptr, &WgState::canopy, nullptr},
+    {"soil", &wg_state::soil, nullptr, &WgState::soil, nullptr},
+    {"snow", &wg_state::snow, nullptr, &WgState::snow, nullptr},
+    {"storage_transfer", &wg_state::storage_transfer, nullptr, &WgState::transfer, nullptr},
+    {"lai_days", nullptr, &wg_state::lai_days, nullptr, &WgState::lai_days},
+    {"lai_status", nullptr, &wg_state::lai_status, nullptr, &WgState::lai_status},
+    {"lai_precsum", &wg_state::lai_precsum, nullptr, &WgState::lai_precsum, nullptr},
+    {"groundwater", &wg_state::groundwater, nullptr, &WgState::gw, nullptr},
+    {"loc_lake", &wg_state::loc_lake, nullptr, &WgState::loc_lake, nullptr},
+    {"loc_wetl", &wg_state::loc_wetl, nullptr, &WgState::loc_wetl, nullptr},
+    {"glo_lake", &wg_state::glo_lake, nullptr, &WgState::glo_lake, nullptr},
+    {"glo_wetl", &wg_state::glo_wetl, nullptr, &WgState::glo_wetl, nullptr},
+    {"reservoir", &wg_state::reservoir, nullptr, &WgState::res, nullptr},
+    {"river", &wg_state::river, nullptr, &WgState::river, nullptr},
+    {"red_loc_lake", &wg_state::red_loc_lake, nullptr, &WgState::red_ll, nullptr},
+    {"red_loc_wetl", &wg_state::red_loc_wetl, nullptr, &WgState::red_lw, nullptr},
+    {"red_glo_lake", &wg_state::red_glo_lake, nullptr, &WgState::red_gl, nullptr},
+    {"red_glo_wetl", &wg_state::red_glo_wetl, nullptr, &WgState::red_gw, nullptr},
+    {"red_res", &wg_state::red_res, nullptr, &WgState::red_res, nullptr},
+    {"red_river", &wg_state::red_river, nullptr, &WgState::red_river, nullptr},
+    {"river_frac_next", &wg_state::river_frac_next, nullptr, &WgState::river_frac_next, nullptr},
+    {"fswb_frac_next", &wg_state::fswb_frac_next, nullptr, &WgState::fswb_next, nullptr},
+    {"laf", &wg_state::laf, nullptr, &WgState::laf, nullptr},
+    {"laf_prev", &wg_state::laf_prev, nullptr, &WgState::laf_prev, nullptr},
+    {"k_release", &wg_state::k_release, nullptr, &WgState::k_release, nullptr},
+};
+
+}  // namespace
+
+extern "C" void wg_default_options(wg_options_profile *o) {
+  if (!o) return;
+  o->time_series = 1;
+  o->cloud = 1;
+  o->intercept = 1;
+  o->calc_albedo = 0;
+  o->pet_opt = 0;
+  o->use_kc = 0;
+  o->river_velo_opt = 2;
+  o->subtract_use = 0;
+  o->res_opt = 1;
+  o->statcorr_opt = 0;
+  o->arid_area_opt = 1;
+  o->fractional_routing_opt = 1;
+  o->river_evapo_opt = 1;
+  o->aggr_nus_glolakres_opt = 0;
+}
+
+extern "C" const char *wg_last_error(const wg_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+namespace {
+
+// Builds the level schedule and the device order.
+int build_schedule(wg_handle *h, const wg_static_fields *st, int group_cells) {
+  const int32_t n = h->n;
+  std::vector<int32_t> by_order(n, -1);
+  for (int32_t i = 0; i < n; i++) {
+    const int32_t o = st->rout_order[i];
+    if (o < 1 || o > n || by_order[o - 1] >= 0)
+      return fail(h, WG_E_INPUT, "rout_order is not a permutation of 1..N (cell %d)", i);
+    by_order[o - 1] = i;
+  }
+  for (int32_t i = 0; i < n; i++) {
+    const int32_t d = st->downstream[i];
+    if (d < 0 || d > n) return fail(h, WG_E_INPUT, "downstream cell of %d out of range", i);
+    if (d > 0 && st->rout_order[d - 1] <= st->rout_order[i])
+      return fail(h, WG_E_CYCLE, "routing order is not upstream-to-downstream at cell %d", i);
+  }
+  // Kahn level: 0 without upstream cells, else 1 + max level upstream
+  std::vector<int32_t> level(n, 0), outlet(n, -1);
+  for (int32_t k = 0; k < n; k++) {
+    const int32_t i = by_order[k], d = st->downstream[i];
+    if (d > 0) level[d - 1] = std::max(level[d - 1], level[i] + 1);
+  }
+  for (int32_t k = n - 1; k >= 0; k--) {
+    const int32_t i = by_order[k], d = st->downstream[i];
+    outlet[i] = d > 0 ? outlet[d - 1] : i;
+  }
+  // basins: outlet -> (depth, cells)
+  std::vector<int32_t> bid(n, -1);
+  std::vector<int32_t> b_out, b_depth, b_cells;
+  for (int32_t i = 0; i < n; i++) {
+    const int32_t o = outlet[i];
+    if (bid[o] < 0) {
+      bid[o] = int32_t(b_out.size());
+      b_out.push_back(o);
+      b_depth.push_back(0);
+      b_cells.push_back(0);
+    }
+    const int32_t b = bid[o];
+    b_depth[b] = std::max(b_depth[b], level[i] + 1);
+    b_cells[b]++;
+  }
+  const int32_t nb = int32_t(b_out.size());
+  std::vector<int32_t> border(nb);
+  for (int32_t b = 0; b < nb; b++) border[b] = b;
+  std::sort(border.begin(), border.end(), [&](int32_t a, int32_t b) {
+    if (b_depth[a] != b_depth[b]) return b_depth[a] > b_depth[b];
+    if (b_cells[a] != b_cells[b]) return b_cells[a] > b_cells[b];
+    return b_out[a] < b_out[b];
+  });
+  // greedy packing of basins into routing groups of about group_cells cells
+  std::vector<int32_t> group_of_basin(nb, -1);
+  int32_t g = -1, g_cells = 0;
+  std::vector<int32_t> g_depth;
+  for (int32_t k = 0; k < nb; k++) {
+    const int32_t b = border[k];
+    if (g < 0 || g_cells + b_cells[b] > group_cells) {
+      g++;
+      g_cells = 0;
+      g_depth.push_back(0);
+    }
+    group_of_basin[b] = g;
+    g_cells += b_cells[b];
+    g_depth[g] = std::max(g_depth[g], b_depth[b]);
+  }
+  h->n_groups = g + 1;
+  h->max_depth = 0;
+  for (int32_t d : g_depth) h->max_depth = std::max(h->max_depth, d);
+  // device order: (group, level, routing order)
+  std::vector<int32_t> cells(n);
+  for (int32_t i = 0; i < n; i++) cells[i] = i;
+  std::sort(cells.begin(), cells.end(), [&](int32_t a, int32_t b) {
+    const int32_t ga = group_of_basin[bid[outlet[a]]], gb = group_of_basin[bid[outlet[b]]];
+    if (ga != gb) return ga < gb;
+    if (level[a] != level[b]) return level[a] < level[b];
+    return st->rout_order[a] < st->rout_order[b];
+  });
+  h->perm = cells;
+  h->inv.assign(n, 0);
+  for (int32_t p = 0; p < n; p++) h->inv[cells[p]] = p;
+  // level slots
+  h->group_first_level.assign(h->n_groups + 1, 0);
+  for (int32_t gg = 0; gg < h->n_groups; gg++) h->group_first_level[gg + 1] = h->group_first_level[gg] + g_depth[gg];
+  const int32_t n_slots = h->group_first_level[h->n_groups];
+  h->level_off.assign(n_slots + 1, n);
+  std::vector<int32_t> slot_count(n_slots, 0);
+  for (int32_t i = 0; i < n; i++) {
+    const int32_t gg = group_of_basin[bid[outlet[i]]];
+    slot_count[h->group_first_level[gg] + level[i]]++;
+  }
+  int32_t acc = 0;
+  for (int32_t s = 0; s < n_slots; s++) {
+    h->level_off[s] = acc;
+    acc += slot_count[s];
+  }
+  h->level_off[n_slots] = acc;
+  return WG_OK;
+}
+
+}  // namespace
+
+extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options_profile *opt, int32_t n_members,
+                         wg_handle **out) {
+  if (!out) return WG_E_ARG;
+  *out = nullptr;
+  if (n <= 0 || !st || !opt || n_members <= 0 || n_members > 65535) return WG_E_ARG;
+  const void *req[] = {st->row, st->cell_area, st->contfreq, st->aindex, st->ldd, st->land_cover, st->built_up,
+                       st->smax, st->gw_factor, st->rgmax, st->texture, st->elevation, st->params, st->glo_lake,
+                       st->loc_lake, st->glo_wetland, st->loc_wetland, st->lake_area, st->reservoir_area,
+                       st->glo_res, st->stor_cap, st->mean_outflow, st->res_type, st->start_month,
+                       st->river_length, st->river_slope, st->roughness, st->bankfull_flow, st->downstream,
+                       st->rout_order, st->lct};
+  for (const void *p : req)
+    if (!p) return WG_E_ARG;
+  std::unique_ptr<wg_handle> h(new wg_handle());
+  h->n = n;
+  h->m = n_members;
+  if (!options_supported(opt)) return WG_E_ARG;
+  for (int32_t i = 0; i < n; i++) {
+    if (st->aindex[i] != 0 && st->aindex[i] != 1) return WG_E_INPUT;  // daily.cpp:345-347
+    if (st->land_cover[i] < 1 || st->land_cover[i] > WG_NLCT) return WG_E_INPUT;
+    if (st->reservoir_area[i] > 0. && !(st->res_type[i] == 1 || st->res_type[i] == 2)) return WG_E_INPUT;
+    if (st->reservoir_area[i] > 0. && (st->start_month[i] < 1 || st->start_month[i] > 12)) return WG_E_INPUT;
+  }
+  hipError_t e = hipGetDevice(&h->device);
+  if (e != hipSuccess) return WG_E_HIP;
+  e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return WG_E_HIP;
+  if (const char *v = getenv("WG_CHAIN_THREADS")) h->block_threads_chain = std::max(64, std::min(1024, atoi(v)));
+  int group_cells = 1024;
+  if (const char *v = getenv("WG_GROUP_CELLS")) group_cells = std::max(1, atoi(v));
+  int rc = build_schedule(h.get(), st, group_cells);
+  if (rc) {
+    *out = h.release();
+    return rc;
+  }
+  wg_handle *H = h.get();
+  const int32_t M = n_members;
+  const double *par = st->params;
+  auto P = [&](int id, int32_t i) { return par[size_t(id) * n + i]; };
+
+  // tables (lai.cpp:86-89; daily.cpp:1925-1959)
+  for (int c = 0; c < WG_NLCT; c++) {
+    H->T.albedo[c] = st->lct->albedo[c];
+    H->T.albedo_snow[c] = st->lct->albedo_snow[c];
+    H->T.ddf[c] = st->lct->ddf[c];
+    H->T.emissivity[c] = st->lct->emissivity[c];
+    H->T.lai_a[c] = float(0.1 * st->lct->decid_frac[c]);
+    H->T.lai_b[c] = (1 - st->lct->decid_frac[c]) * st->lct->evergreen_red[c];
+    H->T.initial_days[c] = st->lct->initial_days[c];
+  }
+
+  // derived statics in reference order
+  std::vector<double> alpha(n), river_length(n), river_slope(n), roughness(n), width_bf(n), bottom(n),
+      stor_max(n), lake_depth(n), wetl_depth(n), f_glo_lake(n), fswb_init(n), k_sw(n), e_sw(n), inv_ksw(n),
+      ome_sw(n), k_gw(n), e_gw(n), inv_kgw(n), ome_gw(n);
+  std::vector<float> lai_max(n);
+  std::vector<int8_t> contcell(n);
+  for (int32_t i = 0; i < n; i++) {
+    alpha[i] = st->aindex[i] == 1 ? P(WG_P_PTC_ARI, i) : P(WG_P_PTC_HUM, i);
+    lai_max[i] = float(P(WG_M_LAI, i) * st->lct->lai[st->land_cover[i] - 1]);
+    contcell[i] = (st->contfreq[i] < 0.000001) ? 0 : 1;
+    river_length[i] = double(st->river_length[i]);
+    river_length[i] *= st->contfreq[i] / 100.;
+    river_slope[i] = double(st->river_slope[i]);
+    roughness[i] = double(st->roughness[i]);
+    double bf = double(st->bankfull_flow[i]);
+    if (bf < 0.05) bf = 0.05;
+    width_bf[i] = 2.71 * pow(bf, 0.557);
+    const double depth_bf = 0.349 * pow(bf, 0.341);
+    bottom[i] = width_bf[i] - 2.0 * 2.0 * depth_bf;
+    stor_max[i] = river_length[i] * 0.5 * depth_bf / 1000. * (bottom[i] / 1000. + width_bf[i] / 1000.);
+    lake_depth[i] = (P(WG_P_LAK_D, i) * 0.001);
+    wetl_depth[i] = (P(WG_P_WET_D, i) * 0.001);
+    const double fll = st->loc_lake[i] / 100., flw = st->loc_wetland[i] / 100., fgw = st->glo_wetland[i] / 100.;
+    f_glo_lake[i] = st->glo_lake[i] / 100.;
+    fswb_init[i] = fll + flw + fgw;
+    k_sw[i] = P(WG_P_SWOUTF_C, i);
+    e_sw[i] = exp(-1. * k_sw[i]);
+    inv_ksw[i] = 1. / k_sw[i];
+    ome_sw[i] = 1. - exp(-1. * k_sw[i]);
+    k_gw[i] = P(WG_P_GWOUTF_C, i);
+    e_gw[i] = exp(-1. * k_gw[i]);
+    inv_kgw[i] = 1. / k_gw[i];
+    ome_gw[i] = 1. - exp(-1. * k_gw[i]);
+  }
+
+  // initial state (one member, reference order)
+  std::vector<double> s_river(n), s_loc_lake(n), s_loc_wetl(n), s_glo_lake(n), s_glo_wetl(n), s_res(n),
+      s_red_river(n), s_rfn(n), s_laf(n), s_one(n, 1.0), s_krel(n, 0.1);
+  for (int32_t i = 0; i < n; i++) {
+    s_river[i] = stor_max[i];
+    s_loc_lake[i] = (st->loc_lake[i] / 100.) * st->cell_area[i] * lake_depth[i];
+    s_loc_wetl[i] = (st->loc_wetland[i] / 100.) * st->cell_area[i] * wetl_depth[i];
+    s_glo_lake[i] = st->lake_area[i] * lake_depth[i];
+    s_glo_wetl[i] = (st->glo_wetland[i] / 100.) * st->cell_area[i] * wetl_depth[i];
+    s_res[i] = st->stor_cap[i];
+    s_red_river[i] = 0.5;
+    s_rfn[i] = s_red_river[i] * river_length[i] * width_bf[i] / 1000. / st->cell_area[i];
+    s_laf[i] = (st->contfreq[i] -
+                (st->glo_lake[i] + st->glo_wetland[i] + st->loc_lake[i] + st->loc_wetland[i] + st->glo_res[i]));
+    if (s_laf[i] < 0.) s_laf[i] = 0.;
+  }
+
+  // ---- upload statics ----
+  WgStatic &S = H->S;
+  S.n = n;
+  S.n_members = M;
+#define UP(field, vec)                          \
+  do {                                          \
+    auto v_ = permute(H, (vec).data());         \
+    int r_ = upload(H, &field, v_);             \
+    if (r_) { *out = h.release(); return r_; }  \
+  } while (0)
+#define UPR(field, ptr_)                        \
+  do {                                          \
+    auto v_ = permute(H, (ptr_));               \
+    int r_ = upload(H, &field, v_);             \
+    if (r_) { *out = h.release(); return r_; }  \
+  } while (0)
+  {
+    std::vector<int32_t> ref(n);
+    for (int32_t i = 0; i < n; i++) ref[i] = i;
+    int32_t *d = nullptr;
+    UP(d, ref);
+    S.ref_index = d;
+  }
+  {
+    double *d;
+    UPR(d, st->cell_area); S.cell_area = d;
+    UPR(d, st->contfreq); S.contfreq = d;
+    UP(d, alpha); S.alpha = d;
+    std::vector<double> tmp(n);
+    const int pids[] = {WG_P_PET_MXDY, WG_M_PREC, WG_M_NETRAD, WG_P_MCWH, WG_P_T_SNOWFZ, WG_P_T_SNOWMT,
+                        WG_P_T_GRADNT, WG_M_DEGDAY_F, WG_P_PCRITGWA, WG_M_EVAREDEX, WG_M_RIVRGH_C};
+    const double **dst[] = {&S.pet_mxdy, &S.m_prec, &S.m_netrad, &S.mcwh, &S.t_snowfz, &S.t_snowmt,
+                            &S.t_gradnt, &S.degday, &S.pcritgwa, &S.evarexp, &S.m_rivrgh};
+    for (size_t q = 0; q < sizeof(pids) / sizeof(pids[0]); q++) {
+      for (int32_t i = 0; i < n; i++) tmp[i] = P(pids[q], i);
+      UP(d, tmp);
+      *dst[q] = d;
+    }
+    UPR(d, st->loc_lake); S.loc_lake = d;
+    UPR(d, st->loc_wetland); S.loc_wetland = d;
+    UPR(d, st->glo_wetland); S.glo_wetland = d;
+    UPR(d, st->lake_area); S.lake_area = d;
+    UPR(d, st->reservoir_area); S.reservoir_area = d;
+    UPR(d, st->stor_cap); S.stor_cap = d;
+    UPR(d, st->mean_outflow); S.mean_outflow = d;
+    UP(d, river_length); S.river_length = d;
+    UP(d, river_slope); S.river_slope = d;
+    UP(d, roughness); S.roughness = d;
+    UP(d, width_bf); S.width_bf = d;
+    UP(d, bottom); S.bottom_width = d;
+    UP(d, stor_max); S.river_stor_max = d;
+    UP(d, lake_depth); S.lake_depth = d;
+    UP(d, wetl_depth); S.wetl_depth = d;
+    UP(d, f_glo_lake); S.f_glo_lake = d;
+    UP(d, fswb_init); S.fswb_init = d;
+    UP(d, k_sw); S.k_sw = d;
+    UP(d, e_sw); S.e_sw = d;
+    UP(d, inv_ksw); S.inv_ksw = d;
+    UP(d, ome_sw); S.ome_sw = d;
+    UP(d, k_gw); S.k_gw = d;
+    UP(d, e_gw); S.e_gw = d;
+    UP(d, inv_kgw); S.inv_kgw = d;
+    UP(d, ome_gw); S.ome_gw = d;
+  }
+  {
+    int8_t *d8;
+    UPR(d8, st->ldd); S.ldd = d8;
+    UPR(d8, st->land_cover); S.land_cover = d8;
+    UPR(d8, st->texture); S.texture = d8;
+    UP(d8, contcell); S.contcell = d8;
+    UPR(d8, st->res_type); S.res_type = d8;
+    UPR(d8, st->start_month); S.start_month = d8;
+    int16_t *d16;
+    UPR(d16, st->aindex); S.aindex = d16;
+    UPR(d16, st->rgmax); S.rgmax = d16;
+    float *df;
+    UPR(df, st->built_up); S.built_up = df;
+    UPR(df, st->gw_factor); S.gw_factor = df;
+    UP(df, lai_max); S.lai_max = df;
+    // elevation [101][N] device order
+    std::vector<int16_t> ev(size_t(WG_NELEV) * n);
+    for (int c = 0; c < WG_NELEV; c++)
+      for (int32_t p = 0; p < n; p++) ev[size_t(c) * n + p] = st->elevation[size_t(H->perm[p]) * WG_NELEV + c];
+    int16_t *dev;
+    int r_ = upload(H, &dev, ev);
+    if (r_) { *out = h.release(); return r_; }
+    S.elevation = dev;
+  }
+  // upstream CSR in device indices, sorted (device order follows routing order within a group)
+  {
+    std::vector<std::vector<int32_t>> ups(n);
+    for (int32_t i = 0; i < n; i++) {
+      const int32_t d = st->downstream[i];
+      if (d > 0) ups[H->inv[d - 1]].push_back(H->inv[i]);
+    }
+    std::vector<int32_t> off(n + 1, 0), idx;
+    for (int32_t p = 0; p < n; p++) {
+      std::sort(ups[p].begin(), ups[p].end());
+      off[p + 1] = off[p] + int32_t(ups[p].size());
+      idx.insert(idx.end(), ups[p].begin(), ups[p].end());
+    }
+    int32_t *d;
+    int r_ = upload(H, &d, off);
+    if (r_) { *out = h.release(); return r_; }
+    S.up_off = d;
+    r_ = upload(H, &d, idx);
+    if (r_) { *out = h.release(); return r_; }
+    S.up_idx = d;
+    std::vector<int32_t> none(n, -1);
+    r_ = upload(H, &d, none);
+    if (r_) { *out = h.release(); return r_; }
+    S.station = d;
+    r_ = upload(H, &H->d_level_off, H->level_off);
+    if (r_) { *out = h.release(); return r_; }
+    r_ = upload(H, &H->d_group_first, H->group_first_level);
+    if (r_) { *out = h.release(); return r_; }
+  }
+#undef UP
+#undef UPR
+
+  // ---- per-member state ----
+  WgState &X = H->X;
+  const size_t NM = size_t(n) * M;
+  double **dstate[] = {&X.canopy, &X.soil, &X.snow, &X.transfer, &X.lai_precsum, &X.gw, &X.loc_lake, &X.loc_wetl,
+                       &X.glo_lake, &X.glo_wetl, &X.res, &X.river, &X.red_ll, &X.red_lw, &X.red_gl, &X.red_gw,
+                       &X.red_res, &X.red_river, &X.river_frac_next, &X.river_frac_change, &X.fswb_next, &X.laf,
+                       &X.laf_prev, &X.k_release, &X.gamma, &X.cfa, &X.tv, &X.c_inflow, &X.c_lrir, &X.c_lgwrir,
+                       &X.c_gwr_ll, &X.c_gwr_lw, &X.c_ow_prec, &X.c_ow_pet, &X.c_recharge_km3, &X.c_rv_e,
+                       &X.c_rv_invk, &X.c_rv_ome, &X.c_rv_term, &X.c_rv_evap, &X.c_rv_prec, &X.d_surf,
+                       &X.d_recharge, &X.d_ow_pet, &X.d_aet, &X.d_prec};
+  for (double **p : dstate) {
+    rc = dev_alloc(H, p, NM);
+    if (rc) { *out = h.release(); return rc; }
+  }
+  rc = dev_alloc(H, &X.lai_days, NM);
+  if (!rc) rc = dev_alloc(H, &X.lai_status, NM);
+  if (!rc) rc = dev_alloc(H, &X.smax, NM);
+  if (!rc) rc = dev_alloc(H, &X.snow_elev, NM * kNsub);
+  if (!rc) rc = dev_alloc(H, &X.st_sum, 1);
+  if (!rc) rc = dev_alloc(H, &X.st_daily, 1);
+  if (!rc) rc = dev_alloc(H, &H->d_clock, 1);
+  if (!rc) rc = dev_alloc(H, &H->d_seeds, size_t(M));
+  if (rc) { *out = h.release(); return rc; }
+  X.member_seed = H->d_seeds;
+  X.perturb = 0;
+  H->snow_ch0.assign(NM, 0.0);
+  auto put = [&](double *dst, const std::vector<double> &ref_vec) -> int {
+    auto v = replicate(permute(H, ref_vec.data()), M);
+    hipError_t e2 = hipMemcpy(dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice);
+    return e2 == hipSuccess ? WG_OK : fail(H, WG_E_HIP, "hipMemcpy: %s", hipGetErrorString(e2));
+  };
+  std::vector<double> gam(n), cfa(n);
+  for (int32_t i = 0; i < n; i++) {
+    gam[i] = P(WG_P_GAMRUN_C, i);
+    cfa[i] = P(WG_P_CFA, i);
+  }
+  rc = put(X.river, s_river);
+  if (!rc) rc = put(X.loc_lake, s_loc_lake);
+  if (!rc) rc = put(X.loc_wetl, s_loc_wetl);
+  if (!rc) rc = put(X.glo_lake, s_glo_lake);
+  if (!rc) rc = put(X.glo_wetl, s_glo_wetl);
+  if (!rc) rc = put(X.res, s_res);
+  if (!rc) rc = put(X.red_ll, s_one);
+  if (!rc) rc = put(X.red_lw, s_one);
+  if (!rc) rc = put(X.red_gl, s_one);
+  if (!rc) rc = put(X.red_gw, s_one);
+  if (!rc) rc = put(X.red_res, s_one);
+  if (!rc) rc = put(X.red_river, s_red_river);
+  if (!rc) rc = put(X.river_frac_next, s_rfn);
+  if (!rc) rc = put(X.fswb_next, fswb_init);
+  if (!rc) rc = put(X.laf, s_laf);
+  if (!rc) rc = put(X.k_release, s_krel);
+  if (!rc) rc = put(X.gamma, gam);
+  if (!rc) rc = put(X.cfa, cfa);
+  if (!rc) {
+    auto v = replicate(permute(H, st->smax), M);
+    hipError_t e2 = hipMemcpy(X.smax, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e2 != hipSuccess) rc = fail(H, WG_E_HIP, "hipMemcpy: %s", hipGetErrorString(e2));
+  }
+  if (rc) { *out = h.release(); return rc; }
+  e = hipEventCreate(&H->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&H->ev1);
+  if (e == hipSuccess) e = hipStreamSynchronize(H->stream);
+  if (e != hipSuccess) {
+    *out = h.release();
+    return WG_E_HIP;
+  }
+  *out = h.release();
+  return WG_OK;
+}
+
+extern "C" int wg_destroy(wg_handle *h) {
+  if (!h) return WG_E_ARG;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto &b : h->bufs) (void)hipFree(b.p);
+  if (h->d_forcing) (void)hipFree(h->d_forcing);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return WG_OK;
+}
+
+extern "C" int wg_get_state(wg_handle *h, int32_t member, wg_state *o) {
+  if (!h || !o || member < 0 || member >= h->m) return WG_E_ARG;
+  HIP_CHECK(h, hipSetDevice(h->device));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  const int32_t n = h->n;
+  const size_t off = size_t(member) * n;
+  std::vector<double> bd(n);
+  std::vector<int32_t> bi(n);
+  for (const StateMap &f : kStateMap) {
+    if (f.ref_d) {
+      double *dst = o->*(f.ref_d);
+      if (!dst) continue;
+      HIP_CHECK(h, hipMemcpy(bd.data(), (h->X.*(f.dev_d)) + off, n * sizeof(double), hipMemcpyDeviceToHost));
+      for (int32_t p = 0; p < n; p++) dst[h->perm[p]] = bd[p];
+    } else {
+      int32_t *dst = o->*(f.ref_i);
+      if (!dst) continue;
+      HIP_CHECK(h, hipMemcpy(bi.data(), (h->X.*(f.dev_i)) + off, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+      for (int32_t p = 0; p < n; p++) dst[h->perm[p]] = bi[p];
+    }
+  }
+  if (o->snow_elev) {
+    std::vector<double> se(size_t(n) * kNsub);
+    HIP_CHECK(h, hipMemcpy(se.data(), h->X.snow_elev + off * kNsub, se.size() * sizeof(double),
+                           hipMemcpyDeviceToHost));
+    for (int32_t p = 0; p < n; p++) {
+      const int32_t i = h->perm[p];
+      o->snow_elev[size_t(i) * WG_NELEV] = h->snow_ch0[off + i];
+      for (int s = 0; s < kNsub; s++) o->snow_elev[size_t(i) * WG_NELEV + 1 + s] = se[size_t(s) * n + p];
+    }
+  }
+  o->started = h->started ? 1 : 0;
+  return WG_OK;
+}
+
+extern "C" int wg_set_state(wg_handle *h, int32_t member, const wg_state *in) {
+  if (!h || !in || member < -1 || member >= h->m) return WG_E_ARG;
+  HIP_CHECK(h, hipSetDevice(h->device));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  const int32_t n = h->n;
+  const int32_t m0 = member < 0 ? 0 : member, m1 = member < 0 ? h->m : member + 1;
+  std::vector<double> bd(n);
+  std::vector<int32_t> bi(n);
+  for (const StateMap &f : kStateMap) {
+    if (f.ref_d) {
+      const double *src = in->*(f.ref_d);
+      if (!src) continue;
+      for (int32_t p = 0; p < n; p++) bd[p] = src[h->perm[p]];
+      for (int32_t mm = m0; mm < m1; mm++)
+        HIP_CHECK(h, hipMemcpy((h->X.*(f.dev_d)) + size_t(mm) * n, bd.data(), n * sizeof(double),
+                               hipMemcpyHostToDevice));
+    } else {
+      const int32_t *src = in->*(f.ref_i);
+      if (!src) continue;
+      for (int32_t p = 0; p < n; p++) bi[p] = src[h->perm[p]];
+      for (int32_t mm = m0; mm < m1; mm++)
+        HIP_CHECK(h, hipMemcpy((h->X.*(f.dev_i)) + size_t(mm) * n, bi.data(), n * sizeof(int32_t),
+                               hipMemcpyHostToDevice));
+    }
+  }
+  if (in->snow_elev) {
+    std::vector<double> se(size_t(n) * kNsub);
+    for (int32_t p = 0; p < n; p++) {
+      const int32_t i = h->perm[p];
+      for (int s = 0; s < kNsub; s++) se[size_t(s) * n + p] = in->snow_elev[size_t(i) * WG_NELEV + 1 + s];
+    }
+    for (int32_t mm = m0; mm < m1; mm++) {
+      for (int32_t i = 0; i < n; i++) h->snow_ch0[size_t(mm) * n + i] = in->snow_elev[size_t(i) * WG_NELEV];
+      HIP_CHECK(h, hipMemcpy(h->X.snow_elev + size_t(mm) * n * kNsub, se.data(), se.size() * sizeof(double),
+                             hipMemcpyHostToDevice));
+    }
+  }
+  h->started = in->started != 0;
+  return WG_OK;
+}
+
+extern "C" int wg_set_member_params(wg_handle *h, const double *gamma, const double *cfa, const float *smax) {
+  if (!h) return WG_E_ARG;
+  HIP_CHECK(h, hipSetDevice(h->device));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  const int32_t n = h->n;
+  for (int32_t mm = 0; mm < h->m; mm++) {
+    const size_t off = size_t(mm) * n;
+    if (gamma) {
+      auto v = permute(h, gamma + off);
+      HIP_CHECK(h, hipMemcpy(h->X.gamma + off, v.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (cfa) {
+      auto v = permute(h, cfa + off);
+      HIP_CHECK(h, hipMemcpy(h->X.cfa + off, v.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (smax) {
+      auto v = permute(h, smax + off);
+      HIP_CHECK(h, hipMemcpy(h->X.smax + off, v.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    }
+  }
+  return WG_OK;
+}
+
+extern "C" int wg_set_forcing_perturbation(wg_handle *h, const uint64_t *seeds, double prec_sigma,
+                                           double temp_sigma) {
+  if (!h) return WG_E_ARG;
+  HIP_CHECK(h, hipSetDevice(h->device));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  if (!seeds) {
+    h->X.perturb = 0;
+  } else {
+    HIP_CHECK(h, hipMemcpy(h->d_seeds, seeds, size_t(h->m) * sizeof(uint64_t), hipMemcpyHostToDevice));
+    h->X.perturb = 1;
+    h->X.prec_sigma = prec_sigma;
+    h->X.temp_sigma = temp_sigma;
+  }
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
+  return WG_OK;
+}
+
+extern "C" int wg_load_forcing(wg_handle *h, const float *forcing, int32_t n_days) {
+  if (!h || !forcing || n_days <= 0) return WG_E_ARG;
+  HIP_CHECK(h, hipSetDevice(h->device));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  const int32_t n = h->n;
+  if (h->d_forcing) {
+    HIP_CHECK(h, hipFree(h->d_forcing));
+    h->d_forcing = nullptr;
+  }
+  const size_t day_elems = size_t(4) * n;
+  HIP_CHECK(h, hipMalloc(&h->d_forcing, day_elems * n_days * sizeof(float)));
+  std::vector<float> buf(day_elems);
+  for (int32_t d = 0; d < n_days; d++) {
+    const float *src = forcing + day_elems * d;
+    for (int v = 0; v < 4; v++)
+      for (int32_t p = 0; p < n; p++) buf[size_t(v) * n + p] = src[size_t(v) * n + h->perm[p]];
+    HIP_CHECK(h, hipMemcpy(h->d_forcing + day_elems * d, buf.data(), day_elems * sizeof(float),
+                           hipMemcpyHostToDevice));
+  }
+  h->n_forcing_days = n_days;
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
+  return WG_OK;
+}
+
+extern "C" int wg_set_stations(wg_handle *h, int32_t n_st, const int32_t *cells) {
+  if (!h || n_st < 0 || (n_st > 0 && !cells)) return WG_E_ARG;
+  HIP_CHECK(h, hipSetDevice(h->device));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  std::vector<int32_t> slot(h->n, -1);
+  for (int32_t b = 0; b < n_st; b++) {
+    if (cells[b] < 0 || cells[b] >= h->n) return fail(h, WG_E_ARG, "station cell %d out of range", cells[b]);
+    // the reference stops at the first station of a cell (routing.cpp:4232-4238)
+    if (slot[h->inv[cells[b]]] < 0) slot[h->inv[cells[b]]] = b;
+  }
+  HIP_CHECK(h, hipMemcpy(const_cast<int32_t *>(h->S.station), slot.data(), h->n * sizeof(int32_t),
+                         hipMemcpyHostToDevice));
+  // reallocate the accumulators
+  for (auto it = h->bufs.begin(); it != h->bufs.end();) {
+    if (it->p == h->X.st_sum || it->p == h->X.st_daily) {
+      (void)hipFree(it->p);
+      it = h->bufs.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  h->n_stations = n_st;
+  h->station_cells.assign(cells, cells + n_st);
+  int rc = dev_alloc(h, &h->X.st_sum, size_t(h->m) * std::max(n_st, 1));
+  if (!rc) rc = dev_alloc(h, &h->X.st_daily, size_t(h->m) * WG_DAILY_SLOTS * std::max(n_st, 1));
+  if (rc) return rc;
+  h->S.n_stations = n_st;
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
+  return WG_OK;
+}
+
+namespace {
+
+// Enqueues one block of `days` days on the stream (captured or eager).
+int enqueue_block(wg_handle *h, int days, hipEvent_t *evs) {
+  const int32_t n = h->n, M = h->m;
+  const dim3 cgrid((n + h->block_threads_cell - 1) / h->block_threads_cell, M);
+  const dim3 chgrid(h->n_groups, M);
+  int ev = 0;
+  for (int k = 0; k < days; k++) {
+    const int slot = k % WG_DAILY_SLOTS;
+    if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], h->stream));
+    hipLaunchKernelGGL(wg_cell_kernel, cgrid, dim3(h->block_threads_cell), 0, h->stream, h->S, h->X, h->T,
+                       h->d_forcing, h->n_forcing_days, h->d_clock, k, k > 0 ? 1 : 0, 1, k == days - 1 ? 1 : 0,
+                       slot);
+    HIP_CHECK(h, hipGetLastError());
+    if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], h->stream));
+    hipLaunchKernelGGL(wg_chain_kernel, chgrid, dim3(h->block_threads_chain), 0, h->stream, h->S, h->X, h->T,
+                       h->d_level_off, h->d_group_first, h->d_clock, k, slot);
+    HIP_CHECK(h, hipGetLastError());
+  }
+  if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], h->stream));
+  // closing pass: finish the last day so that the state is complete
+  hipLaunchKernelGGL(wg_cell_kernel, cgrid, dim3(h->block_threads_cell), 0, h->stream, h->S, h->X, h->T,
+                     h->d_forcing, h->n_forcing_days, h->d_clock, 0, 1, 0, 0, 0);
+  HIP_CHECK(h, hipGetLastError());
+  if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], h->stream));
+  return WG_OK;
+}
+
+int get_graph(wg_handle *h, int days, hipGraphExec_t *out) {
+  auto it = h->graphs.find(days);
+  if (it != h->graphs.end()) {
+    *out = it->second;
+    return WG_OK;
+  }
+  hipGraph_t graph;
+  HIP_CHECK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_block(h, days, nullptr);
+  hipError_t e = hipStreamEndCapture(h->stream, &graph);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(h, WG_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  hipGraphExec_t exec;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(h, WG_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  h->graphs[days] = exec;
+  *out = exec;
+  return WG_OK;
+}
+
+int check_step_args(wg_handle *h, int32_t forcing_day0, int32_t first_doy, int32_t n_days) {
+  if (n_days <= 0 || first_doy < 1 || first_doy > 365 || forcing_day0 < 0)
+    return fail(h, WG_E_ARG, "invalid day range");
+  if (!h->d_forcing) return fail(h, WG_E_STATE, "no forcing loaded");
+  if (int64_t(forcing_day0) + n_days > h->n_forcing_days)
+    return fail(h, WG_E_ARG, "forcing days %d..%d exceed the %lld loaded days", forcing_day0,
+                forcing_day0 + n_days - 1, (long long)h->n_forcing_days);
+  return WG_OK;
+}
+
+}  // namespace
+
+extern "C" int wg_step_days(wg_handle *h, int32_t forcing_day0, int32_t first_doy, int32_t n_days) {
+  if (!h) return WG_E_ARG;
+  int rc = check_step_args(h, forcing_day0, first_doy, n_days);
+  if (rc) return rc;
+  HIP_CHECK(h, hipSetDevice(h->device));
+  int block = 365;
+  if (const char *v = getenv("WG_GRAPH_DAYS")) block = std::max(1, std::min(3650, atoi(v)));
+  HIP_CHECK(h, hipEventRecord(h->ev0, h->stream));
+  int64_t day = forcing_day0;
+  int doy = first_doy;
+  int remaining = n_days;
+  int64_t launches = 0;
+  while (remaining > 0) {
+    const int k = std::min(block, remaining);
+    hipGraphExec_t g;
+    rc = get_graph(h, k, &g);
+    if (rc) return rc;
+    hipLaunchKernelGGL(wg_clock_set, dim3(1), dim3(64), 0, h->stream, h->d_clock, day, doy, h->started ? 1 : 0);
+    HIP_CHECK(h, hipGetLastError());
+    HIP_CHECK(h, hipGraphLaunch(g, h->stream));
+    h->started = true;
+    day += k;
+    doy = (doy - 1 + k) % 365 + 1;
+    remaining -= k;
+    launches += 2 * k + 1;
+  }
+  HIP_CHECK(h, hipEventRecord(h->ev1, h->stream));
+  HIP_CHECK(h, hipEventSynchronize(h->ev1));
+  float ms = 0;
+  HIP_CHECK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms[2] = ms;
+  h->last_launches[2] = launches;
+  h->last_daily_days = std::min(WG_DAILY_SLOTS, (n_days - 1) % block + 1);
+  return WG_OK;
+}
+
+// Eager variant with an event pair around every kernel: per-kernel durations for
+// the roofline figures (wg_last_timing).
+extern "C" int wg_step_days_profiled(wg_handle *h, int32_t forcing_day0, int32_t first_doy, int32_t n_days) {
+  if (!h) return WG_E_ARG;
+  int rc = check_step_args(h, forcing_day0, first_doy, n_days);
+  if (rc) return rc;
+  HIP_CHECK(h, hipSetDevice(h->device));
+  std::vector<hipEvent_t> evs(size_t(2) * n_days + 2);
+  for (auto &e : evs) HIP_CHECK(h, hipEventCreate(&e));
+  hipLaunchKernelGGL(wg_clock_set, dim3(1), dim3(64), 0, h->stream, h->d_clock, int64_t(forcing_day0), first_doy,
+                     h->started ? 1 : 0);
+  rc = enqueue_block(h, n_days, evs.data());
+  if (rc) return rc;
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  h->started = true;
+  double cell = 0, chain = 0;
+  for (int32_t k = 0; k < n_days; k++) {
+    float a = 0, b = 0;
+    HIP_CHECK(h, hipEventElapsedTime(&a, evs[2 * k], evs[2 * k + 1]));
+    HIP_CHECK(h, hipEventElapsedTime(&b, evs[2 * k + 1], evs[2 * k + 2]));
+    cell += a;
+    chain += b;
+  }
+  float total = 0;
+  HIP_CHECK(h, hipEventElapsedTime(&total, evs[0], evs.back()));
+  for (auto &e : evs) (void)hipEventDestroy(e);
+  h->last_ms[0] = cell;
+  h->last_ms[1] = chain;
+  h->last_ms[2] = total;
+  h->last_launches[0] = n_days;
+  h->last_launches[1] = n_days;
+  h->last_launches[2] = 2 * int64_t(n_days) + 1;
+  h->last_daily_days = std::min(n_days, WG_DAILY_SLOTS);
+  return WG_OK;
+}
+
+extern "C" int wg_get_station_discharge(wg_handle *h, double *sum, double *daily, int32_t max_days) {
+  if (!h) return WG_E_ARG;
+  HIP_CHECK(h, hipSetDevice(h->device));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  const int32_t ns = h->n_stations;
+  if (ns == 0) return WG_OK;
+  if (sum) HIP_CHECK(h, hipMemcpy(sum, h->X.st_sum, size_t(h->m) * ns * sizeof(double), hipMemcpyDeviceToHost));
+  if (daily) {
+    const int32_t nd = std::min(max_days, h->last_daily_days);
+    std::vector<double> buf(size_t(h->m) * WG_DAILY_SLOTS * ns);
+    HIP_CHECK(h, hipMemcpy(buf.data(), h->X.st_daily, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int32_t mm = 0; mm < h->m; mm++)
+      for (int32_t d = 0; d < nd; d++)
+        for (int32_t b = 0; b < ns; b++)
+          daily[(size_t(mm) * max_days + d) * ns + b] = buf[(size_t(mm) * WG_DAILY_SLOTS + d) * ns + b];
+  }
+  return WG_OK;
+}
+
+extern "C" int wg_reset_station_discharge(wg_handle *h) {
+  if (!h) return WG_E_ARG;
+  HIP_CHECK(h, hipSetDevice(h->device));
+  if (h->n_stations > 0) {
+    HIP_CHECK(h, hipMemsetAsync(h->X.st_sum, 0, size_t(h->m) * h->n_stations * sizeof(double), h->stream));
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  }
+  return WG_OK;
+}
+
+extern "C" int wg_get_day_outputs(wg_handle *h, int32_t member, wg_day_outputs *o) {
+  if (!h || !o || member < 0 || member >= h->m) return WG_E_ARG;
+  HIP_CHECK(h, hipSetDevice(h->device));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  const int32_t n = h->n;
+  const size_t off = size_t(member) * n;
+  std::vector<double> bd(n);
+  std::pair<double *, double *> pairs[] = {{o->discharge, h->X.tv},          {o->surface_runoff, h->X.d_surf},
+                                           {o->gw_recharge, h->X.d_recharge}, {o->open_water_pet, h->X.d_ow_pet},
+                                           {o->cell_aet_uncorr, h->X.d_aet},  {o->prec, h->X.d_prec}};
+  for (auto &pr : pairs) {
+    if (!pr.first) continue;
+    HIP_CHECK(h, hipMemcpy(bd.data(), pr.second + off, n * sizeof(double), hipMemcpyDeviceToHost));
+    for (int32_t p = 0; p < n; p++) pr.first[h->perm[p]] = bd[p];
+  }
+  return WG_OK;
+}
+
+extern "C" int wg_synchronize(wg_handle *h) {
+  if (!h) return WG_E_ARG;
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  return WG_OK;
+}
+
+extern "C" void *wg_stream(wg_handle *h) { return h ? (void *)h->stream : nullptr; }
+
+extern "C" int wg_last_timing(wg_handle *h, double *ms, int64_t *n_launches) {
+  if (!h) return WG_E_ARG;
+  for (int i = 0; i < 3; i++) {
+    if (ms) ms[i] = h->last_ms[i];
+    if (n_launches) n_launches[i] = h->last_launches[i];
+  }
+  return WG_OK;
+}
+
+// Schedule facts for tests and the benchmark: [0] groups, [1] max group depth,
+// [2] level slots, [3] cells in level 0 of their group.
+extern "C" int wg_schedule_info(wg_handle *h, int32_t *info) {
+  if (!h || !info) return WG_E_ARG;
+  info[0] = h->n_groups;
+  info[1] = h->max_depth;
+  info[2] = int32_t(h->level_off.size()) - 1;
+  int32_t lvl0 = 0;
+  for (int32_t g = 0; g < h->n_groups; g++) {
+    const int32_t s = h->group_first_level[g];
+    lvl0 += h->level_off[s + 1] - h->level_off[s];
+  }
+  info[3] = lvl0;
+  return WG_OK;
+}

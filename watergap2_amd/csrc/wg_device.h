@@ -1,0 +1,75 @@
+// wg_device.h -- device data layout shared by the kernels and the host API.
+//
+// Cells live in a device order p (routing group, then Kahn level, then routing
+// order), so that every level of a group is a contiguous range.  Per-member
+// arrays are [member][N]; sub-cell arrays are [member][sub-cell][N] so that the
+// 64 lanes of a wavefront (64 consecutive cells) read one sub-cell contiguously.
+#ifndef WG_DEVICE_H
+#define WG_DEVICE_H
+
+#include <stdint.h>
+
+#define WG_DEV_NSUB 100
+
+// Land-cover tables indexed by class-1 (LCT_22.DAT, LAI_22.DAT).
+struct WgTables {
+  double albedo[18], albedo_snow[18], ddf[18], emissivity[18];
+  float lai_a[18], lai_b[18];
+  int32_t initial_days[18];
+};
+
+// Static per-cell fields (device order).  Derived geometry is computed on the
+// host exactly as routingClass::init does.
+struct WgStatic {
+  int32_t n;            // cells
+  int32_t n_members;
+  const int32_t *ref_index;     // reference cell index of device cell p
+  const double *cell_area, *contfreq;
+  const int8_t *ldd, *land_cover, *texture, *contcell;
+  const int16_t *aindex, *rgmax;
+  const float *built_up, *gw_factor, *lai_max;
+  const int16_t *elevation;     // [101][N]
+  // calibration parameters used by the day loop
+  const double *alpha, *pet_mxdy, *m_prec, *m_netrad, *mcwh, *t_snowfz, *t_snowmt, *t_gradnt,
+      *degday, *pcritgwa, *evarexp, *m_rivrgh;
+  // routing statics
+  const double *loc_lake, *loc_wetland, *glo_wetland, *lake_area, *reservoir_area, *stor_cap,
+      *mean_outflow, *river_length, *river_slope, *roughness, *width_bf, *bottom_width,
+      *river_stor_max, *lake_depth, *wetl_depth, *f_glo_lake, *fswb_init;
+  const int8_t *res_type, *start_month;
+  // exp(-k) terms of the surface-water and groundwater outflow coefficients
+  const double *k_sw, *e_sw, *inv_ksw, *ome_sw, *k_gw, *e_gw, *inv_kgw, *ome_gw;
+  // schedule
+  const int32_t *up_off, *up_idx;   // CSR of upstream cells sorted by routing order
+  const int32_t *station;           // station slot of each cell, -1 = none
+  int32_t n_stations;
+};
+
+// Per-member prognostic state and day buffers ([member][N] unless noted).
+struct WgState {
+  double *canopy, *soil, *snow, *transfer, *lai_precsum;
+  int32_t *lai_days, *lai_status;
+  double *snow_elev;            // [M][100][N]
+  double *gw, *loc_lake, *loc_wetl, *glo_lake, *glo_wetl, *res, *river;
+  double *red_ll, *red_lw, *red_gl, *red_gw, *red_res, *red_river;
+  double *river_frac_next, *river_frac_change, *fswb_next, *laf, *laf_prev, *k_release;
+  double *gamma, *cfa;
+  float *smax;
+  double *tv;                   // transported volume of the day
+  // chain record written by the cell kernel for cells with upstream inflow
+  double *c_inflow, *c_lrir, *c_lgwrir, *c_gwr_ll, *c_gwr_lw, *c_ow_prec, *c_ow_pet,
+      *c_recharge_km3, *c_rv_e, *c_rv_invk, *c_rv_ome, *c_rv_term, *c_rv_evap, *c_rv_prec;
+  // diagnostics of the last day
+  double *d_surf, *d_recharge, *d_ow_pet, *d_aet, *d_prec;
+  // stations
+  double *st_sum;               // [M][n_stations]
+  double *st_daily;             // [M][WG_DAILY_SLOTS][n_stations]
+  // forcing perturbation
+  const uint64_t *member_seed;
+  double prec_sigma, temp_sigma;
+  int32_t perturb;
+};
+
+#define WG_DAILY_SLOTS 366
+
+#endif

@@ -1,0 +1,1060 @@
+// wg_kernels.hip -- HIP kernels for gfx950: the daily vertical water balance and
+// the lateral routing of WaterGAP 2 (parity option profile).
+//
+// Two kernels per simulated day:
+//   wg_cell_kernel   one lane per (member, cell).  It finishes the previous day
+//                    (reduction factors, river width, surface-water and land area
+//                    fractions: routing.cpp:2478-2488, 2792-2802, 3071-3078,
+//                    3286-3295, 3546-3586, 5034-5188, 5343-5352), runs calcNewDay
+//                    (daily.cpp:94-1279, lai.cpp:152-306) and the part of the
+//                    routing loop that reads no upstream water (routing.cpp:1877-2617),
+//                    and for cells without upstream cells the whole rest of the
+//                    routing loop as well.
+//   wg_chain_kernel  one workgroup per (routing group, member).  It walks the Kahn
+//                    levels of the group with workgroup barriers; each cell gathers
+//                    the transported volume of its upstream cells in routing order
+//                    (the order of the serial += at routing.cpp:3955-3958) and runs
+//                    routing.cpp:2623-3958 (global lake, reservoir, global wetland,
+//                    arid groundwater, river).
+// Arithmetic follows the reference expression by expression (operand order and
+// association), compiled with -ffp-contract=off, in FP64 with the reference's
+// float/short inputs promoted exactly where the reference promotes them.
+#include <hip/hip_runtime.h>
+
+#include "wg_device.h"
+
+namespace {
+
+constexpr double kCanopyEvapoExp = 0.66666666;  // daily.cpp:1867
+constexpr double kRunoffFracBuiltUp = 0.5;      // daily.cpp:1868
+constexpr double kOpenWaterAlbedo = 0.08;       // daily.cpp:1870
+constexpr double kLakeOutflowExp = 1.5;         // routing.cpp:125
+constexpr double kWetlOutflowExp = 2.5;         // routing.cpp:126
+constexpr double kEvapoRedExp = 3.32193;        // routing.cpp:127
+constexpr double kEvapoRedExpRes = 2.81383;     // routing.cpp:128
+constexpr double kKswbgw = 10.;                 // routing.cpp:1662
+constexpr double kMinStorVol = 1.e-15;          // routing.h:24
+
+__constant__ short kFirstDayInMonth[12] = {1, 32, 60, 91, 121, 152, 182, 213, 244, 274, 305, 335};
+__constant__ short kDaysInMonth[12] = {31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31};
+
+__device__ inline int month_of_doy(int doy) {
+  int m = 0;
+#pragma unroll
+  for (int k = 1; k < 12; k++) m += (doy >= kFirstDayInMonth[k]) ? 1 : 0;
+  return m;
+}
+
+// ---- ensemble perturbation: Philox4x32-10 + Irwin-Hall + basic-op exp ----
+__device__ inline void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+    const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+__device__ double irwin_hall(uint64_t seed, uint32_t cell, uint32_t day, uint32_t stream) {
+  double s = 0.;
+  for (uint32_t blk = 0; blk < 3; blk++) {
+    uint32_t c[4] = {cell, day, stream, blk};
+    philox(c, uint32_t(seed), uint32_t(seed >> 32));
+    for (int k = 0; k < 4; k++) s += (double(c[k]) + 0.5) * 2.3283064365386963e-10;
+  }
+  return s - 6.;
+}
+
+__device__ double det_exp(double x) {
+  const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+  const double t = x * 1.44269504088896338700e+00;
+  const double k = floor(t + 0.5);
+  const double r = (x - k * ln2_hi) - k * ln2_lo;
+  const double inv_fact[13] = {1.0 / 479001600.0, 1.0 / 39916800.0, 1.0 / 3628800.0, 1.0 / 362880.0,
+                               1.0 / 40320.0, 1.0 / 5040.0, 1.0 / 720.0, 1.0 / 120.0, 1.0 / 24.0,
+                               1.0 / 6.0, 0.5, 1.0, 1.0};
+  double p = 1.0 / 6227020800.0;
+  for (int j = 0; j < 13; j++) p = p * r + inv_fact[j];
+  return ldexp(p, int(k));
+}
+
+// ---- the upstream-dependent part of one routing cell ----
+
+struct ChainIn {
+  double inflow;        // outflow of the local lake/wetland system (routing.cpp:2616)
+  double lrir, lgwrir;  // G_localRunoffIntoRiver, G_localGWRunoffIntoRiver (humid cells)
+  double gwr_ll, gwr_lw;
+  double ow_prec, ow_pet;
+  double recharge_km3;  // arid cells: recharge term of netGWin (routing.cpp:3320-3321)
+  double rv_e, rv_invk, rv_ome, rv_term, rv_evap, rv_prec;
+};
+
+// routing.cpp:2623-3958 for one cell; returns transportedVolume.
+__device__ double chain_step(const WgStatic &S, const WgState &X, const WgTables &T, int64_t mp, int p,
+                             const ChainIn &in, double inflow_up, int day, int month) {
+  const double cellArea = S.cell_area[p];
+  const double cf = S.contfreq[p];
+  const double cfa = X.cfa[mp];
+  const int ldd = S.ldd[p];
+  const bool arid_swb = (S.aindex[p] == 1) && (ldd >= 0);
+  const double ow_prec = in.ow_prec, ow_pet = in.ow_pet;
+  double inflow = in.inflow;
+  double outflow = 0., maxStorage, totalInflow;
+  double gwr_glolak = 0., gwr_glowet = 0., gwr_res = 0.;
+
+  inflow += inflow_up;  // routing.cpp:2623
+
+  // global lake, routing.cpp:2630-2804
+  const double lake_area = S.lake_area[p];
+  if (lake_area > 0.) {
+    const double prev = X.glo_lake[mp], rf = X.red_gl[mp];
+    const double k_sw = S.k_sw[p];
+    double use, useMax, evap, st;
+    maxStorage = lake_area * S.lake_depth[p];
+    evap = ((1.0 - cfa) * ow_prec) + (cfa * ow_pet * rf);
+    if (evap < 0.) evap = 0.;
+    totalInflow = inflow + (ow_prec * (lake_area / 1000000.));
+    if (arid_swb)
+      gwr_glolak = kKswbgw * rf * (lake_area / (cellArea * (cf / 100.)));
+    else
+      gwr_glolak = 0.;
+    use = evap * (lake_area / 1000000.) + gwr_glolak * cellArea * (cf / 100.) / 1000000. + 0.;
+    useMax = totalInflow + maxStorage + prev;
+    if (use > useMax) {
+      st = (-1.) * maxStorage;
+      outflow = 0.;
+      gwr_glolak *= useMax / use;
+    } else {
+      st = prev * S.e_sw[p] + S.inv_ksw[p] * (totalInflow - use) * S.ome_sw[p];
+      outflow = totalInflow + prev - st - use;
+      if (st > maxStorage) {
+        outflow += (st - maxStorage);
+        st = maxStorage;
+      }
+      if (outflow < 0.) {
+        outflow = 0.;
+        st = prev + totalInflow - use;
+      }
+    }
+    if (fabs(st) <= kMinStorVol) st = 0.;
+    X.glo_lake[mp] = st;
+    inflow = outflow;
+    (void)k_sw;
+  }
+
+  // reservoir, routing.cpp:2807-3082
+  const double res_area = S.reservoir_area[p];
+  if (res_area > 0.) {
+    const double prev = X.res[mp], rf = X.red_res[mp];
+    const double cap = S.stor_cap[p], qmean = S.mean_outflow[p];
+    double evap, PETgwr, PETgwrMax, st, prov_rel = 0., release;
+    const double c_ratio = cap / (qmean * 31536000. / 1000000000.);
+    maxStorage = cap;
+    evap = ((1.0 - cfa) * ow_prec) + (cfa * (ow_pet * rf));
+    if (evap < 0.) evap = 0.;
+    totalInflow = inflow + (ow_prec * (res_area / 1000000.));
+    if (arid_swb)
+      gwr_res = kKswbgw * rf * (res_area / (cellArea * (cf / 100.)));
+    else
+      gwr_res = 0.;
+    PETgwr = evap * (res_area / 1000000.) + gwr_res * cellArea * (cf / 100.) / 1000000.;
+    PETgwrMax = prev + totalInflow;
+    if (PETgwr > PETgwrMax) {
+      st = prev + totalInflow - PETgwrMax;
+      gwr_res *= PETgwrMax / PETgwr;
+    } else {
+      st = prev + totalInflow - PETgwr;
+    }
+    if (fabs(st) <= kMinStorVol) st = 0.;
+    double krel = X.k_release[mp];
+    if (month == S.start_month[p] - 1) {
+      if (day == kFirstDayInMonth[month]) {
+        if (st < (cap * 0.1))
+          krel = 0.1;
+        else
+          krel = st / (maxStorage * 0.85);
+        X.k_release[mp] = krel;
+      }
+    }
+    const int rtype = S.res_type[p];
+    if (rtype == 1) {
+      // irrigation reservoir without water use: zero demand and zero downstream use
+      const double mean_demand = 0.;
+      double monthlyUse = 0.0 * kDaysInMonth[month];
+      monthlyUse = monthlyUse * 1000000000. / (kDaysInMonth[month] * 86400.);
+      if (mean_demand >= 0.5 * qmean)
+        prov_rel = qmean / 2. * (1. + monthlyUse / mean_demand);
+      else
+        prov_rel = qmean + monthlyUse - mean_demand;
+    } else if (rtype == 2) {
+      prov_rel = qmean;
+    }
+    if (c_ratio >= 0.5)
+      release = krel * prov_rel;
+    else
+      release = ((4. * c_ratio * c_ratio) * krel * prov_rel) +
+                ((1.0 - ((4. * c_ratio * c_ratio))) * inflow * 1000000000. / (24. * 3600.));
+    if (st >= (cap * 0.1))
+      outflow = release * (24. * 3600.) / 1000000000.;
+    else
+      outflow = 0.1 * release * (24. * 3600.) / 1000000000.;
+    if (outflow < 0.) outflow = 0.;
+    st -= outflow;
+    if (st > maxStorage) {
+      outflow += (st - maxStorage);
+      st = maxStorage;
+    }
+    if (st < 0.) {
+      outflow += st;
+      st = 0.;
+    }
+    X.res[mp] = st;
+    inflow = outflow;
+  }
+
+  // global wetland, routing.cpp:3178-3297
+  const double glo_wetland = S.glo_wetland[p];
+  if (glo_wetland > 0) {
+    const double prev = X.glo_wetl[mp], rf = X.red_gw[mp];
+    double evap, PETgwr, PETgwrMax, st;
+    maxStorage = ((glo_wetland) / 100.) * cellArea * S.wetl_depth[p];
+    evap = ((1.0 - cfa) * (ow_prec * rf)) + (cfa * (ow_pet * rf));
+    if (evap < 0.) evap = 0.;
+    totalInflow = inflow + (ow_prec * rf * (cellArea / 1000000.) * (glo_wetland / 100.));
+    if (arid_swb)
+      gwr_glowet = kKswbgw * rf * glo_wetland / 100. / (cf / 100.);
+    else
+      gwr_glowet = 0.;
+    PETgwr = evap * (cellArea / 1000000.) * ((glo_wetland) / 100.) + gwr_glowet * cellArea * (cf / 100.) / 1000000.;
+    PETgwrMax = totalInflow + prev;
+    if (PETgwr > PETgwrMax) {
+      st = 0.;
+      outflow = 0.;
+      gwr_glowet *= PETgwrMax / PETgwr;
+    } else {
+      st = prev * S.e_sw[p] + S.inv_ksw[p] * (totalInflow - PETgwr) * S.ome_sw[p];
+      outflow = totalInflow + prev - st - PETgwr;
+    }
+    if (st > maxStorage) {
+      outflow += (st - maxStorage);
+      st = maxStorage;
+    }
+    if (fabs(st) <= kMinStorVol) st = 0.;
+    X.glo_wetl[mp] = st;
+    inflow = outflow;
+  }
+
+  // groundwater of (semi-)arid cells, routing.cpp:3305-3386
+  double lgwrir = in.lgwrir;
+  if (arid_swb) {
+    const double gwr_swb = in.gwr_ll + gwr_glolak + in.gwr_lw + gwr_glowet + gwr_res;
+    const double gw_prev = X.gw[mp];
+    const double netGWin = gwr_swb * cellArea * (cf / 100.) / 1000000. + in.recharge_km3;
+    double gw = gw_prev * S.e_gw[p] + S.inv_kgw[p] * netGWin * S.ome_gw[p];
+    if (fabs(gw) <= kMinStorVol) gw = 0.;
+    double gw_runoff = gw_prev - gw + netGWin;
+    if (gw_runoff <= 0.) {
+      gw_runoff = 0.;
+      gw = gw_prev + netGWin;
+      if (fabs(gw) <= kMinStorVol) gw = 0.;
+    }
+    X.gw[mp] = gw;
+    lgwrir = gw_runoff;
+  }
+
+  // river, routing.cpp:3402-3514
+  double riverInflow = inflow;
+  if (ldd >= 0) {
+    riverInflow += in.lrir;
+    riverInflow += lgwrir;
+  }
+  riverInflow += in.rv_prec;
+  const double prev = X.river[mp];
+  const double evapRemUse = 0. + in.rv_evap;
+  const double evapRemUseMax = riverInflow + in.rv_term;
+  double tv, st;
+  if (evapRemUse > evapRemUseMax) {
+    st = 0.;
+    tv = riverInflow + prev - evapRemUseMax;
+    if (tv < 0.) tv = 0.;
+  } else {
+    st = prev * in.rv_e + in.rv_invk * (riverInflow - evapRemUse) * in.rv_ome;
+    if (fabs(st) <= kMinStorVol) st = 0.;
+    tv = riverInflow + prev - st - evapRemUse;
+    if (tv < 0.) tv = 0.;
+  }
+  X.river[mp] = st;
+  (void)T;
+  return tv;
+}
+
+// routing.cpp:7309-7341 getNewRiverVelocity
+__device__ inline double new_river_velocity(double slope, double bottom, double rough, double stor,
+                                            double length, double mult) {
+  const double csa = (stor / length) * 1000000.;
+  const double depth = -bottom / 4.0 + sqrt(bottom * bottom / 16.0 + 0.5 * csa);
+  const double wp = bottom + 2.0 * depth * sqrt(5.0);
+  const double hr = csa / wp;
+  double v = 1. / (mult * rough) * pow(hr, (2. / 3.)) * pow(slope, 0.5);
+  v = v * 86.4;
+  return (v < 0.00001) ? 0.00001 : v;
+}
+
+// LAI state machine, lai.cpp:179-293
+__device__ inline double lai_day(int &days, int init_days, int &status, int lct, bool arid, double lmin,
+                                 double lmax, double &precsum, double prec, bool growing) {
+  if (growing) {
+    if (status == 0) {
+      if (days >= init_days) {
+        days++;
+        precsum += prec;
+        if (precsum > 40.) {
+          if (days >= init_days + 30) {
+            days = init_days + 30;
+            status = 1;
+          }
+          return (lmin + (lmax - lmin) * (days - init_days) / 30.);
+        }
+        days = init_days;
+        return lmin;
+      }
+      days++;
+      precsum += prec;
+      return lmin;
+    }
+    if (days <= 30) {
+      days--;
+      if (lct <= 2) status = 0;
+      if (days <= 0) {
+        days = 0;
+        status = 0;
+        precsum = 0.;
+      }
+      return (lmax - (lmax - lmin) * (30 - days) / 30.);
+    }
+    if (arid && (prec < 0.5))
+      days--;
+    else
+      days = 30 + init_days;
+    return lmax;
+  }
+  if (status == 0) {
+    if (days > init_days) {
+      days++;
+      precsum += prec;
+      if (precsum > 40.) {
+        if (days >= init_days + 30) {
+          days = init_days + 30;
+          status = 1;
+        }
+        return (lmin + (lmax - lmin) * (days - init_days) / 30.);
+      }
+      days = init_days;
+      return lmin;
+    }
+    precsum += prec;
+    return lmin;
+  }
+  if (days <= 30) {
+    days--;
+    if (days <= 0) {
+      days = 0;
+      status = 0;
+      precsum = 0.;
+    }
+    return (lmax - (lmax - lmin) * (30 - days) / 30.);
+  }
+  days--;
+  return lmax;
+}
+
+// Finish the previous day for one cell: reduction factors of the global
+// storages, river width and area fraction (routing.cpp:2792-2802, 3071-3078,
+// 3286-3295, 3546-3586), then the surface-water and land area fractions
+// (routing.cpp:5046-5174) and updateLandAreaFrac (routing.cpp:5343-5352).
+__device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p) {
+  const double evarexp = S.evarexp[p];
+  const double cellArea = S.cell_area[p];
+  double red_ll = X.red_ll[mp], red_lw = X.red_lw[mp], red_gw = X.red_gw[mp];
+  if (S.lake_area[p] > 0.) {
+    const double maxStorage = S.lake_area[p] * S.lake_depth[p];
+    double r = 1. - pow(fabs(X.glo_lake[mp] - maxStorage) / (2. * maxStorage), (evarexp * kEvapoRedExp));
+    if (r < 0.) r = 0.;
+    if (r > 1.) r = 1.;
+    X.red_gl[mp] = r;
+  }
+  if (S.reservoir_area[p] > 0.) {
+    const double maxStorage = S.stor_cap[p];
+    double r = 1. - pow(fabs(X.res[mp] - maxStorage) / maxStorage, kEvapoRedExpRes);
+    if (r < 0.) r = 0.;
+    if (r > 1.) r = 1.;
+    X.red_res[mp] = r;
+  }
+  const double glo_wetland = S.glo_wetland[p];
+  if (glo_wetland > 0) {
+    const double maxStorage = ((glo_wetland) / 100.) * cellArea * S.wetl_depth[p];
+    double r = 1. - pow(fabs(X.glo_wetl[mp] - maxStorage) / maxStorage, (evarexp * kEvapoRedExp));
+    if (r < 0.) r = 0.;
+    if (r > 1.) r = 1.;
+    red_gw = r;
+  }
+  double river_frac_next = X.river_frac_next[mp];
+  double river_frac_change = X.river_frac_change[mp];
+  double red_river = X.red_river[mp];
+  if (S.contcell[p]) {
+    const double L = S.river_length[p], B = S.bottom_width[p];
+    const double st = X.river[mp];
+    const double riverAreaFrac = river_frac_next;
+    const double csa = st / L;
+    const double depth = -B / (4. * 1000.) + sqrt(B / 1000. * B / (16. * 1000.) + 0.5 * csa);
+    double width = B / 1000. + 4. * depth;
+    if (width > S.width_bf[p] / 1000.) width = S.width_bf[p] / 1000.;
+    const double smax = S.river_stor_max[p];
+    red_river = 1. - pow(fabs(st - smax) / smax, (evarexp * kEvapoRedExp));
+    if (red_river < 0.) red_river = 0.;
+    if (red_river > 1.) red_river = 1.;
+    river_frac_next = red_river * L * width * 100. / cellArea;
+    river_frac_change = river_frac_next - riverAreaFrac;
+  }
+  // fractions for the next day
+  const double loc_lake = S.loc_lake[p], loc_wetland = S.loc_wetland[p];
+  double fLocLake, fLocWet, fGloWet;
+  if ((loc_lake > 0.) && (red_ll > 0.))
+    fLocLake = (red_ll * loc_lake / 100.);
+  else
+    fLocLake = 0.;
+  if ((loc_wetland > 0.) && (red_lw > 0.))
+    fLocWet = (red_lw * loc_wetland / 100.);
+  else
+    fLocWet = 0.;
+  if ((glo_wetland > 0.) && (red_gw > 0.))
+    fGloWet = (red_gw * glo_wetland / 100.);
+  else
+    fGloWet = 0.;
+  const double fswb_old = X.fswb_next[mp];
+  double fswb_next = fLocLake + fLocWet + fGloWet;
+  const double f_glo_lake = S.f_glo_lake[p];
+  const double max_raf = S.contfreq[p] / 100. - f_glo_lake;
+  if (((S.lake_area[p] > 0.) || (S.reservoir_area[p] > 0.)) && (f_glo_lake == 1.)) {
+    river_frac_next = 0.;
+    river_frac_change = 0.;
+    red_river = 0.;
+  } else {
+    if (river_frac_next <= max_raf) {
+      if (fswb_next > (max_raf - river_frac_next)) {
+        const double corr = (max_raf - river_frac_next) / fswb_next;
+        red_ll *= corr;
+        red_lw *= corr;
+        red_gw *= corr;
+        if ((fLocLake > 0.) && (red_ll > 0.)) {
+          fLocLake = (red_ll * loc_lake / 100.);
+        } else {
+          red_ll = 0.;
+          fLocLake = 0.;
+        }
+        if ((fLocWet > 0.) && (red_lw > 0.)) {
+          fLocWet = (red_lw * loc_wetland / 100.);
+        } else {
+          red_lw = 0.;
+          fLocWet = 0.;
+        }
+        if ((fGloWet > 0.) && (red_gw > 0.)) {
+          fGloWet = (red_gw * glo_wetland / 100.);
+        } else {
+          red_gw = 0.;
+          fGloWet = 0.;
+        }
+      }
+    } else {
+      const double deficit = river_frac_next - max_raf;
+      river_frac_change -= deficit;
+      red_river *= max_raf / river_frac_next;
+      river_frac_next = max_raf;
+      fLocLake = 0.;
+      fLocWet = 0.;
+      fGloWet = 0.;
+    }
+  }
+  fswb_next = fLocLake + fLocWet + fGloWet;
+  const double change_pct = fswb_next * 100. - fswb_old * 100.;
+  const double laf = X.laf[mp];
+  double laf_next = laf - (change_pct + (river_frac_change * 100.));
+  if (laf_next < 0.) laf_next = 0.;
+  X.laf_prev[mp] = laf;
+  X.laf[mp] = laf_next;
+  X.fswb_next[mp] = fswb_next;
+  X.red_ll[mp] = red_ll;
+  X.red_lw[mp] = red_lw;
+  X.red_gw[mp] = red_gw;
+  X.red_river[mp] = red_river;
+  X.river_frac_next[mp] = river_frac_next;
+  X.river_frac_change[mp] = river_frac_change;
+}
+
+}  // namespace
+
+struct WgClock {
+  int64_t day0;   // forcing day (index into the loaded forcing) of launch offset 0
+  int32_t doy0;   // julian day of launch offset 0
+  int32_t started;
+};
+
+// One lane per (member, cell).  Grid: x over cells, y over members.
+__global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgTables T,
+                                                      const float *__restrict__ forcing, int64_t n_forcing_days,
+                                                      const WgClock *__restrict__ clock, int k,
+                                                      int do_post, int do_vertical, int write_diag,
+                                                      int daily_slot) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = blockIdx.y;
+  const int n = S.n;
+  if (p >= n) return;
+  const int64_t mp = int64_t(m) * n + p;
+
+  if (do_post) post_day(S, X, mp, p);
+  // calcNewDay and routing run for continental cells only (integrateWGHM.cpp:770,
+  // routing.cpp:1876); other cells keep a zero transported volume.
+  if (!do_vertical || !S.contcell[p]) return;
+
+  const int64_t fday = clock->day0 + k;
+  const int doy = int((clock->doy0 - 1 + k) % 365) + 1;
+  const int month = month_of_doy(doy);
+  const bool started = (clock->started != 0) || (k > 0) || (do_post != 0);
+  if (fday < 0 || fday >= n_forcing_days) return;
+  const float *f = forcing + fday * 4 * int64_t(n);
+  float tf = f[p], pf = f[n + p];
+  const float swf = f[2 * n + p], lwf = f[3 * n + p];
+  if (X.perturb) {
+    const uint64_t seed = X.member_seed[m];
+    const uint32_t ref = uint32_t(S.ref_index[p]);
+    const double zp = irwin_hall(seed, ref, uint32_t(fday), 0x50524543u);
+    const double zt = irwin_hall(seed, ref, uint32_t(fday), 0x54454d50u);
+    pf = float(double(pf) * det_exp(X.prec_sigma * zp));
+    tf = float(double(tf) + X.temp_sigma * zt);
+  }
+
+  // ------------------------------------------------------------------
+  // calcNewDay, daily.cpp:94-1279
+  // ------------------------------------------------------------------
+  const int lct = S.land_cover[p];
+  const bool arid_gw = (S.aindex[p] == 1);
+  const double landAreaFrac = X.laf[mp];
+  const double lafPrev = started ? X.laf_prev[mp] : landAreaFrac;  // daily.cpp:159-169
+  double dailyTempC = tf, dailyPrec = pf;
+  const double dailyShortWave = swf, dailyLongWave = lwf;
+  dailyPrec = S.m_prec[p] * dailyPrec;  // daily.cpp:248
+  const double temp2 = dailyTempC + 237.3;
+  const double e_s = 0.6108 * exp(17.27 * dailyTempC / temp2);
+  const double alpha = S.alpha[p];
+  const double maxDailyPET = S.pet_mxdy[p];
+
+  // lai.cpp:152-177
+  double dailyLai;
+  {
+    const float lmax_f = S.lai_max[p];
+    const double lmax = double(lmax_f);
+    const double lmin = T.lai_a[lct - 1] + T.lai_b[lct - 1] * lmax_f;
+    int days = X.lai_days[mp], status = X.lai_status[mp];
+    double precsum = X.lai_precsum[mp];
+    dailyLai = lai_day(days, T.initial_days[lct - 1], status, lct, arid_gw, lmin, lmax, precsum, dailyPrec,
+                       dailyTempC > 8.);
+    X.lai_days[mp] = days;
+    X.lai_status[mp] = status;
+    X.lai_precsum[mp] = precsum;
+  }
+  // daily.cpp:366-462 albedo, latent heat, radiation (cloud = 1)
+  const double albedo = (X.snow[mp] > 3.) ? T.albedo_snow[lct - 1] : T.albedo[lct - 1];
+  const double lat_heat = (dailyTempC > 0) ? (2.501 - 0.002361 * dailyTempC) : 2.835;
+  const double conv_Wm2_to_mmd = 0.0864 / lat_heat;
+  const double solar_rad = conv_Wm2_to_mmd * dailyShortWave;
+  const double long_wave_rad_in = conv_Wm2_to_mmd * dailyLongWave;
+  const double temp_K = dailyTempC + 273.2;
+  const double long_wave_rad_out = T.emissivity[lct - 1] * 0.000000004903 * pow(temp_K, 4.) / lat_heat;
+  const double net_long_wave_rad = long_wave_rad_in - long_wave_rad_out;
+  const double net_short_wave_rad = solar_rad * (1. - albedo);
+  const double net_rad = S.m_netrad[p] * (net_short_wave_rad + net_long_wave_rad);
+  const double openWaterNetRad = solar_rad * (1. - kOpenWaterAlbedo) + net_long_wave_rad;
+  // daily.cpp:476-522 Priestley-Taylor
+  const double inc_svp = 4098. * e_s / (temp2 * temp2);
+  const double gamma = (0.0016286 * 101.3) / lat_heat;
+  const double dailyPET = (net_rad <= 0.) ? 0. : alpha * (inc_svp * net_rad) / (inc_svp + gamma);
+  const double dailyOpenWaterPET =
+      (openWaterNetRad <= 0.) ? 0. : alpha * (inc_svp * openWaterNetRad) / (inc_svp + gamma);
+
+  double landStorageChangeSum = 0., initialStorage = 0.;
+  double dailyCanopyEvapo = 0., daily_prec_to_soil = 0., dailySoilPET = 0.;
+  double dailySnowEvapo = 0., dailyEffPrec = 0., immediate_runoff = 0., dailyAET = 0.;
+  double daily_runoff = 0., total_daily_runoff = 0., daily_gw_recharge = 0., pot_gw_recharge = 0.;
+  double soil_water_overflow = 0., neg_land_aet = 0.;
+  double dailyCellLandAET_uncorr = 0.;
+  double transfer = X.transfer[mp];
+
+  // interception, daily.cpp:825-895
+  double canopy = X.canopy[mp];
+  if (landAreaFrac <= 0.) {
+    transfer = canopy;
+    canopy = 0.;
+  } else {
+    canopy *= lafPrev / landAreaFrac;
+    if (fabs(canopy) <= kMinStorVol) canopy = 0.;
+    initialStorage = canopy;
+    if (dailyLai > 0.00001) {
+      const double max_canopy_storage = S.mcwh[p] * dailyLai;
+      const double canopy_deficiency = max_canopy_storage - canopy;
+      if (dailyPrec < canopy_deficiency) {
+        canopy += dailyPrec;
+        daily_prec_to_soil = 0.;
+      } else {
+        canopy = max_canopy_storage;
+        daily_prec_to_soil = dailyPrec - canopy_deficiency;
+      }
+      const double canopy_water_content = canopy;
+      dailyCanopyEvapo = dailyPET * pow((canopy_water_content / max_canopy_storage), kCanopyEvapoExp);
+      if (dailyCanopyEvapo > canopy_water_content) {
+        dailyCanopyEvapo = canopy_water_content;
+        dailySoilPET = dailyPET - canopy_water_content;
+        canopy = 0.0;
+      } else {
+        canopy -= dailyCanopyEvapo;
+        dailySoilPET = dailyPET - dailyCanopyEvapo;
+      }
+    } else {
+      daily_prec_to_soil = dailyPrec;
+      dailySoilPET = dailyPET;
+      dailyCanopyEvapo = 0.0;
+    }
+    landStorageChangeSum += canopy - initialStorage;
+  }
+  X.canopy[mp] = canopy;
+  if (dailySoilPET < 0.) dailySoilPET = 0.0;
+
+  // snow on the elevation sub-cells, daily.cpp:897-1062, serial in sub-cell order
+  const double T_SNOWFZ = S.t_snowfz[p], T_SNOWMT = S.t_snowmt[p];
+  const double T_GRADNT = S.t_gradnt[p], DEGDAY_F = S.degday[p];
+  const double ddf = T.ddf[lct - 1];
+  double TempElevMax = 0., snowStorageChange = 0., snow_sum = 0.;
+  int thresh_elev = 0;
+  const int elev0 = S.elevation[p];
+  double *se = X.snow_elev + int64_t(m) * WG_DEV_NSUB * n + p;
+  const int16_t *ev = S.elevation + n + p;
+  if (landAreaFrac <= 0.) {
+    for (int e = 0; e < WG_DEV_NSUB; e++) {
+      transfer += se[int64_t(e) * n] / 100.;
+      se[int64_t(e) * n] = 0.;
+    }
+    snow_sum = 0.;
+  } else {
+    const double scale_num = lafPrev;
+#pragma unroll 4
+    for (int e = 0; e < WG_DEV_NSUB; e++) {
+      const int elev_e = ev[int64_t(e) * n];
+      double s = se[int64_t(e) * n];
+      double temp_elev = dailyTempC - ((elev_e - elev0) * T_GRADNT);
+      double daily_snow_to_soil_elev = 0., snowmelt_elev = 0., eff_before = 0.;
+      s = s * scale_num / landAreaFrac;
+      if (fabs(s) <= kMinStorVol) s = 0.;
+      const double init_s = s;
+      if (s > 1000.) {
+        if (thresh_elev == 0.) {
+          thresh_elev = elev_e;
+        } else if (thresh_elev > 0.) {
+          temp_elev = dailyTempC - ((thresh_elev - elev0) * T_GRADNT);
+        }
+      }
+      if (temp_elev <= T_SNOWFZ) {
+        daily_snow_to_soil_elev = daily_prec_to_soil;
+        s += daily_snow_to_soil_elev;
+        if (s > dailySoilPET) {
+          s -= dailySoilPET;
+          dailySnowEvapo += dailySoilPET;
+        } else {
+          dailySnowEvapo += s;
+          s = 0.;
+        }
+      } else {
+        eff_before = daily_prec_to_soil;
+      }
+      if (temp_elev > T_SNOWMT) {
+        if (!(s < 0.)) {
+          snowmelt_elev = DEGDAY_F * ddf * (temp_elev - T_SNOWMT);
+          if (snowmelt_elev > s) {
+            snowmelt_elev = s;
+            s = 0.;
+          } else {
+            s -= snowmelt_elev;
+          }
+        }
+      }
+      const double eff_elev = eff_before + snowmelt_elev;
+      snowStorageChange += s - init_s;
+      if (e == 0) TempElevMax = temp_elev;
+      snow_sum += s;
+      dailyEffPrec += eff_elev;
+      se[int64_t(e) * n] = s;
+      (void)daily_snow_to_soil_elev;
+    }
+    snow_sum /= 100.;
+    dailyEffPrec /= 100.;
+    dailySnowEvapo /= 100.;
+    snowStorageChange /= 100.;
+    landStorageChangeSum += snowStorageChange;
+  }
+  X.snow[mp] = snow_sum;
+
+  // immediate runoff, daily.cpp:1066-1071
+  const float built_up = S.built_up[p];
+  if (built_up > 0.) {
+    immediate_runoff = kRunoffFracBuiltUp * dailyEffPrec * built_up;
+    dailyEffPrec -= immediate_runoff;
+  }
+
+  // soil, runoff, AET, recharge, daily.cpp:1080-1239
+  const double cfa = X.cfa[mp];
+  const float smax = X.smax[mp];
+  double soil = X.soil[mp];
+  if (landAreaFrac <= 0.) {
+    transfer += soil;
+    transfer *= cfa;
+    soil = 0.;
+    daily_gw_recharge = 0.;
+    total_daily_runoff = 0.;
+  } else {
+    soil *= lafPrev / landAreaFrac;
+    initialStorage = soil;
+    soil_water_overflow = 0;
+    if (soil > smax) {
+      soil_water_overflow = soil - smax;
+      soil = smax;
+    }
+    if (TempElevMax > T_SNOWFZ) {
+      if (smax > 0.) {
+        const double soil_saturation = soil / smax;
+        daily_runoff = dailyEffPrec * pow(soil_saturation, X.gamma[mp]);
+        if (dailySoilPET > (maxDailyPET - dailyCanopyEvapo) * soil_saturation)
+          dailyAET = (maxDailyPET - dailyCanopyEvapo) * soil_saturation;
+        else
+          dailyAET = dailySoilPET;
+        soil += dailyEffPrec - dailyAET - daily_runoff;
+        if (fabs(soil) <= kMinStorVol) soil = 0.;
+        dailyEffPrec = 0.;
+        if (soil < 0.) {
+          dailyAET += soil;
+          soil = 0.;
+        }
+        daily_runoff *= cfa;
+        immediate_runoff *= cfa;
+        const double rgmax = S.rgmax[p] / 100.;
+        const float fg = S.gw_factor[p];
+        if (rgmax < (fg * daily_runoff))
+          daily_gw_recharge = rgmax;
+        else
+          daily_gw_recharge = fg * daily_runoff;
+        pot_gw_recharge = 0.;
+        if (((arid_gw) && (S.texture[p] < 21)) && (S.ldd[p] >= 0)) {
+          if (dailyPrec <= S.pcritgwa[p]) {
+            pot_gw_recharge = daily_gw_recharge;
+            daily_gw_recharge = 0.;
+          }
+        }
+        daily_runoff -= pot_gw_recharge;
+        pot_gw_recharge /= cfa;
+        soil += pot_gw_recharge;
+        if (soil > smax) {
+          soil_water_overflow += soil - smax;
+          soil = smax;
+        }
+        soil_water_overflow *= cfa;
+        total_daily_runoff = daily_runoff + immediate_runoff + soil_water_overflow;
+      } else {
+        total_daily_runoff = 0.;
+        daily_gw_recharge = 0.;
+      }
+    } else {
+      soil_water_overflow *= cfa;
+      dailyEffPrec *= cfa;
+      total_daily_runoff += soil_water_overflow + dailyEffPrec;
+      daily_gw_recharge = 0.;
+      dailyAET = 0.;
+    }
+    landStorageChangeSum += soil - initialStorage;
+    double cell_aet = landStorageChangeSum * (cfa - 1.0) - dailyPrec * (cfa - 1.0) +
+                      (dailyAET + dailyCanopyEvapo + dailySnowEvapo) * cfa;
+    if (cell_aet < 0.) neg_land_aet = cell_aet;
+    dailyCellLandAET_uncorr = (dailyAET + dailyCanopyEvapo + dailySnowEvapo);
+  }
+  // G_dailyGwRecharge is handed to routing before the corrections below (daily.cpp:1214)
+  const double recharge_handoff = daily_gw_recharge;
+  // surface runoff, daily.cpp:1241-1257
+  if (neg_land_aet < 0.) total_daily_runoff = total_daily_runoff + neg_land_aet;
+  if (total_daily_runoff < 0.) total_daily_runoff = 0.;
+  if ((total_daily_runoff - daily_gw_recharge) < 0.) {
+    const double neg_runoff = total_daily_runoff - daily_gw_recharge;
+    daily_gw_recharge = total_daily_runoff;
+    soil += neg_runoff;
+  }
+  const double surf_runoff = total_daily_runoff - daily_gw_recharge;
+  X.soil[mp] = soil;
+  X.transfer[mp] = transfer;
+  if (write_diag) {
+    X.d_surf[mp] = surf_runoff;
+    X.d_recharge[mp] = recharge_handoff;
+    X.d_ow_pet[mp] = dailyOpenWaterPET;
+    X.d_aet[mp] = dailyCellLandAET_uncorr;
+    X.d_prec[mp] = dailyPrec;
+  }
+
+  // ------------------------------------------------------------------
+  // routing without upstream water, routing.cpp:1877-2617
+  // ------------------------------------------------------------------
+  const double cellArea = S.cell_area[p];
+  const double cf = S.contfreq[p];
+  const int ldd = S.ldd[p];
+  const double laf = landAreaFrac;      // G_landAreaFrac of the day
+  const double laf_prev = X.laf_prev[mp];
+  const double ow_prec = dailyPrec, ow_pet = dailyOpenWaterPET;
+  double dailyLocalSurfaceRunoff, fswb_catchment = 0.;
+  double localRunoff = 0., lrir = 0., lgwrir = 0., recharge_km3 = 0.;
+  if (laf <= 0.)
+    dailyLocalSurfaceRunoff = transfer * cellArea / 1000000. * laf_prev / 100.;
+  else
+    dailyLocalSurfaceRunoff = surf_runoff * cellArea / 1000000. * laf / 100.;
+  if (ldd >= 0) {
+    fswb_catchment = S.fswb_init[p] * 20.;
+    if (fswb_catchment > 1.) fswb_catchment = 1.;
+    lrir = (1. - fswb_catchment) * dailyLocalSurfaceRunoff;
+  }
+  const bool arid = (S.aindex[p] == 1);
+  if (arid && (ldd >= 0)) {
+    localRunoff = fswb_catchment * dailyLocalSurfaceRunoff;
+    recharge_km3 = recharge_handoff * cellArea * (laf / 100.) / 1000000.;
+  } else {
+    // humid cells (routing.cpp:1968-2039) and inland sinks (2112-2159)
+    const double gw_prev = X.gw[mp];
+    const double netGWin = recharge_handoff * cellArea * (laf / 100.) / 1000000.;
+    double gw = gw_prev * S.e_gw[p] + S.inv_kgw[p] * netGWin * S.ome_gw[p];
+    if (fabs(gw) <= kMinStorVol) gw = 0.;
+    double gw_runoff = gw_prev - gw + netGWin;
+    if (gw_runoff <= 0.) {
+      gw_runoff = 0.;
+      gw = gw_prev + netGWin;
+      if (fabs(gw) <= kMinStorVol) gw = 0.;
+    }
+    X.gw[mp] = gw;
+    if (ldd >= 0) {
+      lgwrir = (1. - fswb_catchment) * gw_runoff;
+      const double localGWRunoff = fswb_catchment * gw_runoff;
+      localRunoff = (fswb_catchment * dailyLocalSurfaceRunoff) + localGWRunoff;
+    } else {
+      if (laf == 0.)
+        dailyLocalSurfaceRunoff = transfer * cellArea / 1000000. * laf_prev / 100.;
+      else
+        dailyLocalSurfaceRunoff = surf_runoff * cellArea / 1000000. * laf / 100.;
+      localRunoff = dailyLocalSurfaceRunoff + gw_runoff;
+    }
+  }
+
+  double inflow = localRunoff, outflow = 0.;
+  const double k_sw = S.k_sw[p];
+  const double evarexp = S.evarexp[p];
+  const bool arid_swb = arid && (ldd >= 0);
+  double gwr_loclak = 0., gwr_locwet = 0.;
+  // local lake, routing.cpp:2316-2490
+  const double loc_lake = S.loc_lake[p];
+  if (loc_lake > 0.) {
+    const double prev = X.loc_lake[mp], rf = X.red_ll[mp];
+    const double maxStorage = ((loc_lake) / 100.) * cellArea * S.lake_depth[p];
+    double evap = ((1.0 - cfa) * ow_prec * rf) + (cfa * (ow_pet * rf));
+    if (evap < 0.) evap = 0.;
+    const double totalInflow = inflow + (ow_prec * rf) * (cellArea / 1000000.) * (loc_lake / 100.);
+    if (arid_swb) gwr_loclak = kKswbgw * rf * loc_lake / 100. / (cf / 100.);
+    const double PETgwr = evap * (cellArea / 1000000.) * (loc_lake / 100.) + gwr_loclak * cellArea * (cf / 100.) / 1000000.;
+    double PETgwrMax = prev + maxStorage + totalInflow;
+    if (PETgwrMax < 0.) PETgwrMax = 0.;
+    double st;
+    if (PETgwr > PETgwrMax) {
+      st = (-1.) * maxStorage;
+      gwr_loclak *= PETgwrMax / PETgwr;
+    } else {
+      st = prev + totalInflow - PETgwr;
+    }
+    if (prev > 0.) {
+      outflow = k_sw * prev * pow((prev / maxStorage), kLakeOutflowExp);
+      if (st <= 0.) {
+        outflow = 0;
+      } else {
+        if (outflow > st) outflow = st;
+      }
+    } else {
+      outflow = 0.;
+    }
+    st -= outflow;
+    if (fabs(st) <= kMinStorVol) st = 0.;
+    if (st > maxStorage) {
+      outflow += (st - maxStorage);
+      st = maxStorage;
+    }
+    inflow = outflow;
+    double r = 1. - pow(fabs(st - maxStorage) / (2. * maxStorage), (evarexp * kEvapoRedExp));
+    if (r < 0.) r = 0.;
+    if (r > 1.) r = 1.;
+    X.loc_lake[mp] = st;
+    X.red_ll[mp] = r;
+  }
+  // local wetland, routing.cpp:2495-2617
+  const double loc_wetland = S.loc_wetland[p];
+  if (loc_wetland > 0.) {
+    const double prev = X.loc_wetl[mp], rf = X.red_lw[mp];
+    const double maxStorage = ((loc_wetland) / 100.) * cellArea * S.wetl_depth[p];
+    double evap = ((1.0 - cfa) * ow_prec * rf) + (cfa * (ow_pet * rf));
+    if (evap < 0.) evap = 0.;
+    const double totalInflow = inflow + (ow_prec * rf * (cellArea / 1000000.) * (loc_wetland / 100.));
+    if (arid_swb) gwr_locwet = kKswbgw * rf * loc_wetland / 100. / (cf / 100.);
+    const double PETgwr = evap * (cellArea / 1000000.) * (loc_wetland / 100.) + gwr_locwet * cellArea * (cf / 100.) / 1000000.;
+    const double PETgwrMax = prev + totalInflow;
+    double st;
+    if (PETgwr > PETgwrMax) {
+      st = 0.;
+      gwr_locwet *= PETgwrMax / PETgwr;
+    } else {
+      st = prev + totalInflow - PETgwr;
+    }
+    if (fabs(st) <= kMinStorVol) st = 0.;
+    if (st > 0.) {
+      outflow = k_sw * st * pow((st / maxStorage), kWetlOutflowExp);
+      if (outflow > st) outflow = st;
+    } else {
+      outflow = 0.;
+    }
+    st -= outflow;
+    if (st > maxStorage) {
+      outflow += (st - maxStorage);
+      st = maxStorage;
+    }
+    inflow = outflow;
+    double r = 1. - pow(fabs(st - maxStorage) / (maxStorage), (evarexp * kEvapoRedExp));
+    if (r < 0.) r = 0.;
+    if (r > 1.) r = 1.;
+    X.loc_wetl[mp] = st;
+    X.red_lw[mp] = r;
+  }
+
+  // river terms that do not depend on the inflow, routing.cpp:3424-3479
+  ChainIn in;
+  in.inflow = inflow;
+  in.lrir = lrir;
+  in.lgwrir = lgwrir;
+  in.gwr_ll = gwr_loclak;
+  in.gwr_lw = gwr_locwet;
+  in.ow_prec = ow_prec;
+  in.ow_pet = ow_pet;
+  in.recharge_km3 = recharge_km3;
+  {
+    const double L = S.river_length[p];
+    const double prev = X.river[mp];
+    const double velocity = new_river_velocity(S.river_slope[p], S.bottom_width[p], S.roughness[p], prev, L,
+                                               S.m_rivrgh[p]);
+    const double K = velocity / L;
+    const double eK = exp(-1. * K);
+    const double riverAreaFrac = X.river_frac_next[mp];
+    in.rv_e = eK;
+    in.rv_invk = 1. / K;
+    in.rv_ome = 1. - eK;
+    in.rv_term = (K * prev * eK) / (1. - eK);
+    in.rv_evap = ((1.0 - cfa) * (ow_prec) + (cfa * ow_pet)) * riverAreaFrac / 100. * cellArea / 1000000.;
+    in.rv_prec = ow_prec * riverAreaFrac / 100. * cellArea / 1000000.;
+  }
+  const int n_up = S.up_off[p + 1] - S.up_off[p];
+  if (n_up == 0) {
+    // no upstream cell: G_riverInflow[n] is still 0 when the cell is routed
+    const double tv = chain_step(S, X, T, mp, p, in, 0.0, doy, month);
+    X.tv[mp] = tv;
+    const int st = S.station[p];
+    if (st >= 0) {
+      const double q = (ldd >= 0) ? tv : 0.;
+      X.st_sum[int64_t(m) * S.n_stations + st] += q;
+      X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + st] = q;
+    }
+  } else {
+    X.c_inflow[mp] = in.inflow;
+    X.c_lrir[mp] = in.lrir;
+    X.c_lgwrir[mp] = in.lgwrir;
+    X.c_gwr_ll[mp] = in.gwr_ll;
+    X.c_gwr_lw[mp] = in.gwr_lw;
+    X.c_ow_prec[mp] = in.ow_prec;
+    X.c_ow_pet[mp] = in.ow_pet;
+    X.c_recharge_km3[mp] = in.recharge_km3;
+    X.c_rv_e[mp] = in.rv_e;
+    X.c_rv_invk[mp] = in.rv_invk;
+    X.c_rv_ome[mp] = in.rv_ome;
+    X.c_rv_term[mp] = in.rv_term;
+    X.c_rv_evap[mp] = in.rv_evap;
+    X.c_rv_prec[mp] = in.rv_prec;
+  }
+}
+
+// One workgroup per (routing group, member): levels 1..depth of the group.
+__global__ __launch_bounds__(1024) void wg_chain_kernel(WgStatic S, WgState X, WgTables T,
+                                                        const int32_t *__restrict__ group_level_off,
+                                                        const int32_t *__restrict__ group_first_level,
+                                                        const WgClock *__restrict__ clock, int k,
+                                                        int daily_slot) {
+  const int g = blockIdx.x;
+  const int m = blockIdx.y;
+  const int n = S.n;
+  const int doy = int((clock->doy0 - 1 + k) % 365) + 1;
+  const int month = month_of_doy(doy);
+  const int l0 = group_first_level[g], l1 = group_first_level[g + 1];
+  const int64_t base = int64_t(m) * n;
+  // level 0 of every group was routed by the cell kernel
+  for (int l = l0 + 1; l < l1; l++) {
+    const int beg = group_level_off[l], end = group_level_off[l + 1];
+    for (int p = beg + int(threadIdx.x); p < end; p += blockDim.x) {
+      if (!S.contcell[p]) continue;
+      const int64_t mp = base + p;
+      ChainIn in;
+      in.inflow = X.c_inflow[mp];
+      in.lrir = X.c_lrir[mp];
+      in.lgwrir = X.c_lgwrir[mp];
+      in.gwr_ll = X.c_gwr_ll[mp];
+      in.gwr_lw = X.c_gwr_lw[mp];
+      in.ow_prec = X.c_ow_prec[mp];
+      in.ow_pet = X.c_ow_pet[mp];
+      in.recharge_km3 = X.c_recharge_km3[mp];
+      in.rv_e = X.c_rv_e[mp];
+      in.rv_invk = X.c_rv_invk[mp];
+      in.rv_ome = X.c_rv_ome[mp];
+      in.rv_term = X.c_rv_term[mp];
+      in.rv_evap = X.c_rv_evap[mp];
+      in.rv_prec = X.c_rv_prec[mp];
+      // ordered gather: the serial += of routing.cpp:3955-3958 in routing order
+      double up = 0.;
+      const int u0 = S.up_off[p], u1 = S.up_off[p + 1];
+      for (int u = u0; u < u1; u++) up += X.tv[base + S.up_idx[u]];
+      const double tv = chain_step(S, X, T, mp, p, in, up, doy, month);
+      X.tv[mp] = tv;
+      const int st = S.station[p];
+      if (st >= 0) {
+        const double q = (S.ldd[p] >= 0) ? tv : 0.;
+        X.st_sum[int64_t(m) * S.n_stations + st] += q;
+        X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + st] = q;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Sets the day clock read by the kernels of a captured block of days; launched
+// before each replay so that one graph serves every block of the same length.
+__global__ void wg_clock_set(WgClock *clock, int64_t day0, int32_t doy0, int32_t started) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    clock->day0 = day0;
+    clock->doy0 = doy0;
+    clock->started = started;
+  }
+}
