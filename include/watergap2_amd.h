@@ -261,6 +261,11 @@ int wg_last_timing(wg_handle *h, double *ms, int64_t *n_launches);
  * slots, [3] cells routed by the cell kernel (no upstream cell). */
 int wg_schedule_info(wg_handle *h, int32_t *info);
 
+/* Diagnostic with no reference counterpart: evaluates the library's FP64 math on
+ * the device, fn 0 exp(x), 1 pow(x, y), 2 sqrt(x), 3 log(x), for n arguments in
+ * host arrays.  Used to check that the device and the host produce the same bits. */
+int wg_libm_eval(int32_t fn, int32_t n, const double *x, const double *y, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,17 @@
 #include "wg_oracle.h"
 
 #include <math.h>
+/* The C library functions the reference calls.  By default the deterministic
+ * exp/pow of wg_libm.h (correctly rounded in all sampled cases, the same bits on
+ * the CPU and the GPU); built with -DWG_ORACLE_GLIBC the host C library's. */
+#include "../watergap2_amd/csrc/wg_libm.h"
+#ifdef WG_ORACLE_GLIBC
+#define EXP exp
+#define POW pow
+#else
+#define EXP wg_exp
+#define POW wg_pow
+#endif
 #include <stdlib.h>
 #include <string.h>
 
@@ -215,8 +226,8 @@ wgo_model *wgo_create(int32_t n, const wg_static_fields *st) {
       double bf = (double)st->bankfull_flow[i];
       double depth_bf;
       if (bf < 0.05) bf = 0.05;
-      m->width_bf[i] = 2.71 * pow(bf, 0.557);
-      depth_bf = 0.349 * pow(bf, 0.341);
+      m->width_bf[i] = 2.71 * POW(bf, 0.557);
+      depth_bf = 0.349 * POW(bf, 0.341);
       m->bottom_width[i] = m->width_bf[i] - 2.0 * 2.0 * depth_bf;
       m->river_stor_max[i] = m->river_length[i] * 0.5 * depth_bf / 1000. *
                              (m->bottom_width[i] / 1000. + m->width_bf[i] / 1000.);
@@ -396,7 +407,7 @@ static void vertical_cell(wgo_model *m, int32_t i, double tempC_f, double prec_f
   dailyLongWave = lw_f;
   dailyPrec = P(WG_M_PREC) * dailyPrec;
   double temp2 = dailyTempC + 237.3;
-  double e_s = 0.6108 * exp(17.27 * dailyTempC / temp2);
+  double e_s = 0.6108 * EXP(17.27 * dailyTempC / temp2);
 
   /* daily.cpp:329-349 arid/humid from G_aindex */
   if (st->aindex[i] == 1) {
@@ -440,7 +451,7 @@ static void vertical_cell(wgo_model *m, int32_t i, double tempC_f, double prec_f
   double emissivity = m->lct->emissivity[lct - 1];
   double temp_K = dailyTempC + 273.2;
   const double stefan_boltz_const = 0.000000004903;
-  double long_wave_rad_out = emissivity * stefan_boltz_const * pow(temp_K, 4.) / lat_heat;
+  double long_wave_rad_out = emissivity * stefan_boltz_const * POW(temp_K, 4.) / lat_heat;
   double net_long_wave_rad = long_wave_rad_in - long_wave_rad_out;
   double net_short_wave_rad = solar_rad * (1. - albedo);
   double net_rad = P(WG_M_NETRAD) * (net_short_wave_rad + net_long_wave_rad);
@@ -499,7 +510,7 @@ static void vertical_cell(wgo_model *m, int32_t i, double tempC_f, double prec_f
         daily_prec_to_soil = dailyPrec - canopy_deficiency;
       }
       canopy_water_content = m->canopy[i];
-      dailyCanopyEvapo = dailyPET * pow((canopy_water_content / max_canopy_storage), CANOPY_EVAPO_EXP);
+      dailyCanopyEvapo = dailyPET * POW((canopy_water_content / max_canopy_storage), CANOPY_EVAPO_EXP);
       if (dailyCanopyEvapo > canopy_water_content) {
         dailyCanopyEvapo = canopy_water_content;
         dailySoilPET = dailyPET - canopy_water_content;
@@ -615,7 +626,7 @@ static void vertical_cell(wgo_model *m, int32_t i, double tempC_f, double prec_f
     if (TempElevMax > T_SNOWFZ) {
       if (smax > 0.) {
         soil_saturation = m->soil[i] / smax;
-        daily_runoff = dailyEffPrec * pow(soil_saturation, (double)m->gamma[i]);
+        daily_runoff = dailyEffPrec * POW(soil_saturation, (double)m->gamma[i]);
         if (dailySoilPET > (maxDailyPET - dailyCanopyEvapo) * soil_saturation)
           dailyAET = (maxDailyPET - dailyCanopyEvapo) * soil_saturation;
         else
@@ -709,7 +720,7 @@ static double new_river_velocity(double slope, double bottom, double rough, doub
   double depth = -bottom / 4.0 + sqrt(bottom * bottom / 16.0 + 0.5 * csa);
   double wp = bottom + 2.0 * depth * sqrt(5.0);
   double hr = csa / wp;
-  double v = 1. / (mult * rough) * pow(hr, (2. / 3.)) * pow(slope, 0.5);
+  double v = 1. / (mult * rough) * POW(hr, (2. / 3.)) * POW(slope, 0.5);
   v = v * 86.4;
   if (v < 0.00001)
     return 0.00001;
@@ -758,7 +769,7 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
     /* routing.cpp:1968-2039 humid cells */
     gw_prev = m->gw[i];
     netGWin = (double)m->gw_recharge[i] * cellArea * ((double)m->laf[i] / 100.) / 1000000.;
-    m->gw[i] = gw_prev * exp(-1. * k_gw) + (1. / k_gw) * netGWin * (1. - exp(-1. * k_gw));
+    m->gw[i] = gw_prev * EXP(-1. * k_gw) + (1. / k_gw) * netGWin * (1. - EXP(-1. * k_gw));
     if (fabs(m->gw[i]) <= MIN_STOR_VOL) m->gw[i] = 0.;
     gw_runoff = gw_prev - m->gw[i] + netGWin;
     if (gw_runoff <= 0.) {
@@ -774,7 +785,7 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
     /* routing.cpp:2112-2159 inland sinks */
     gw_prev = m->gw[i];
     netGWin = m->gw_recharge[i] * cellArea * (m->laf[i] / 100.) / 1000000.;
-    m->gw[i] = gw_prev * exp(-1. * k_gw) + (1. / k_gw) * netGWin * (1. - exp(-1. * k_gw));
+    m->gw[i] = gw_prev * EXP(-1. * k_gw) + (1. / k_gw) * netGWin * (1. - EXP(-1. * k_gw));
     if (fabs(m->gw[i]) <= MIN_STOR_VOL) m->gw[i] = 0.;
     gw_runoff = gw_prev - m->gw[i] + netGWin;
     if (gw_runoff <= 0.) {
@@ -817,7 +828,7 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
       m->loc_lake[i] = prev + totalInflow - PETgwr;
     }
     if (prev > 0.) {
-      outflow = k_sw * prev * pow((prev / maxStorage), LAKE_OUTFLOW_EXP);
+      outflow = k_sw * prev * POW((prev / maxStorage), LAKE_OUTFLOW_EXP);
       if (m->loc_lake[i] <= 0.) {
         outflow = 0;
       } else {
@@ -833,7 +844,7 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
       m->loc_lake[i] = maxStorage;
     }
     inflow = outflow;
-    m->red_ll[i] = 1. - pow(fabs(m->loc_lake[i] - maxStorage) / (2. * maxStorage), (evarexp * EVAPO_RED_EXP));
+    m->red_ll[i] = 1. - POW(fabs(m->loc_lake[i] - maxStorage) / (2. * maxStorage), (evarexp * EVAPO_RED_EXP));
     if (m->red_ll[i] < 0.) m->red_ll[i] = 0.;
     if (m->red_ll[i] > 1.) m->red_ll[i] = 1.;
   }
@@ -860,7 +871,7 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
     }
     if (fabs(m->loc_wetl[i]) <= MIN_STOR_VOL) m->loc_wetl[i] = 0.;
     if (m->loc_wetl[i] > 0.) {
-      outflow = k_sw * m->loc_wetl[i] * pow((m->loc_wetl[i] / maxStorage), WETL_OUTFLOW_EXP);
+      outflow = k_sw * m->loc_wetl[i] * POW((m->loc_wetl[i] / maxStorage), WETL_OUTFLOW_EXP);
       if (outflow > m->loc_wetl[i]) outflow = m->loc_wetl[i];
     } else {
       outflow = 0.;
@@ -871,7 +882,7 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
       m->loc_wetl[i] = maxStorage;
     }
     inflow = outflow;
-    m->red_lw[i] = 1. - pow(fabs(m->loc_wetl[i] - maxStorage) / (maxStorage), (evarexp * EVAPO_RED_EXP));
+    m->red_lw[i] = 1. - POW(fabs(m->loc_wetl[i] - maxStorage) / (maxStorage), (evarexp * EVAPO_RED_EXP));
     if (m->red_lw[i] < 0.) m->red_lw[i] = 0.;
     if (m->red_lw[i] > 1.) m->red_lw[i] = 1.;
   }
@@ -897,7 +908,7 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
       outflow = 0.;
       gwr_glolak *= useMax / use;
     } else {
-      m->glo_lake[i] = prev * exp(-1. * k_sw) + (1. / k_sw) * (totalInflow - use) * (1. - exp(-1. * k_sw));
+      m->glo_lake[i] = prev * EXP(-1. * k_sw) + (1. / k_sw) * (totalInflow - use) * (1. - EXP(-1. * k_sw));
       outflow = totalInflow + prev - m->glo_lake[i] - use;
       if (m->glo_lake[i] > maxStorage) {
         outflow += (m->glo_lake[i] - maxStorage);
@@ -910,7 +921,7 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
     }
     if (fabs(m->glo_lake[i]) <= MIN_STOR_VOL) m->glo_lake[i] = 0.;
     inflow = outflow;
-    m->red_gl[i] = 1. - pow(fabs(m->glo_lake[i] - maxStorage) / (2. * maxStorage), (evarexp * EVAPO_RED_EXP));
+    m->red_gl[i] = 1. - POW(fabs(m->glo_lake[i] - maxStorage) / (2. * maxStorage), (evarexp * EVAPO_RED_EXP));
     if (m->red_gl[i] < 0.) m->red_gl[i] = 0.;
     if (m->red_gl[i] > 1.) m->red_gl[i] = 1.;
   }
@@ -978,7 +989,7 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
       m->res[i] = 0.;
     }
     inflow = outflow;
-    m->red_res[i] = 1. - pow(fabs(m->res[i] - maxStorage) / maxStorage, EVAPO_RED_EXP_RES);
+    m->red_res[i] = 1. - POW(fabs(m->res[i] - maxStorage) / maxStorage, EVAPO_RED_EXP_RES);
     if (m->red_res[i] < 0.) m->red_res[i] = 0.;
     if (m->red_res[i] > 1.) m->red_res[i] = 1.;
   }
@@ -1002,7 +1013,7 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
       outflow = 0.;
       gwr_glowet *= PETgwrMax / PETgwr;
     } else {
-      m->glo_wetl[i] = prev * exp(-1. * k_sw) + (1. / k_sw) * (totalInflow - PETgwr) * (1. - exp(-1. * k_sw));
+      m->glo_wetl[i] = prev * EXP(-1. * k_sw) + (1. / k_sw) * (totalInflow - PETgwr) * (1. - EXP(-1. * k_sw));
       outflow = totalInflow + prev - m->glo_wetl[i] - PETgwr;
     }
     if (m->glo_wetl[i] > maxStorage) {
@@ -1011,7 +1022,7 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
     }
     if (fabs(m->glo_wetl[i]) <= MIN_STOR_VOL) m->glo_wetl[i] = 0.;
     inflow = outflow;
-    m->red_gw[i] = 1. - pow(fabs(m->glo_wetl[i] - maxStorage) / maxStorage, (evarexp * EVAPO_RED_EXP));
+    m->red_gw[i] = 1. - POW(fabs(m->glo_wetl[i] - maxStorage) / maxStorage, (evarexp * EVAPO_RED_EXP));
     if (m->red_gw[i] < 0.) m->red_gw[i] = 0.;
     if (m->red_gw[i] > 1.) m->red_gw[i] = 1.;
   }
@@ -1022,7 +1033,7 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
     gw_prev = m->gw[i];
     netGWin = gwr_swb * cellArea * (cf / 100.) / 1000000. +
               (double)m->gw_recharge[i] * cellArea * ((double)m->laf[i] / 100.) / 1000000.;
-    m->gw[i] = gw_prev * exp(-1. * k_gw) + (1. / k_gw) * netGWin * (1. - exp(-1. * k_gw));
+    m->gw[i] = gw_prev * EXP(-1. * k_gw) + (1. / k_gw) * netGWin * (1. - EXP(-1. * k_gw));
     if (fabs(m->gw[i]) <= MIN_STOR_VOL) m->gw[i] = 0.;
     gw_runoff = gw_prev - m->gw[i] + netGWin;
     if (gw_runoff <= 0.) {
@@ -1051,14 +1062,14 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
   double riverPrecip = ow_prec * riverAreaFrac / 100. * cellArea / 1000000.;
   riverInflow += riverPrecip;
   double evapRemUse = 0. + riverEvapo;
-  double evapRemUseMax = riverInflow + (K * prev * exp(-1. * K)) / (1. - exp(-1. * K));
+  double evapRemUseMax = riverInflow + (K * prev * EXP(-1. * K)) / (1. - EXP(-1. * K));
   double tv;
   if (evapRemUse > evapRemUseMax) {
     m->river[i] = 0.;
     tv = riverInflow + prev - evapRemUseMax;
     if (tv < 0.) tv = 0.;
   } else {
-    m->river[i] = prev * exp(-1. * K) + (1. / K) * (riverInflow - evapRemUse) * (1. - exp(-1. * K));
+    m->river[i] = prev * EXP(-1. * K) + (1. / K) * (riverInflow - evapRemUse) * (1. - EXP(-1. * K));
     if (fabs(m->river[i]) <= MIN_STOR_VOL) m->river[i] = 0.;
     tv = riverInflow + prev - m->river[i] - evapRemUse;
     if (tv < 0.) tv = 0.;
@@ -1070,7 +1081,7 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
     double depth = -B / (4. * 1000.) + sqrt(B / 1000. * B / (16. * 1000.) + 0.5 * csa);
     double width = B / 1000. + 4. * depth;
     if (width > m->width_bf[i] / 1000.) width = m->width_bf[i] / 1000.;
-    m->red_river[i] = 1. - pow(fabs(m->river[i] - m->river_stor_max[i]) / m->river_stor_max[i],
+    m->red_river[i] = 1. - POW(fabs(m->river[i] - m->river_stor_max[i]) / m->river_stor_max[i],
                                (evarexp * EVAPO_RED_EXP));
     if (m->red_river[i] < 0.) m->red_river[i] = 0.;
     if (m->red_river[i] > 1.) m->red_river[i] = 1.;
@@ -1303,24 +1314,6 @@ static double irwin_hall_normal(uint64_t seed, uint32_t cell, uint32_t day, uint
   return s - 6.;
 }
 
-/* exp(x) for |x| < 700 from IEEE basic operations only: x = k*ln2 + r with the
- * two-part Cody-Waite constant, a degree-13 Taylor polynomial for e^r in Horner
- * form, and an exact power-of-two scaling.  Deterministic on every IEEE-754 target
- * when compiled without contraction. */
-static double det_exp(double x) {
-  const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
-  double t = x * 1.44269504088896338700e+00;
-  double k = floor(t + 0.5);
-  double r = (x - k * ln2_hi) - k * ln2_lo;
-  double p = 1.0 / 6227020800.0;
-  static const double inv_fact[13] = {1.0 / 479001600.0, 1.0 / 39916800.0, 1.0 / 3628800.0,
-                                      1.0 / 362880.0,    1.0 / 40320.0,    1.0 / 5040.0,
-                                      1.0 / 720.0,       1.0 / 120.0,      1.0 / 24.0,
-                                      1.0 / 6.0,         0.5,              1.0,
-                                      1.0};
-  for (int j = 0; j < 13; j++) p = p * r + inv_fact[j];
-  return ldexp(p, (int)k);
-}
 
 void wgo_perturb_forcing(uint64_t seed, int32_t day_index, int32_t n, double prec_sigma,
                          double temp_sigma, const float *temp_in, const float *prec_in,
@@ -1328,7 +1321,12 @@ void wgo_perturb_forcing(uint64_t seed, int32_t day_index, int32_t n, double pre
   for (int32_t i = 0; i < n; i++) {
     double zp = irwin_hall_normal(seed, (uint32_t)i, (uint32_t)day_index, 0x50524543u);
     double zt = irwin_hall_normal(seed, (uint32_t)i, (uint32_t)day_index, 0x54454d50u);
-    prec_out[i] = (float)((double)prec_in[i] * det_exp(prec_sigma * zp));
+    prec_out[i] = (float)((double)prec_in[i] * wg_exp(prec_sigma * zp));
     temp_out[i] = (float)((double)temp_in[i] + temp_sigma * zt);
   }
 }
+
+/* The math functions in use, exported for the tests. */
+double wgo_exp(double x) { return EXP(x); }
+double wgo_pow(double x, double y) { return POW(x, y); }
+double wgo_log(double x) { return wg_log(x); }

@@ -63,6 +63,11 @@ void wgo_perturb_forcing(uint64_t seed, int32_t day_index, int32_t n, double pre
                          double temp_sigma, const float *temp_in, const float *prec_in,
                          float *temp_out, float *prec_out);
 
+/* The exp, pow and log in use (wg_libm.h, or the C library's in the glibc build). */
+double wgo_exp(double x);
+double wgo_pow(double x, double y);
+double wgo_log(double x);
+
 /* Month (0..11) of julian day 1..365 (integrateWGHM.cpp:100-102 calendar). */
 int wgo_month_of_day(int day);
 

@@ -14,22 +14,25 @@ from watergap2_amd._abi import NELEV, STATE_FIELDS, DAY_OUTPUT_FIELDS
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libwg_oracle.so")
+GLIBC_LIB_PATH = os.path.join(HERE, "libwg_oracle_glibc.so")
 
 
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-_LIB = None
+_LIBS = {}
 
 
-def load():
-    global _LIB
-    if _LIB is not None:
-        return _LIB
-    if not os.path.exists(LIB_PATH):
+def load(variant="wg"):
+    """The oracle library: variant "wg" uses the deterministic exp/pow shared with
+    the kernels, "glibc" the host C library's."""
+    if variant in _LIBS:
+        return _LIBS[variant]
+    path = LIB_PATH if variant == "wg" else GLIBC_LIB_PATH
+    if not os.path.exists(path):
         build()
-    lib = C.CDLL(LIB_PATH)
+    lib = C.CDLL(path)
     i32 = C.c_int32
     P_i32, P_i16, P_i8, P_f, P_d = _abi.P_i32, _abi.P_i16, _abi.P_i8, _abi.P_f, _abi.P_d
     lib.wgo_inflow_outflow.argtypes = [i32, P_i8, P_i16, P_i16, P_i32, P_i32, P_i32]
@@ -56,7 +59,12 @@ def load():
     lib.wgo_perturb_forcing.argtypes = [C.c_uint64, i32, i32, C.c_double, C.c_double, P_f, P_f, P_f, P_f]
     lib.wgo_perturb_forcing.restype = None
     lib.wgo_month_of_day.argtypes = [C.c_int]
-    _LIB = lib
+    for fn in ("wgo_exp", "wgo_log"):
+        getattr(lib, fn).argtypes = [C.c_double]
+        getattr(lib, fn).restype = C.c_double
+    lib.wgo_pow.argtypes = [C.c_double, C.c_double]
+    lib.wgo_pow.restype = C.c_double
+    _LIBS[variant] = lib
     return lib
 
 
@@ -98,8 +106,8 @@ def perturb_forcing(seed, day_index, temp, prec, prec_sigma, temp_sigma):
 class Oracle:
     """One CPU model: calcNewDay for every cell, serial routing, land-area update."""
 
-    def __init__(self, world):
-        self._lib = load()
+    def __init__(self, world, variant="wg"):
+        self._lib = load(variant)
         self.world = world
         self.n = world.n
         self._static = world.static_fields()

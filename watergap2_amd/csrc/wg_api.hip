@@ -22,6 +22,7 @@
 
 #include "../../include/watergap2_amd.h"
 #include "wg_device.h"
+#include "wg_libm.h"
 // The kernels are compiled in this translation unit so that their launches need
 // no relocatable device code.
 #include "wg_kernels.hip"
@@ -101,8 +102,11 @@ int dev_alloc(wg_handle *h, T **out, size_t count) {
   const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
   hipError_t e = hipMalloc(&p, bytes);
   if (e != hipSuccess) return fail(h, WG_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  // The handle's stream does not synchronise with the null stream that the
+  // uploads use: finish the zero fill before any copy can target the buffer.
   e = hipMemsetAsync(p, 0, bytes, h->stream);
-  if (e != hipSuccess) return fail(h, WG_E_HIP, "hipMemsetAsync failed: %s", hipGetErrorString(e));
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return fail(h, WG_E_HIP, "zero fill failed: %s", hipGetErrorString(e));
   h->bufs.push_back({p, bytes});
   *out = static_cast<T *>(p);
   return WG_OK;
@@ -371,8 +375,8 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     roughness[i] = double(st->roughness[i]);
     double bf = double(st->bankfull_flow[i]);
     if (bf < 0.05) bf = 0.05;
-    width_bf[i] = 2.71 * pow(bf, 0.557);
-    const double depth_bf = 0.349 * pow(bf, 0.341);
+    width_bf[i] = 2.71 * wg_pow(bf, 0.557);
+    const double depth_bf = 0.349 * wg_pow(bf, 0.341);
     bottom[i] = width_bf[i] - 2.0 * 2.0 * depth_bf;
     stor_max[i] = river_length[i] * 0.5 * depth_bf / 1000. * (bottom[i] / 1000. + width_bf[i] / 1000.);
     lake_depth[i] = (P(WG_P_LAK_D, i) * 0.001);
@@ -381,13 +385,13 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     f_glo_lake[i] = st->glo_lake[i] / 100.;
     fswb_init[i] = fll + flw + fgw;
     k_sw[i] = P(WG_P_SWOUTF_C, i);
-    e_sw[i] = exp(-1. * k_sw[i]);
+    e_sw[i] = wg_exp(-1. * k_sw[i]);
     inv_ksw[i] = 1. / k_sw[i];
-    ome_sw[i] = 1. - exp(-1. * k_sw[i]);
+    ome_sw[i] = 1. - wg_exp(-1. * k_sw[i]);
     k_gw[i] = P(WG_P_GWOUTF_C, i);
-    e_gw[i] = exp(-1. * k_gw[i]);
+    e_gw[i] = wg_exp(-1. * k_gw[i]);
     inv_kgw[i] = 1. / k_gw[i];
-    ome_gw[i] = 1. - exp(-1. * k_gw[i]);
+    ome_gw[i] = 1. - wg_exp(-1. * k_gw[i]);
   }
 
   // initial state (one member, reference order)
@@ -999,4 +1003,26 @@ extern "C" int wg_schedule_info(wg_handle *h, int32_t *info) {
   }
   info[3] = lvl0;
   return WG_OK;
+}
+
+extern "C" int wg_libm_eval(int32_t fn, int32_t n, const double *x, const double *y, double *out) {
+  if (n <= 0 || !x || !out || fn < 0 || fn > 3 || (fn == 1 && !y)) return WG_E_ARG;
+  double *dx = nullptr, *dy = nullptr, *dout = nullptr;
+  const size_t b = size_t(n) * sizeof(double);
+  int rc = WG_OK;
+  if (hipMalloc(&dx, b) != hipSuccess || hipMalloc(&dy, b) != hipSuccess || hipMalloc(&dout, b) != hipSuccess) {
+    rc = WG_E_NOMEM;
+  } else if (hipMemcpy(dx, x, b, hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(dy, y ? y : x, b, hipMemcpyHostToDevice) != hipSuccess) {
+    rc = WG_E_HIP;
+  } else {
+    hipLaunchKernelGGL(wg_libm_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, fn, n, dx, dy, dout);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(out, dout, b, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = WG_E_HIP;
+  }
+  if (dx) (void)hipFree(dx);
+  if (dy) (void)hipFree(dy);
+  if (dout) (void)hipFree(dout);
+  return rc;
 }

@@ -18,10 +18,12 @@
 //                    arid groundwater, river).
 // Arithmetic follows the reference expression by expression (operand order and
 // association), compiled with -ffp-contract=off, in FP64 with the reference's
-// float/short inputs promoted exactly where the reference promotes them.
+// float/short inputs promoted exactly where the reference promotes them; exp and
+// pow are the deterministic ones of wg_libm.h, shared with the CPU oracle.
 #include <hip/hip_runtime.h>
 
 #include "wg_device.h"
+#include "wg_libm.h"
 
 namespace {
 
@@ -45,7 +47,7 @@ __device__ inline int month_of_doy(int doy) {
   return m;
 }
 
-// ---- ensemble perturbation: Philox4x32-10 + Irwin-Hall + basic-op exp ----
+// ---- ensemble perturbation: Philox4x32-10 + Irwin-Hall normal deviates ----
 __device__ inline void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; r++) {
@@ -69,19 +71,6 @@ __device__ double irwin_hall(uint64_t seed, uint32_t cell, uint32_t day, uint32_
     for (int k = 0; k < 4; k++) s += (double(c[k]) + 0.5) * 2.3283064365386963e-10;
   }
   return s - 6.;
-}
-
-__device__ double det_exp(double x) {
-  const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
-  const double t = x * 1.44269504088896338700e+00;
-  const double k = floor(t + 0.5);
-  const double r = (x - k * ln2_hi) - k * ln2_lo;
-  const double inv_fact[13] = {1.0 / 479001600.0, 1.0 / 39916800.0, 1.0 / 3628800.0, 1.0 / 362880.0,
-                               1.0 / 40320.0, 1.0 / 5040.0, 1.0 / 720.0, 1.0 / 120.0, 1.0 / 24.0,
-                               1.0 / 6.0, 0.5, 1.0, 1.0};
-  double p = 1.0 / 6227020800.0;
-  for (int j = 0; j < 13; j++) p = p * r + inv_fact[j];
-  return ldexp(p, int(k));
 }
 
 // ---- the upstream-dependent part of one routing cell ----
@@ -301,7 +290,7 @@ __device__ inline double new_river_velocity(double slope, double bottom, double 
   const double depth = -bottom / 4.0 + sqrt(bottom * bottom / 16.0 + 0.5 * csa);
   const double wp = bottom + 2.0 * depth * sqrt(5.0);
   const double hr = csa / wp;
-  double v = 1. / (mult * rough) * pow(hr, (2. / 3.)) * pow(slope, 0.5);
+  double v = 1. / (mult * rough) * wg_pow(hr, (2. / 3.)) * wg_pow(slope, 0.5);
   v = v * 86.4;
   return (v < 0.00001) ? 0.00001 : v;
 }
@@ -384,14 +373,14 @@ __device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p)
   double red_ll = X.red_ll[mp], red_lw = X.red_lw[mp], red_gw = X.red_gw[mp];
   if (S.lake_area[p] > 0.) {
     const double maxStorage = S.lake_area[p] * S.lake_depth[p];
-    double r = 1. - pow(fabs(X.glo_lake[mp] - maxStorage) / (2. * maxStorage), (evarexp * kEvapoRedExp));
+    double r = 1. - wg_pow(fabs(X.glo_lake[mp] - maxStorage) / (2. * maxStorage), (evarexp * kEvapoRedExp));
     if (r < 0.) r = 0.;
     if (r > 1.) r = 1.;
     X.red_gl[mp] = r;
   }
   if (S.reservoir_area[p] > 0.) {
     const double maxStorage = S.stor_cap[p];
-    double r = 1. - pow(fabs(X.res[mp] - maxStorage) / maxStorage, kEvapoRedExpRes);
+    double r = 1. - wg_pow(fabs(X.res[mp] - maxStorage) / maxStorage, kEvapoRedExpRes);
     if (r < 0.) r = 0.;
     if (r > 1.) r = 1.;
     X.red_res[mp] = r;
@@ -399,7 +388,7 @@ __device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p)
   const double glo_wetland = S.glo_wetland[p];
   if (glo_wetland > 0) {
     const double maxStorage = ((glo_wetland) / 100.) * cellArea * S.wetl_depth[p];
-    double r = 1. - pow(fabs(X.glo_wetl[mp] - maxStorage) / maxStorage, (evarexp * kEvapoRedExp));
+    double r = 1. - wg_pow(fabs(X.glo_wetl[mp] - maxStorage) / maxStorage, (evarexp * kEvapoRedExp));
     if (r < 0.) r = 0.;
     if (r > 1.) r = 1.;
     red_gw = r;
@@ -416,7 +405,7 @@ __device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p)
     double width = B / 1000. + 4. * depth;
     if (width > S.width_bf[p] / 1000.) width = S.width_bf[p] / 1000.;
     const double smax = S.river_stor_max[p];
-    red_river = 1. - pow(fabs(st - smax) / smax, (evarexp * kEvapoRedExp));
+    red_river = 1. - wg_pow(fabs(st - smax) / smax, (evarexp * kEvapoRedExp));
     if (red_river < 0.) red_river = 0.;
     if (red_river > 1.) red_river = 1.;
     river_frac_next = red_river * L * width * 100. / cellArea;
@@ -535,7 +524,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     const uint32_t ref = uint32_t(S.ref_index[p]);
     const double zp = irwin_hall(seed, ref, uint32_t(fday), 0x50524543u);
     const double zt = irwin_hall(seed, ref, uint32_t(fday), 0x54454d50u);
-    pf = float(double(pf) * det_exp(X.prec_sigma * zp));
+    pf = float(double(pf) * wg_exp(X.prec_sigma * zp));
     tf = float(double(tf) + X.temp_sigma * zt);
   }
 
@@ -550,7 +539,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   const double dailyShortWave = swf, dailyLongWave = lwf;
   dailyPrec = S.m_prec[p] * dailyPrec;  // daily.cpp:248
   const double temp2 = dailyTempC + 237.3;
-  const double e_s = 0.6108 * exp(17.27 * dailyTempC / temp2);
+  const double e_s = 0.6108 * wg_exp(17.27 * dailyTempC / temp2);
   const double alpha = S.alpha[p];
   const double maxDailyPET = S.pet_mxdy[p];
 
@@ -575,7 +564,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   const double solar_rad = conv_Wm2_to_mmd * dailyShortWave;
   const double long_wave_rad_in = conv_Wm2_to_mmd * dailyLongWave;
   const double temp_K = dailyTempC + 273.2;
-  const double long_wave_rad_out = T.emissivity[lct - 1] * 0.000000004903 * pow(temp_K, 4.) / lat_heat;
+  const double long_wave_rad_out = T.emissivity[lct - 1] * 0.000000004903 * wg_pow(temp_K, 4.) / lat_heat;
   const double net_long_wave_rad = long_wave_rad_in - long_wave_rad_out;
   const double net_short_wave_rad = solar_rad * (1. - albedo);
   const double net_rad = S.m_netrad[p] * (net_short_wave_rad + net_long_wave_rad);
@@ -615,7 +604,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
         daily_prec_to_soil = dailyPrec - canopy_deficiency;
       }
       const double canopy_water_content = canopy;
-      dailyCanopyEvapo = dailyPET * pow((canopy_water_content / max_canopy_storage), kCanopyEvapoExp);
+      dailyCanopyEvapo = dailyPET * wg_pow((canopy_water_content / max_canopy_storage), kCanopyEvapoExp);
       if (dailyCanopyEvapo > canopy_water_content) {
         dailyCanopyEvapo = canopy_water_content;
         dailySoilPET = dailyPET - canopy_water_content;
@@ -735,7 +724,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     if (TempElevMax > T_SNOWFZ) {
       if (smax > 0.) {
         const double soil_saturation = soil / smax;
-        daily_runoff = dailyEffPrec * pow(soil_saturation, X.gamma[mp]);
+        daily_runoff = dailyEffPrec * wg_pow(soil_saturation, X.gamma[mp]);
         if (dailySoilPET > (maxDailyPET - dailyCanopyEvapo) * soil_saturation)
           dailyAET = (maxDailyPET - dailyCanopyEvapo) * soil_saturation;
         else
@@ -884,7 +873,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
       st = prev + totalInflow - PETgwr;
     }
     if (prev > 0.) {
-      outflow = k_sw * prev * pow((prev / maxStorage), kLakeOutflowExp);
+      outflow = k_sw * prev * wg_pow((prev / maxStorage), kLakeOutflowExp);
       if (st <= 0.) {
         outflow = 0;
       } else {
@@ -900,7 +889,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
       st = maxStorage;
     }
     inflow = outflow;
-    double r = 1. - pow(fabs(st - maxStorage) / (2. * maxStorage), (evarexp * kEvapoRedExp));
+    double r = 1. - wg_pow(fabs(st - maxStorage) / (2. * maxStorage), (evarexp * kEvapoRedExp));
     if (r < 0.) r = 0.;
     if (r > 1.) r = 1.;
     X.loc_lake[mp] = st;
@@ -926,7 +915,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     }
     if (fabs(st) <= kMinStorVol) st = 0.;
     if (st > 0.) {
-      outflow = k_sw * st * pow((st / maxStorage), kWetlOutflowExp);
+      outflow = k_sw * st * wg_pow((st / maxStorage), kWetlOutflowExp);
       if (outflow > st) outflow = st;
     } else {
       outflow = 0.;
@@ -937,7 +926,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
       st = maxStorage;
     }
     inflow = outflow;
-    double r = 1. - pow(fabs(st - maxStorage) / (maxStorage), (evarexp * kEvapoRedExp));
+    double r = 1. - wg_pow(fabs(st - maxStorage) / (maxStorage), (evarexp * kEvapoRedExp));
     if (r < 0.) r = 0.;
     if (r > 1.) r = 1.;
     X.loc_wetl[mp] = st;
@@ -960,7 +949,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     const double velocity = new_river_velocity(S.river_slope[p], S.bottom_width[p], S.roughness[p], prev, L,
                                                S.m_rivrgh[p]);
     const double K = velocity / L;
-    const double eK = exp(-1. * K);
+    const double eK = wg_exp(-1. * K);
     const double riverAreaFrac = X.river_frac_next[mp];
     in.rv_e = eK;
     in.rv_invk = 1. / K;
@@ -1057,4 +1046,21 @@ __global__ void wg_clock_set(WgClock *clock, int64_t day0, int32_t doy0, int32_t
     clock->doy0 = doy0;
     clock->started = started;
   }
+}
+
+// Evaluates the shared math functions on the device for a batch of arguments
+// (0: wg_exp(x), 1: wg_pow(x, y), 2: sqrt(x), 3: wg_log(x)); used to show that
+// the device returns the host's bits.
+__global__ void wg_libm_kernel(int fn, int n, const double *__restrict__ x, const double *__restrict__ y,
+                               double *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double r;
+  switch (fn) {
+    case 0: r = wg_exp(x[i]); break;
+    case 1: r = wg_pow(x[i], y[i]); break;
+    case 2: r = sqrt(x[i]); break;
+    default: r = wg_log(x[i]); break;
+  }
+  out[i] = r;
 }
