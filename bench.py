@@ -77,7 +77,19 @@ def main():
     ap.add_argument("--cpu-days", type=int, default=240,
                     help="days of the CPU-oracle sample (bounded CPU baseline)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this size (counter calibration) and exit")
     args = ap.parse_args()
+
+    if args.copy_probe_mib:
+        import torch
+        a = torch.empty(args.copy_probe_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 4, "launches": 3}))
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the path shards by basin with no "
