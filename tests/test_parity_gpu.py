@@ -137,7 +137,7 @@ def test_level_schedule_invariance(wg, regional_world):
     w = regional_world
     f = w.forcing(seed=3, day0=0, n_days=30)
     states = []
-    for cells, threads in ((1024, 256), (1, 64), (100000, 1024)):
+    for cells, threads in ((1024, 256), (1, 64), (100000, 128)):
         os.environ["WG_GROUP_CELLS"] = str(cells)
         os.environ["WG_CHAIN_THREADS"] = str(threads)
         try:

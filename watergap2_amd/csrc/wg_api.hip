@@ -68,6 +68,7 @@ struct wg_handle {
   // graphs keyed by block length
   std::map<int, hipGraphExec_t> graphs;
   int block_threads_cell = 256, block_threads_chain = 256;
+  int32_t max_group_cells = 0;
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms[3] = {0, 0, 0};
@@ -335,7 +336,7 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
   if (e != hipSuccess) return WG_E_HIP;
   e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e != hipSuccess) return WG_E_HIP;
-  if (const char *v = getenv("WG_CHAIN_THREADS")) h->block_threads_chain = std::max(64, std::min(1024, atoi(v)));
+  if (const char *v = getenv("WG_CHAIN_THREADS")) h->block_threads_chain = std::max(64, std::min(256, atoi(v)));
   int group_cells = 1024;
   if (const char *v = getenv("WG_GROUP_CELLS")) group_cells = std::max(1, atoi(v));
   int rc = build_schedule(h.get(), st, group_cells);
@@ -345,6 +346,18 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
   }
   wg_handle *H = h.get();
   const int32_t M = n_members;
+  {
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, H->device) != hipSuccess)
+      lds = 65536;
+    int32_t biggest = 0;
+    for (int32_t gg = 0; gg < H->n_groups; gg++)
+      biggest = std::max(biggest, H->level_off[H->group_first_level[gg + 1]] - H->level_off[H->group_first_level[gg]]);
+    if (size_t(biggest) * sizeof(double) > size_t(lds)) {
+      *out = h.release();
+      return fail(*out, WG_E_INPUT, "a basin of %d cells exceeds the %d bytes of LDS per workgroup", biggest, lds);
+    }
+  }
   const double *par = st->params;
   auto P = [&](int id, int32_t i) { return par[size_t(id) * n + i]; };
 
@@ -490,14 +503,23 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     UPR(df, st->built_up); S.built_up = df;
     UPR(df, st->gw_factor); S.gw_factor = df;
     UP(df, lai_max); S.lai_max = df;
-    // elevation [101][N] device order
-    std::vector<int16_t> ev(size_t(WG_NELEV) * n);
-    for (int c = 0; c < WG_NELEV; c++)
-      for (int32_t p = 0; p < n; p++) ev[size_t(c) * n + p] = st->elevation[size_t(H->perm[p]) * WG_NELEV + c];
-    int16_t *dev;
-    int r_ = upload(H, &dev, ev);
+    // elevation: channel 0 [N], sub-cells as packed int16 pairs [50][N]
+    std::vector<int16_t> e0(n);
+    std::vector<int32_t> pairs(size_t(WG_DEV_NPAIR) * n);
+    for (int32_t p = 0; p < n; p++) {
+      const int16_t *ev = st->elevation + size_t(H->perm[p]) * WG_NELEV;
+      e0[p] = ev[0];
+      for (int k = 0; k < WG_DEV_NPAIR; k++)
+        pairs[size_t(k) * n + p] = int32_t(uint32_t(uint16_t(ev[1 + 2 * k])) | (uint32_t(uint16_t(ev[2 + 2 * k])) << 16));
+    }
+    int16_t *de0;
+    int r_ = upload(H, &de0, e0);
     if (r_) { *out = h.release(); return r_; }
-    S.elevation = dev;
+    S.elev0 = de0;
+    int32_t *dpairs;
+    r_ = upload(H, &dpairs, pairs);
+    if (r_) { *out = h.release(); return r_; }
+    S.elev_pairs = dpairs;
   }
   // upstream CSR in device indices, sorted (device order follows routing order within a group)
   {
@@ -506,11 +528,20 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
       const int32_t d = st->downstream[i];
       if (d > 0) ups[H->inv[d - 1]].push_back(H->inv[i]);
     }
+    // group of every device cell, for group-local upstream indices (the chain
+    // kernel keeps a group's transported volumes in LDS)
+    std::vector<int32_t> gstart(n, 0);
+    H->max_group_cells = 0;
+    for (int32_t gg = 0; gg < H->n_groups; gg++) {
+      const int32_t a = H->level_off[H->group_first_level[gg]], b = H->level_off[H->group_first_level[gg + 1]];
+      for (int32_t p = a; p < b; p++) gstart[p] = a;
+      H->max_group_cells = std::max(H->max_group_cells, b - a);
+    }
     std::vector<int32_t> off(n + 1, 0), idx;
     for (int32_t p = 0; p < n; p++) {
       std::sort(ups[p].begin(), ups[p].end());
       off[p + 1] = off[p] + int32_t(ups[p].size());
-      idx.insert(idx.end(), ups[p].begin(), ups[p].end());
+      for (int32_t u : ups[p]) idx.push_back(u - gstart[p]);
     }
     int32_t *d;
     int r_ = upload(H, &d, off);
@@ -548,7 +579,7 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
   rc = dev_alloc(H, &X.lai_days, NM);
   if (!rc) rc = dev_alloc(H, &X.lai_status, NM);
   if (!rc) rc = dev_alloc(H, &X.smax, NM);
-  if (!rc) rc = dev_alloc(H, &X.snow_elev, NM * kNsub);
+  if (!rc) rc = dev_alloc(H, &X.snow2, NM * WG_DEV_NPAIR);
   if (!rc) rc = dev_alloc(H, &X.st_sum, 1);
   if (!rc) rc = dev_alloc(H, &X.st_daily, 1);
   if (!rc) rc = dev_alloc(H, &H->d_clock, 1);
@@ -638,13 +669,17 @@ extern "C" int wg_get_state(wg_handle *h, int32_t member, wg_state *o) {
     }
   }
   if (o->snow_elev) {
-    std::vector<double> se(size_t(n) * kNsub);
-    HIP_CHECK(h, hipMemcpy(se.data(), h->X.snow_elev + off * kNsub, se.size() * sizeof(double),
+    std::vector<WgD2> se(size_t(n) * WG_DEV_NPAIR);
+    HIP_CHECK(h, hipMemcpy(se.data(), h->X.snow2 + off * WG_DEV_NPAIR, se.size() * sizeof(WgD2),
                            hipMemcpyDeviceToHost));
     for (int32_t p = 0; p < n; p++) {
       const int32_t i = h->perm[p];
-      o->snow_elev[size_t(i) * WG_NELEV] = h->snow_ch0[off + i];
-      for (int s = 0; s < kNsub; s++) o->snow_elev[size_t(i) * WG_NELEV + 1 + s] = se[size_t(s) * n + p];
+      double *dst = o->snow_elev + size_t(i) * WG_NELEV;
+      dst[0] = h->snow_ch0[off + i];
+      for (int k = 0; k < WG_DEV_NPAIR; k++) {
+        dst[1 + 2 * k] = se[size_t(k) * n + p].x;
+        dst[2 + 2 * k] = se[size_t(k) * n + p].y;
+      }
     }
   }
   o->started = h->started ? 1 : 0;
@@ -677,14 +712,14 @@ extern "C" int wg_set_state(wg_handle *h, int32_t member, const wg_state *in) {
     }
   }
   if (in->snow_elev) {
-    std::vector<double> se(size_t(n) * kNsub);
+    std::vector<WgD2> se(size_t(n) * WG_DEV_NPAIR);
     for (int32_t p = 0; p < n; p++) {
-      const int32_t i = h->perm[p];
-      for (int s = 0; s < kNsub; s++) se[size_t(s) * n + p] = in->snow_elev[size_t(i) * WG_NELEV + 1 + s];
+      const double *src = in->snow_elev + size_t(h->perm[p]) * WG_NELEV;
+      for (int k = 0; k < WG_DEV_NPAIR; k++) se[size_t(k) * n + p] = WgD2{src[1 + 2 * k], src[2 + 2 * k]};
     }
     for (int32_t mm = m0; mm < m1; mm++) {
       for (int32_t i = 0; i < n; i++) h->snow_ch0[size_t(mm) * n + i] = in->snow_elev[size_t(i) * WG_NELEV];
-      HIP_CHECK(h, hipMemcpy(h->X.snow_elev + size_t(mm) * n * kNsub, se.data(), se.size() * sizeof(double),
+      HIP_CHECK(h, hipMemcpy(h->X.snow2 + size_t(mm) * n * WG_DEV_NPAIR, se.data(), se.size() * sizeof(WgD2),
                              hipMemcpyHostToDevice));
     }
   }
@@ -807,7 +842,8 @@ int enqueue_block(wg_handle *h, int days, hipEvent_t *evs) {
                        slot);
     HIP_CHECK(h, hipGetLastError());
     if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], h->stream));
-    hipLaunchKernelGGL(wg_chain_kernel, chgrid, dim3(h->block_threads_chain), 0, h->stream, h->S, h->X, h->T,
+    hipLaunchKernelGGL(wg_chain_kernel, chgrid, dim3(h->block_threads_chain),
+                       size_t(h->max_group_cells) * sizeof(double), h->stream, h->S, h->X, h->T,
                        h->d_level_off, h->d_group_first, h->d_clock, k, slot);
     HIP_CHECK(h, hipGetLastError());
   }

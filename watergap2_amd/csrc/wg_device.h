@@ -2,14 +2,20 @@
 //
 // Cells live in a device order p (routing group, then Kahn level, then routing
 // order), so that every level of a group is a contiguous range.  Per-member
-// arrays are [member][N]; sub-cell arrays are [member][sub-cell][N] so that the
-// 64 lanes of a wavefront (64 consecutive cells) read one sub-cell contiguously.
+// arrays are [member][N]; sub-cell arrays are [member][sub-cell pair][N] so that the
+// 64 lanes of a wavefront (64 consecutive cells) read two sub-cells contiguously.
 #ifndef WG_DEVICE_H
 #define WG_DEVICE_H
 
 #include <stdint.h>
 
 #define WG_DEV_NSUB 100
+#define WG_DEV_NPAIR 50
+
+// Two consecutive elevation sub-cells of one cell (one 16-byte load).
+struct alignas(16) WgD2 {
+  double x, y;
+};
 
 // Land-cover tables indexed by class-1 (LCT_22.DAT, LAI_22.DAT).
 struct WgTables {
@@ -28,7 +34,8 @@ struct WgStatic {
   const int8_t *ldd, *land_cover, *texture, *contcell;
   const int16_t *aindex, *rgmax;
   const float *built_up, *gw_factor, *lai_max;
-  const int16_t *elevation;     // [101][N]
+  const int16_t *elev0;         // [N] mean cell elevation (channel 0 of G_Elevation)
+  const int32_t *elev_pairs;    // [50][N] sub-cell elevations 2k+1 (low half) and 2k+2 (high half)
   // calibration parameters used by the day loop
   const double *alpha, *pet_mxdy, *m_prec, *m_netrad, *mcwh, *t_snowfz, *t_snowmt, *t_gradnt,
       *degday, *pcritgwa, *evarexp, *m_rivrgh;
@@ -49,7 +56,7 @@ struct WgStatic {
 struct WgState {
   double *canopy, *soil, *snow, *transfer, *lai_precsum;
   int32_t *lai_days, *lai_status;
-  double *snow_elev;            // [M][100][N]
+  WgD2 *snow2;                  // [M][50][N] sub-cell snow storages 2k+1, 2k+2
   double *gw, *loc_lake, *loc_wetl, *glo_lake, *glo_wetl, *res, *river;
   double *red_ll, *red_lw, *red_gl, *red_gw, *red_res, *red_river;
   double *river_frac_next, *river_frac_change, *fswb_next, *laf, *laf_prev, *k_release;

@@ -75,37 +75,120 @@ __device__ double irwin_hall(uint64_t seed, uint32_t cell, uint32_t day, uint32_
 
 // ---- the upstream-dependent part of one routing cell ----
 
-struct ChainIn {
+// Everything routing.cpp:2623-3958 reads for one cell, loaded up front so that
+// the loads of the next level can be issued before a workgroup barrier.
+struct ChainCell {
+  // record written by the cell kernel
   double inflow;        // outflow of the local lake/wetland system (routing.cpp:2616)
   double lrir, lgwrir;  // G_localRunoffIntoRiver, G_localGWRunoffIntoRiver (humid cells)
   double gwr_ll, gwr_lw;
   double ow_prec, ow_pet;
   double recharge_km3;  // arid cells: recharge term of netGWin (routing.cpp:3320-3321)
   double rv_e, rv_invk, rv_ome, rv_term, rv_evap, rv_prec;
+  // statics
+  double cellArea, cf, lake_area, lake_depth, res_area, stor_cap, mean_outflow, glo_wetland, wetl_depth;
+  double e_sw, inv_ksw, ome_sw, e_gw, inv_kgw, ome_gw;
+  int ldd, aindex, start_month, res_type;
+  // state of the previous day
+  double cfa, glo_lake, red_gl, res, red_res, k_release, glo_wetl, red_gw, gw, river;
 };
 
-// routing.cpp:2623-3958 for one cell; returns transportedVolume.
-__device__ double chain_step(const WgStatic &S, const WgState &X, const WgTables &T, int64_t mp, int p,
-                             const ChainIn &in, double inflow_up, int day, int month) {
-  const double cellArea = S.cell_area[p];
-  const double cf = S.contfreq[p];
-  const double cfa = X.cfa[mp];
-  const int ldd = S.ldd[p];
-  const bool arid_swb = (S.aindex[p] == 1) && (ldd >= 0);
-  const double ow_prec = in.ow_prec, ow_pet = in.ow_pet;
-  double inflow = in.inflow;
+struct ChainOut {
+  double glo_lake, res, k_release, glo_wetl, gw, river, tv;
+};
+
+__device__ inline void load_chain_statics(const WgStatic &S, int p, ChainCell &c) {
+  c.cellArea = S.cell_area[p];
+  c.cf = S.contfreq[p];
+  c.lake_area = S.lake_area[p];
+  c.lake_depth = S.lake_depth[p];
+  c.res_area = S.reservoir_area[p];
+  c.stor_cap = S.stor_cap[p];
+  c.mean_outflow = S.mean_outflow[p];
+  c.glo_wetland = S.glo_wetland[p];
+  c.wetl_depth = S.wetl_depth[p];
+  c.e_sw = S.e_sw[p];
+  c.inv_ksw = S.inv_ksw[p];
+  c.ome_sw = S.ome_sw[p];
+  c.e_gw = S.e_gw[p];
+  c.inv_kgw = S.inv_kgw[p];
+  c.ome_gw = S.ome_gw[p];
+  c.ldd = S.ldd[p];
+  c.aindex = S.aindex[p];
+  c.start_month = S.start_month[p];
+  c.res_type = S.res_type[p];
+}
+
+__device__ inline void load_chain_state(const WgState &X, int64_t mp, ChainCell &c) {
+  c.cfa = X.cfa[mp];
+  c.glo_lake = X.glo_lake[mp];
+  c.red_gl = X.red_gl[mp];
+  c.res = X.res[mp];
+  c.red_res = X.red_res[mp];
+  c.k_release = X.k_release[mp];
+  c.glo_wetl = X.glo_wetl[mp];
+  c.red_gw = X.red_gw[mp];
+  c.gw = X.gw[mp];
+  c.river = X.river[mp];
+}
+
+__device__ inline void load_chain_record(const WgState &X, int64_t mp, ChainCell &c) {
+  c.inflow = X.c_inflow[mp];
+  c.lrir = X.c_lrir[mp];
+  c.lgwrir = X.c_lgwrir[mp];
+  c.gwr_ll = X.c_gwr_ll[mp];
+  c.gwr_lw = X.c_gwr_lw[mp];
+  c.ow_prec = X.c_ow_prec[mp];
+  c.ow_pet = X.c_ow_pet[mp];
+  c.recharge_km3 = X.c_recharge_km3[mp];
+  c.rv_e = X.c_rv_e[mp];
+  c.rv_invk = X.c_rv_invk[mp];
+  c.rv_ome = X.c_rv_ome[mp];
+  c.rv_term = X.c_rv_term[mp];
+  c.rv_evap = X.c_rv_evap[mp];
+  c.rv_prec = X.c_rv_prec[mp];
+}
+
+// Writes what chain_step changed.
+__device__ inline void store_chain(const WgState &X, int64_t mp, const ChainCell &c, const ChainOut &o) {
+  if (c.lake_area > 0.) X.glo_lake[mp] = o.glo_lake;
+  if (c.res_area > 0.) {
+    X.res[mp] = o.res;
+    X.k_release[mp] = o.k_release;
+  }
+  if (c.glo_wetland > 0) X.glo_wetl[mp] = o.glo_wetl;
+  if ((c.aindex == 1) && (c.ldd >= 0)) X.gw[mp] = o.gw;
+  X.river[mp] = o.river;
+  X.tv[mp] = o.tv;
+}
+
+// routing.cpp:2623-3958 for one cell (parity profile: no water use, no station
+// correction).  inflow_up is G_riverInflow[n] when the cell is reached.
+__device__ ChainOut chain_step(const ChainCell &c, double inflow_up, int day, int month) {
+  ChainOut o;
+  const double cellArea = c.cellArea;
+  const double cf = c.cf;
+  const double cfa = c.cfa;
+  const int ldd = c.ldd;
+  const bool arid_swb = (c.aindex == 1) && (ldd >= 0);
+  const double ow_prec = c.ow_prec, ow_pet = c.ow_pet;
+  double inflow = c.inflow;
   double outflow = 0., maxStorage, totalInflow;
   double gwr_glolak = 0., gwr_glowet = 0., gwr_res = 0.;
+  o.glo_lake = c.glo_lake;
+  o.res = c.res;
+  o.k_release = c.k_release;
+  o.glo_wetl = c.glo_wetl;
+  o.gw = c.gw;
 
   inflow += inflow_up;  // routing.cpp:2623
 
   // global lake, routing.cpp:2630-2804
-  const double lake_area = S.lake_area[p];
+  const double lake_area = c.lake_area;
   if (lake_area > 0.) {
-    const double prev = X.glo_lake[mp], rf = X.red_gl[mp];
-    const double k_sw = S.k_sw[p];
+    const double prev = c.glo_lake, rf = c.red_gl;
     double use, useMax, evap, st;
-    maxStorage = lake_area * S.lake_depth[p];
+    maxStorage = lake_area * c.lake_depth;
     evap = ((1.0 - cfa) * ow_prec) + (cfa * ow_pet * rf);
     if (evap < 0.) evap = 0.;
     totalInflow = inflow + (ow_prec * (lake_area / 1000000.));
@@ -120,7 +203,7 @@ __device__ double chain_step(const WgStatic &S, const WgState &X, const WgTables
       outflow = 0.;
       gwr_glolak *= useMax / use;
     } else {
-      st = prev * S.e_sw[p] + S.inv_ksw[p] * (totalInflow - use) * S.ome_sw[p];
+      st = prev * c.e_sw + c.inv_ksw * (totalInflow - use) * c.ome_sw;
       outflow = totalInflow + prev - st - use;
       if (st > maxStorage) {
         outflow += (st - maxStorage);
@@ -132,16 +215,15 @@ __device__ double chain_step(const WgStatic &S, const WgState &X, const WgTables
       }
     }
     if (fabs(st) <= kMinStorVol) st = 0.;
-    X.glo_lake[mp] = st;
+    o.glo_lake = st;
     inflow = outflow;
-    (void)k_sw;
   }
 
   // reservoir, routing.cpp:2807-3082
-  const double res_area = S.reservoir_area[p];
+  const double res_area = c.res_area;
   if (res_area > 0.) {
-    const double prev = X.res[mp], rf = X.red_res[mp];
-    const double cap = S.stor_cap[p], qmean = S.mean_outflow[p];
+    const double prev = c.res, rf = c.red_res;
+    const double cap = c.stor_cap, qmean = c.mean_outflow;
     double evap, PETgwr, PETgwrMax, st, prov_rel = 0., release;
     const double c_ratio = cap / (qmean * 31536000. / 1000000000.);
     maxStorage = cap;
@@ -161,18 +243,16 @@ __device__ double chain_step(const WgStatic &S, const WgState &X, const WgTables
       st = prev + totalInflow - PETgwr;
     }
     if (fabs(st) <= kMinStorVol) st = 0.;
-    double krel = X.k_release[mp];
-    if (month == S.start_month[p] - 1) {
+    double krel = c.k_release;
+    if (month == c.start_month - 1) {
       if (day == kFirstDayInMonth[month]) {
         if (st < (cap * 0.1))
           krel = 0.1;
         else
           krel = st / (maxStorage * 0.85);
-        X.k_release[mp] = krel;
       }
     }
-    const int rtype = S.res_type[p];
-    if (rtype == 1) {
+    if (c.res_type == 1) {
       // irrigation reservoir without water use: zero demand and zero downstream use
       const double mean_demand = 0.;
       double monthlyUse = 0.0 * kDaysInMonth[month];
@@ -181,7 +261,7 @@ __device__ double chain_step(const WgStatic &S, const WgState &X, const WgTables
         prov_rel = qmean / 2. * (1. + monthlyUse / mean_demand);
       else
         prov_rel = qmean + monthlyUse - mean_demand;
-    } else if (rtype == 2) {
+    } else if (c.res_type == 2) {
       prov_rel = qmean;
     }
     if (c_ratio >= 0.5)
@@ -203,16 +283,17 @@ __device__ double chain_step(const WgStatic &S, const WgState &X, const WgTables
       outflow += st;
       st = 0.;
     }
-    X.res[mp] = st;
+    o.res = st;
+    o.k_release = krel;
     inflow = outflow;
   }
 
   // global wetland, routing.cpp:3178-3297
-  const double glo_wetland = S.glo_wetland[p];
+  const double glo_wetland = c.glo_wetland;
   if (glo_wetland > 0) {
-    const double prev = X.glo_wetl[mp], rf = X.red_gw[mp];
+    const double prev = c.glo_wetl, rf = c.red_gw;
     double evap, PETgwr, PETgwrMax, st;
-    maxStorage = ((glo_wetland) / 100.) * cellArea * S.wetl_depth[p];
+    maxStorage = ((glo_wetland) / 100.) * cellArea * c.wetl_depth;
     evap = ((1.0 - cfa) * (ow_prec * rf)) + (cfa * (ow_pet * rf));
     if (evap < 0.) evap = 0.;
     totalInflow = inflow + (ow_prec * rf * (cellArea / 1000000.) * (glo_wetland / 100.));
@@ -227,7 +308,7 @@ __device__ double chain_step(const WgStatic &S, const WgState &X, const WgTables
       outflow = 0.;
       gwr_glowet *= PETgwrMax / PETgwr;
     } else {
-      st = prev * S.e_sw[p] + S.inv_ksw[p] * (totalInflow - PETgwr) * S.ome_sw[p];
+      st = prev * c.e_sw + c.inv_ksw * (totalInflow - PETgwr) * c.ome_sw;
       outflow = totalInflow + prev - st - PETgwr;
     }
     if (st > maxStorage) {
@@ -235,17 +316,17 @@ __device__ double chain_step(const WgStatic &S, const WgState &X, const WgTables
       st = maxStorage;
     }
     if (fabs(st) <= kMinStorVol) st = 0.;
-    X.glo_wetl[mp] = st;
+    o.glo_wetl = st;
     inflow = outflow;
   }
 
   // groundwater of (semi-)arid cells, routing.cpp:3305-3386
-  double lgwrir = in.lgwrir;
+  double lgwrir = c.lgwrir;
   if (arid_swb) {
-    const double gwr_swb = in.gwr_ll + gwr_glolak + in.gwr_lw + gwr_glowet + gwr_res;
-    const double gw_prev = X.gw[mp];
-    const double netGWin = gwr_swb * cellArea * (cf / 100.) / 1000000. + in.recharge_km3;
-    double gw = gw_prev * S.e_gw[p] + S.inv_kgw[p] * netGWin * S.ome_gw[p];
+    const double gwr_swb = c.gwr_ll + gwr_glolak + c.gwr_lw + gwr_glowet + gwr_res;
+    const double gw_prev = c.gw;
+    const double netGWin = gwr_swb * cellArea * (cf / 100.) / 1000000. + c.recharge_km3;
+    double gw = gw_prev * c.e_gw + c.inv_kgw * netGWin * c.ome_gw;
     if (fabs(gw) <= kMinStorVol) gw = 0.;
     double gw_runoff = gw_prev - gw + netGWin;
     if (gw_runoff <= 0.) {
@@ -253,34 +334,34 @@ __device__ double chain_step(const WgStatic &S, const WgState &X, const WgTables
       gw = gw_prev + netGWin;
       if (fabs(gw) <= kMinStorVol) gw = 0.;
     }
-    X.gw[mp] = gw;
+    o.gw = gw;
     lgwrir = gw_runoff;
   }
 
   // river, routing.cpp:3402-3514
   double riverInflow = inflow;
   if (ldd >= 0) {
-    riverInflow += in.lrir;
+    riverInflow += c.lrir;
     riverInflow += lgwrir;
   }
-  riverInflow += in.rv_prec;
-  const double prev = X.river[mp];
-  const double evapRemUse = 0. + in.rv_evap;
-  const double evapRemUseMax = riverInflow + in.rv_term;
+  riverInflow += c.rv_prec;
+  const double prev = c.river;
+  const double evapRemUse = 0. + c.rv_evap;
+  const double evapRemUseMax = riverInflow + c.rv_term;
   double tv, st;
   if (evapRemUse > evapRemUseMax) {
     st = 0.;
     tv = riverInflow + prev - evapRemUseMax;
     if (tv < 0.) tv = 0.;
   } else {
-    st = prev * in.rv_e + in.rv_invk * (riverInflow - evapRemUse) * in.rv_ome;
+    st = prev * c.rv_e + c.rv_invk * (riverInflow - evapRemUse) * c.rv_ome;
     if (fabs(st) <= kMinStorVol) st = 0.;
     tv = riverInflow + prev - st - evapRemUse;
     if (tv < 0.) tv = 0.;
   }
-  X.river[mp] = st;
-  (void)T;
-  return tv;
+  o.river = st;
+  o.tv = tv;
+  return o;
 }
 
 // routing.cpp:7309-7341 getNewRiverVelocity
@@ -486,6 +567,66 @@ __device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p)
   X.river_frac_change[mp] = river_frac_change;
 }
 
+
+// ---- one elevation sub-cell of the snow loop, daily.cpp:920-1044 ----
+constexpr int kSnowChunk = 10;  // sub-cell pairs per double-buffered chunk
+
+struct SnowDay {
+  double tempC, gradnt, scale_num, laf, prec_to_soil, soil_pet, snowfz, snowmt, melt_factor;
+  int elev0;
+};
+
+struct SnowAcc {
+  double snow_sum, eff, evapo, change, temp_max;
+  int thresh;
+};
+
+// Updates one sub-cell storage s (elevation elev_e) and the cell's running sums in
+// sub-cell order; melt_factor = M_DEGDAY_F * ddf_lct (daily.cpp:1013).
+__device__ inline double snow_subcell(double s, int elev_e, int first, const SnowDay &d, SnowAcc &a) {
+  double temp_elev = d.tempC - ((elev_e - d.elev0) * d.gradnt);
+  double snowmelt_elev = 0., eff_before = 0.;
+  s = s * d.scale_num / d.laf;
+  if (fabs(s) <= kMinStorVol) s = 0.;
+  const double init_s = s;
+  if (s > 1000.) {
+    if (a.thresh == 0.) {
+      a.thresh = elev_e;
+    } else if (a.thresh > 0.) {
+      temp_elev = d.tempC - ((a.thresh - d.elev0) * d.gradnt);
+    }
+  }
+  if (temp_elev <= d.snowfz) {
+    s += d.prec_to_soil;
+    if (s > d.soil_pet) {
+      s -= d.soil_pet;
+      a.evapo += d.soil_pet;
+    } else {
+      a.evapo += s;
+      s = 0.;
+    }
+  } else {
+    eff_before = d.prec_to_soil;
+  }
+  if (temp_elev > d.snowmt) {
+    if (!(s < 0.)) {
+      snowmelt_elev = d.melt_factor * (temp_elev - d.snowmt);
+      if (snowmelt_elev > s) {
+        snowmelt_elev = s;
+        s = 0.;
+      } else {
+        s -= snowmelt_elev;
+      }
+    }
+  }
+  const double eff_elev = eff_before + snowmelt_elev;
+  a.change += s - init_s;
+  if (first) a.temp_max = temp_elev;
+  a.snow_sum += s;
+  a.eff += eff_elev;
+  return s;
+}
+
 }  // namespace
 
 struct WgClock {
@@ -623,77 +764,85 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   X.canopy[mp] = canopy;
   if (dailySoilPET < 0.) dailySoilPET = 0.0;
 
-  // snow on the elevation sub-cells, daily.cpp:897-1062, serial in sub-cell order
+  // snow on the elevation sub-cells, daily.cpp:897-1062, serial in sub-cell order.
+  // Sub-cells are stored in pairs ([member][pair][cell] x 2 doubles, elevations as
+  // packed int16 pairs) and streamed in double-buffered chunks so that a
+  // wavefront keeps many 16-byte loads in flight.
   const double T_SNOWFZ = S.t_snowfz[p], T_SNOWMT = S.t_snowmt[p];
   const double T_GRADNT = S.t_gradnt[p], DEGDAY_F = S.degday[p];
   const double ddf = T.ddf[lct - 1];
-  double TempElevMax = 0., snowStorageChange = 0., snow_sum = 0.;
-  int thresh_elev = 0;
-  const int elev0 = S.elevation[p];
-  double *se = X.snow_elev + int64_t(m) * WG_DEV_NSUB * n + p;
-  const int16_t *ev = S.elevation + n + p;
+  SnowAcc acc;
+  acc.snow_sum = 0.;
+  acc.eff = 0.;
+  acc.evapo = 0.;
+  acc.change = 0.;
+  acc.temp_max = 0.;
+  acc.thresh = 0;
+  const int elev0 = S.elev0[p];
+  WgD2 *sp = X.snow2 + int64_t(m) * WG_DEV_NPAIR * n + p;
+  const int32_t *ep = S.elev_pairs + p;
   if (landAreaFrac <= 0.) {
-    for (int e = 0; e < WG_DEV_NSUB; e++) {
-      transfer += se[int64_t(e) * n] / 100.;
-      se[int64_t(e) * n] = 0.;
+    for (int k = 0; k < WG_DEV_NPAIR; k++) {
+      const WgD2 v = sp[int64_t(k) * n];
+      transfer += v.x / 100.;
+      transfer += v.y / 100.;
+      sp[int64_t(k) * n] = WgD2{0., 0.};
     }
-    snow_sum = 0.;
   } else {
-    const double scale_num = lafPrev;
-#pragma unroll 4
-    for (int e = 0; e < WG_DEV_NSUB; e++) {
-      const int elev_e = ev[int64_t(e) * n];
-      double s = se[int64_t(e) * n];
-      double temp_elev = dailyTempC - ((elev_e - elev0) * T_GRADNT);
-      double daily_snow_to_soil_elev = 0., snowmelt_elev = 0., eff_before = 0.;
-      s = s * scale_num / landAreaFrac;
-      if (fabs(s) <= kMinStorVol) s = 0.;
-      const double init_s = s;
-      if (s > 1000.) {
-        if (thresh_elev == 0.) {
-          thresh_elev = elev_e;
-        } else if (thresh_elev > 0.) {
-          temp_elev = dailyTempC - ((thresh_elev - elev0) * T_GRADNT);
-        }
-      }
-      if (temp_elev <= T_SNOWFZ) {
-        daily_snow_to_soil_elev = daily_prec_to_soil;
-        s += daily_snow_to_soil_elev;
-        if (s > dailySoilPET) {
-          s -= dailySoilPET;
-          dailySnowEvapo += dailySoilPET;
-        } else {
-          dailySnowEvapo += s;
-          s = 0.;
-        }
-      } else {
-        eff_before = daily_prec_to_soil;
-      }
-      if (temp_elev > T_SNOWMT) {
-        if (!(s < 0.)) {
-          snowmelt_elev = DEGDAY_F * ddf * (temp_elev - T_SNOWMT);
-          if (snowmelt_elev > s) {
-            snowmelt_elev = s;
-            s = 0.;
-          } else {
-            s -= snowmelt_elev;
-          }
-        }
-      }
-      const double eff_elev = eff_before + snowmelt_elev;
-      snowStorageChange += s - init_s;
-      if (e == 0) TempElevMax = temp_elev;
-      snow_sum += s;
-      dailyEffPrec += eff_elev;
-      se[int64_t(e) * n] = s;
-      (void)daily_snow_to_soil_elev;
+    SnowDay sd;
+    sd.tempC = dailyTempC;
+    sd.elev0 = elev0;
+    sd.gradnt = T_GRADNT;
+    sd.scale_num = lafPrev;
+    sd.laf = landAreaFrac;
+    sd.prec_to_soil = daily_prec_to_soil;
+    sd.soil_pet = dailySoilPET;
+    sd.snowfz = T_SNOWFZ;
+    sd.snowmt = T_SNOWMT;
+    sd.melt_factor = DEGDAY_F * ddf;
+    WgD2 sa[kSnowChunk], sb[kSnowChunk];
+    int32_t ea[kSnowChunk], eb[kSnowChunk];
+#pragma unroll
+    for (int j = 0; j < kSnowChunk; j++) {
+      sa[j] = sp[int64_t(j) * n];
+      ea[j] = ep[int64_t(j) * n];
     }
-    snow_sum /= 100.;
-    dailyEffPrec /= 100.;
+#pragma unroll 1
+    for (int c = 0; c < WG_DEV_NPAIR / kSnowChunk; c++) {
+      const bool more = c + 1 < WG_DEV_NPAIR / kSnowChunk;
+      if (more) {
+#pragma unroll
+        for (int j = 0; j < kSnowChunk; j++) {
+          sb[j] = sp[int64_t((c + 1) * kSnowChunk + j) * n];
+          eb[j] = ep[int64_t((c + 1) * kSnowChunk + j) * n];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kSnowChunk; j++) {
+        const int32_t e2 = ea[j];
+        WgD2 v = sa[j];
+        v.x = snow_subcell(v.x, int(int16_t(e2 & 0xffff)), c == 0 && j == 0, sd, acc);
+        v.y = snow_subcell(v.y, int(int16_t(e2 >> 16)), 0, sd, acc);
+        sp[int64_t(c * kSnowChunk + j) * n] = v;
+      }
+      if (more) {
+#pragma unroll
+        for (int j = 0; j < kSnowChunk; j++) {
+          sa[j] = sb[j];
+          ea[j] = eb[j];
+        }
+      }
+    }
+    dailySnowEvapo = acc.evapo;
+    acc.snow_sum /= 100.;
+    acc.eff /= 100.;
     dailySnowEvapo /= 100.;
-    snowStorageChange /= 100.;
-    landStorageChangeSum += snowStorageChange;
+    acc.change /= 100.;
+    landStorageChangeSum += acc.change;
   }
+  const double snow_sum = acc.snow_sum;
+  const double TempElevMax = acc.temp_max;
+  dailyEffPrec = acc.eff;
   X.snow[mp] = snow_sum;
 
   // immediate runoff, daily.cpp:1066-1071
@@ -934,15 +1083,15 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   }
 
   // river terms that do not depend on the inflow, routing.cpp:3424-3479
-  ChainIn in;
-  in.inflow = inflow;
-  in.lrir = lrir;
-  in.lgwrir = lgwrir;
-  in.gwr_ll = gwr_loclak;
-  in.gwr_lw = gwr_locwet;
-  in.ow_prec = ow_prec;
-  in.ow_pet = ow_pet;
-  in.recharge_km3 = recharge_km3;
+  ChainCell c;
+  c.inflow = inflow;
+  c.lrir = lrir;
+  c.lgwrir = lgwrir;
+  c.gwr_ll = gwr_loclak;
+  c.gwr_lw = gwr_locwet;
+  c.ow_prec = ow_prec;
+  c.ow_pet = ow_pet;
+  c.recharge_km3 = recharge_km3;
   {
     const double L = S.river_length[p];
     const double prev = X.river[mp];
@@ -951,91 +1100,132 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     const double K = velocity / L;
     const double eK = wg_exp(-1. * K);
     const double riverAreaFrac = X.river_frac_next[mp];
-    in.rv_e = eK;
-    in.rv_invk = 1. / K;
-    in.rv_ome = 1. - eK;
-    in.rv_term = (K * prev * eK) / (1. - eK);
-    in.rv_evap = ((1.0 - cfa) * (ow_prec) + (cfa * ow_pet)) * riverAreaFrac / 100. * cellArea / 1000000.;
-    in.rv_prec = ow_prec * riverAreaFrac / 100. * cellArea / 1000000.;
+    c.rv_e = eK;
+    c.rv_invk = 1. / K;
+    c.rv_ome = 1. - eK;
+    c.rv_term = (K * prev * eK) / (1. - eK);
+    c.rv_evap = ((1.0 - cfa) * (ow_prec) + (cfa * ow_pet)) * riverAreaFrac / 100. * cellArea / 1000000.;
+    c.rv_prec = ow_prec * riverAreaFrac / 100. * cellArea / 1000000.;
   }
   const int n_up = S.up_off[p + 1] - S.up_off[p];
   if (n_up == 0) {
     // no upstream cell: G_riverInflow[n] is still 0 when the cell is routed
-    const double tv = chain_step(S, X, T, mp, p, in, 0.0, doy, month);
-    X.tv[mp] = tv;
+    load_chain_statics(S, p, c);
+    load_chain_state(X, mp, c);
+    const ChainOut o = chain_step(c, 0.0, doy, month);
+    store_chain(X, mp, c, o);
     const int st = S.station[p];
     if (st >= 0) {
-      const double q = (ldd >= 0) ? tv : 0.;
+      const double q = (ldd >= 0) ? o.tv : 0.;
       X.st_sum[int64_t(m) * S.n_stations + st] += q;
       X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + st] = q;
     }
   } else {
-    X.c_inflow[mp] = in.inflow;
-    X.c_lrir[mp] = in.lrir;
-    X.c_lgwrir[mp] = in.lgwrir;
-    X.c_gwr_ll[mp] = in.gwr_ll;
-    X.c_gwr_lw[mp] = in.gwr_lw;
-    X.c_ow_prec[mp] = in.ow_prec;
-    X.c_ow_pet[mp] = in.ow_pet;
-    X.c_recharge_km3[mp] = in.recharge_km3;
-    X.c_rv_e[mp] = in.rv_e;
-    X.c_rv_invk[mp] = in.rv_invk;
-    X.c_rv_ome[mp] = in.rv_ome;
-    X.c_rv_term[mp] = in.rv_term;
-    X.c_rv_evap[mp] = in.rv_evap;
-    X.c_rv_prec[mp] = in.rv_prec;
+    X.c_inflow[mp] = c.inflow;
+    X.c_lrir[mp] = c.lrir;
+    X.c_lgwrir[mp] = c.lgwrir;
+    X.c_gwr_ll[mp] = c.gwr_ll;
+    X.c_gwr_lw[mp] = c.gwr_lw;
+    X.c_ow_prec[mp] = c.ow_prec;
+    X.c_ow_pet[mp] = c.ow_pet;
+    X.c_recharge_km3[mp] = c.recharge_km3;
+    X.c_rv_e[mp] = c.rv_e;
+    X.c_rv_invk[mp] = c.rv_invk;
+    X.c_rv_ome[mp] = c.rv_ome;
+    X.c_rv_term[mp] = c.rv_term;
+    X.c_rv_evap[mp] = c.rv_evap;
+    X.c_rv_prec[mp] = c.rv_prec;
   }
 }
 
-// One workgroup per (routing group, member): levels 1..depth of the group.
-__global__ __launch_bounds__(1024) void wg_chain_kernel(WgStatic S, WgState X, WgTables T,
+// Inputs of one chain cell, prefetched a level ahead.
+struct ChainSlot {
+  ChainCell c;
+  int p, u0, nu, contcell, station;
+  int up[3];
+};
+
+__device__ inline void prefetch_slot(const WgStatic &S, const WgState &X, int64_t base, int p, ChainSlot &q) {
+  const int64_t mp = base + p;
+  q.p = p;
+  q.contcell = S.contcell[p];
+  q.station = S.station[p];
+  q.u0 = S.up_off[p];
+  q.nu = S.up_off[p + 1] - q.u0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) q.up[k] = (k < q.nu) ? S.up_idx[q.u0 + k] : 0;
+  load_chain_record(X, mp, q.c);
+  load_chain_statics(S, p, q.c);
+  load_chain_state(X, mp, q.c);
+}
+
+// One workgroup per (routing group, member): levels 1..depth of the group.  The
+// transported volumes of the group live in LDS (level 0 is copied in from the
+// cell kernel's output); up_idx holds group-local indices.
+__global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, WgTables T,
                                                         const int32_t *__restrict__ group_level_off,
                                                         const int32_t *__restrict__ group_first_level,
                                                         const WgClock *__restrict__ clock, int k,
                                                         int daily_slot) {
+  extern __shared__ double s_tv[];
   const int g = blockIdx.x;
   const int m = blockIdx.y;
   const int n = S.n;
+  const int tid = threadIdx.x, nt = blockDim.x;
   const int doy = int((clock->doy0 - 1 + k) % 365) + 1;
   const int month = month_of_doy(doy);
   const int l0 = group_first_level[g], l1 = group_first_level[g + 1];
+  const int gs = group_level_off[l0], ge = group_level_off[l1];
+  const int lvl0_end = group_level_off[l0 + 1];
   const int64_t base = int64_t(m) * n;
-  // level 0 of every group was routed by the cell kernel
+  // level 0 was routed by the cell kernel; cells not routed today read as 0
+  for (int i = gs + tid; i < ge; i += nt) s_tv[i - gs] = (i < lvl0_end) ? X.tv[base + i] : 0.;
+  ChainSlot next;
+  bool have_next = false;
+  if (l0 + 1 < l1) {
+    const int p = group_level_off[l0 + 1] + tid;
+    if (p < group_level_off[l0 + 2]) {
+      prefetch_slot(S, X, base, p, next);
+      have_next = true;
+    }
+  }
+  __syncthreads();
   for (int l = l0 + 1; l < l1; l++) {
     const int beg = group_level_off[l], end = group_level_off[l + 1];
-    for (int p = beg + int(threadIdx.x); p < end; p += blockDim.x) {
-      if (!S.contcell[p]) continue;
-      const int64_t mp = base + p;
-      ChainIn in;
-      in.inflow = X.c_inflow[mp];
-      in.lrir = X.c_lrir[mp];
-      in.lgwrir = X.c_lgwrir[mp];
-      in.gwr_ll = X.c_gwr_ll[mp];
-      in.gwr_lw = X.c_gwr_lw[mp];
-      in.ow_prec = X.c_ow_prec[mp];
-      in.ow_pet = X.c_ow_pet[mp];
-      in.recharge_km3 = X.c_recharge_km3[mp];
-      in.rv_e = X.c_rv_e[mp];
-      in.rv_invk = X.c_rv_invk[mp];
-      in.rv_ome = X.c_rv_ome[mp];
-      in.rv_term = X.c_rv_term[mp];
-      in.rv_evap = X.c_rv_evap[mp];
-      in.rv_prec = X.c_rv_prec[mp];
+    bool first = true;
+    for (int p = beg + tid; p < end; p += nt) {
+      ChainSlot cur;
+      if (first && have_next) {
+        cur = next;
+      } else {
+        prefetch_slot(S, X, base, p, cur);
+      }
+      first = false;
+      if (!cur.contcell) continue;
       // ordered gather: the serial += of routing.cpp:3955-3958 in routing order
       double up = 0.;
-      const int u0 = S.up_off[p], u1 = S.up_off[p + 1];
-      for (int u = u0; u < u1; u++) up += X.tv[base + S.up_idx[u]];
-      const double tv = chain_step(S, X, T, mp, p, in, up, doy, month);
-      X.tv[mp] = tv;
-      const int st = S.station[p];
-      if (st >= 0) {
-        const double q = (S.ldd[p] >= 0) ? tv : 0.;
-        X.st_sum[int64_t(m) * S.n_stations + st] += q;
-        X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + st] = q;
+      for (int u = 0; u < cur.nu; u++) up += s_tv[(u < 3) ? cur.up[u] : S.up_idx[cur.u0 + u]];
+      const ChainOut o = chain_step(cur.c, up, doy, month);
+      s_tv[p - gs] = o.tv;
+      store_chain(X, base + p, cur.c, o);
+      if (cur.station >= 0) {
+        const double q = (cur.c.ldd >= 0) ? o.tv : 0.;
+        X.st_sum[int64_t(m) * S.n_stations + cur.station] += q;
+        X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + cur.station] = q;
+      }
+    }
+    // inputs of the next level do not depend on this level: load them before the barrier
+    have_next = false;
+    if (l + 1 < l1) {
+      const int p = group_level_off[l + 1] + tid;
+      if (p < group_level_off[l + 2]) {
+        prefetch_slot(S, X, base, p, next);
+        have_next = true;
       }
     }
     __syncthreads();
   }
+  (void)T;
 }
 
 // Sets the day clock read by the kernels of a captured block of days; launched
