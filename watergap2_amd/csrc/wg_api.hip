@@ -487,6 +487,9 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     UP(d, e_gw); S.e_gw = d;
     UP(d, inv_kgw); S.inv_kgw = d;
     UP(d, ome_gw); S.ome_gw = d;
+    std::vector<double> cf100(n);
+    for (int32_t i = 0; i < n; i++) cf100[i] = st->contfreq[i] / 100.;
+    UP(d, cf100); S.cf100 = d;
   }
   {
     int8_t *d8;
@@ -494,6 +497,18 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     UPR(d8, st->land_cover); S.land_cover = d8;
     UPR(d8, st->texture); S.texture = d8;
     UP(d8, contcell); S.contcell = d8;
+    std::vector<int8_t> cflags(n);
+    for (int32_t i = 0; i < n; i++) {
+      int f = 0;
+      if (st->lake_area[i] > 0.) f |= CFL_LAKE;
+      if (st->reservoir_area[i] > 0.) f |= CFL_RES;
+      if (st->glo_wetland[i] > 0) f |= CFL_GWET;
+      if (st->aindex[i] == 1 && st->ldd[i] >= 0) f |= CFL_ARID;
+      if (st->ldd[i] >= 0) f |= CFL_LDD;
+      if (contcell[i]) f |= CFL_CONT;
+      cflags[i] = int8_t(f);
+    }
+    UP(d8, cflags); S.cflags = d8;
     UPR(d8, st->res_type); S.res_type = d8;
     UPR(d8, st->start_month); S.start_month = d8;
     int16_t *d16;
@@ -568,15 +583,14 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
   double **dstate[] = {&X.canopy, &X.soil, &X.snow, &X.transfer, &X.lai_precsum, &X.gw, &X.loc_lake, &X.loc_wetl,
                        &X.glo_lake, &X.glo_wetl, &X.res, &X.river, &X.red_ll, &X.red_lw, &X.red_gl, &X.red_gw,
                        &X.red_res, &X.red_river, &X.river_frac_next, &X.river_frac_change, &X.fswb_next, &X.laf,
-                       &X.laf_prev, &X.k_release, &X.gamma, &X.cfa, &X.tv, &X.c_inflow, &X.c_lrir, &X.c_lgwrir,
-                       &X.c_gwr_ll, &X.c_gwr_lw, &X.c_ow_prec, &X.c_ow_pet, &X.c_recharge_km3, &X.c_rv_e,
-                       &X.c_rv_invk, &X.c_rv_ome, &X.c_rv_term, &X.c_rv_evap, &X.c_rv_prec, &X.d_surf,
+                       &X.laf_prev, &X.k_release, &X.gamma, &X.cfa, &X.tv, &X.d_surf,
                        &X.d_recharge, &X.d_ow_pet, &X.d_aet, &X.d_prec};
   for (double **p : dstate) {
     rc = dev_alloc(H, p, NM);
     if (rc) { *out = h.release(); return rc; }
   }
-  rc = dev_alloc(H, &X.lai_days, NM);
+  rc = dev_alloc(H, &X.crec, NM * CF_COUNT);
+  if (!rc) rc = dev_alloc(H, &X.lai_days, NM);
   if (!rc) rc = dev_alloc(H, &X.lai_status, NM);
   if (!rc) rc = dev_alloc(H, &X.smax, NM);
   if (!rc) rc = dev_alloc(H, &X.snow2, NM * WG_DEV_NPAIR);

@@ -46,6 +46,8 @@ struct WgStatic {
   const int8_t *res_type, *start_month;
   // exp(-k) terms of the surface-water and groundwater outflow coefficients
   const double *k_sw, *e_sw, *inv_ksw, *ome_sw, *k_gw, *e_gw, *inv_kgw, *ome_gw;
+  const double *cf100;              // contfreq / 100
+  const int8_t *cflags;             // lake / reservoir / global wetland / arid / LDD >= 0 / continental
   // schedule
   const int32_t *up_off, *up_idx;   // CSR of upstream cells sorted by routing order
   const int32_t *station;           // station slot of each cell, -1 = none
@@ -63,9 +65,9 @@ struct WgState {
   double *gamma, *cfa;
   float *smax;
   double *tv;                   // transported volume of the day
-  // chain record written by the cell kernel for cells with upstream inflow
-  double *c_inflow, *c_lrir, *c_lgwrir, *c_gwr_ll, *c_gwr_lw, *c_ow_prec, *c_ow_pet,
-      *c_recharge_km3, *c_rv_e, *c_rv_invk, *c_rv_ome, *c_rv_term, *c_rv_evap, *c_rv_prec;
+  // routing record written by the cell kernel for cells with upstream inflow:
+  // [field][member][N], see ChainField
+  double *crec;
   // diagnostics of the last day
   double *d_surf, *d_recharge, *d_ow_pet, *d_aet, *d_prec;
   // stations

@@ -75,135 +75,87 @@ __device__ double irwin_hall(uint64_t seed, uint32_t cell, uint32_t day, uint32_
 
 // ---- the upstream-dependent part of one routing cell ----
 
-// Everything routing.cpp:2623-3958 reads for one cell, loaded up front so that
-// the loads of the next level can be issued before a workgroup barrier.
-struct ChainCell {
-  // record written by the cell kernel
-  double inflow;        // outflow of the local lake/wetland system (routing.cpp:2616)
-  double lrir, lgwrir;  // G_localRunoffIntoRiver, G_localGWRunoffIntoRiver (humid cells)
-  double gwr_ll, gwr_lw;
-  double ow_prec, ow_pet;
-  double recharge_km3;  // arid cells: recharge term of netGWin (routing.cpp:3320-3321)
-  double rv_e, rv_invk, rv_ome, rv_term, rv_evap, rv_prec;
-  // statics
-  double cellArea, cf, lake_area, lake_depth, res_area, stor_cap, mean_outflow, glo_wetland, wetl_depth;
-  double e_sw, inv_ksw, ome_sw, e_gw, inv_kgw, ome_gw;
-  int ldd, aindex, start_month, res_type;
-  // state of the previous day
-  double cfa, glo_lake, red_gl, res, red_res, k_release, glo_wetl, red_gw, gw, river;
+// Record of one routing cell: every term of routing.cpp:2623-3958 that does not
+// depend on the water arriving from upstream, computed by the cell kernel with
+// the reference's own expressions, so that the level walk only evaluates the
+// short chain that does.  Stored field-major: crec[field][member][cell].
+enum ChainField {
+  CF_INFLOW = 0,  // outflow of the local lake/wetland system (routing.cpp:2616)
+  CF_LRIR,        // G_localRunoffIntoRiver
+  CF_LGWRIR,      // G_localGWRunoffIntoRiver of humid cells
+  CF_RV_PREC,     // precipitation on the river (routing.cpp:3457)
+  CF_RV_TERM,     // K S0 e^-K / (1 - e^-K) (routing.cpp:3477-3478)
+  CF_RV_EVAP,     // river evaporation (routing.cpp:3449-3453)
+  CF_RV_S0E, CF_RV_INVK, CF_RV_OME, CF_RV_PREV,
+  CF_GL_LP,       // precipitation on the global lake (routing.cpp:2664)
+  CF_GL_USE, CF_GL_MAX, CF_GL_PREV, CF_GL_S0E, CF_GL_GWR,
+  CF_GW_WP,       // precipitation on the global wetland (routing.cpp:3212-3213)
+  CF_GW_PET, CF_GW_MAX, CF_GW_PREV, CF_GW_S0E, CF_GW_GWR,
+  CF_A_GWR_LL, CF_A_GWR_LW, CF_A_RECH, CF_A_PREV, CF_A_S0E,
+  CF_OW_PREC, CF_OW_PET,
+  CF_COUNT
+};
+
+// chain flags (static per cell)
+enum {
+  CFL_LAKE = 1, CFL_RES = 2, CFL_GWET = 4, CFL_ARID = 8, CFL_LDD = 16, CFL_CONT = 32
+};
+
+struct ChainRec {
+  double v[CF_COUNT];
+  double inv_ksw, ome_sw, inv_kgw, ome_gw, area, cf100;
+  int flags;
 };
 
 struct ChainOut {
   double glo_lake, res, k_release, glo_wetl, gw, river, tv;
 };
 
-__device__ inline void load_chain_statics(const WgStatic &S, int p, ChainCell &c) {
-  c.cellArea = S.cell_area[p];
-  c.cf = S.contfreq[p];
-  c.lake_area = S.lake_area[p];
-  c.lake_depth = S.lake_depth[p];
-  c.res_area = S.reservoir_area[p];
-  c.stor_cap = S.stor_cap[p];
-  c.mean_outflow = S.mean_outflow[p];
-  c.glo_wetland = S.glo_wetland[p];
-  c.wetl_depth = S.wetl_depth[p];
-  c.e_sw = S.e_sw[p];
-  c.inv_ksw = S.inv_ksw[p];
-  c.ome_sw = S.ome_sw[p];
-  c.e_gw = S.e_gw[p];
-  c.inv_kgw = S.inv_kgw[p];
-  c.ome_gw = S.ome_gw[p];
-  c.ldd = S.ldd[p];
-  c.aindex = S.aindex[p];
-  c.start_month = S.start_month[p];
-  c.res_type = S.res_type[p];
+__device__ inline void load_chain_rec(const WgStatic &S, const WgState &X, int64_t mp, int p, int64_t stride,
+                                      ChainRec &r) {
+#pragma unroll
+  for (int f = 0; f < CF_COUNT; f++) r.v[f] = X.crec[f * stride + mp];
+  r.inv_ksw = S.inv_ksw[p];
+  r.ome_sw = S.ome_sw[p];
+  r.inv_kgw = S.inv_kgw[p];
+  r.ome_gw = S.ome_gw[p];
+  r.area = S.cell_area[p];
+  r.cf100 = S.cf100[p];
+  r.flags = S.cflags[p];
 }
 
-__device__ inline void load_chain_state(const WgState &X, int64_t mp, ChainCell &c) {
-  c.cfa = X.cfa[mp];
-  c.glo_lake = X.glo_lake[mp];
-  c.red_gl = X.red_gl[mp];
-  c.res = X.res[mp];
-  c.red_res = X.red_res[mp];
-  c.k_release = X.k_release[mp];
-  c.glo_wetl = X.glo_wetl[mp];
-  c.red_gw = X.red_gw[mp];
-  c.gw = X.gw[mp];
-  c.river = X.river[mp];
-}
-
-__device__ inline void load_chain_record(const WgState &X, int64_t mp, ChainCell &c) {
-  c.inflow = X.c_inflow[mp];
-  c.lrir = X.c_lrir[mp];
-  c.lgwrir = X.c_lgwrir[mp];
-  c.gwr_ll = X.c_gwr_ll[mp];
-  c.gwr_lw = X.c_gwr_lw[mp];
-  c.ow_prec = X.c_ow_prec[mp];
-  c.ow_pet = X.c_ow_pet[mp];
-  c.recharge_km3 = X.c_recharge_km3[mp];
-  c.rv_e = X.c_rv_e[mp];
-  c.rv_invk = X.c_rv_invk[mp];
-  c.rv_ome = X.c_rv_ome[mp];
-  c.rv_term = X.c_rv_term[mp];
-  c.rv_evap = X.c_rv_evap[mp];
-  c.rv_prec = X.c_rv_prec[mp];
-}
-
-// Writes what chain_step changed.
-__device__ inline void store_chain(const WgState &X, int64_t mp, const ChainCell &c, const ChainOut &o) {
-  if (c.lake_area > 0.) X.glo_lake[mp] = o.glo_lake;
-  if (c.res_area > 0.) {
-    X.res[mp] = o.res;
-    X.k_release[mp] = o.k_release;
-  }
-  if (c.glo_wetland > 0) X.glo_wetl[mp] = o.glo_wetl;
-  if ((c.aindex == 1) && (c.ldd >= 0)) X.gw[mp] = o.gw;
-  X.river[mp] = o.river;
-  X.tv[mp] = o.tv;
-}
-
-// routing.cpp:2623-3958 for one cell (parity profile: no water use, no station
-// correction).  inflow_up is G_riverInflow[n] when the cell is reached.
-__device__ ChainOut chain_step(const ChainCell &c, double inflow_up, int day, int month) {
+// routing.cpp:2623-3958 for one cell from its record (parity profile: no water
+// use, no station correction).  inflow_up is G_riverInflow[n] when the cell is
+// reached.  The reservoir branch (about one cell in two hundred) reads its own
+// inputs here.
+__device__ ChainOut chain_step(const WgStatic &S, const WgState &X, const ChainRec &c, int64_t mp, int p,
+                               double inflow_up, int day, int month) {
   ChainOut o;
-  const double cellArea = c.cellArea;
-  const double cf = c.cf;
-  const double cfa = c.cfa;
-  const int ldd = c.ldd;
-  const bool arid_swb = (c.aindex == 1) && (ldd >= 0);
-  const double ow_prec = c.ow_prec, ow_pet = c.ow_pet;
-  double inflow = c.inflow;
-  double outflow = 0., maxStorage, totalInflow;
-  double gwr_glolak = 0., gwr_glowet = 0., gwr_res = 0.;
-  o.glo_lake = c.glo_lake;
-  o.res = c.res;
-  o.k_release = c.k_release;
-  o.glo_wetl = c.glo_wetl;
-  o.gw = c.gw;
+  const double *v = c.v;
+  const bool arid_swb = (c.flags & CFL_ARID) != 0;
+  double inflow = v[CF_INFLOW];
+  double outflow = 0.;
+  double gwr_glolak = v[CF_GL_GWR], gwr_glowet = v[CF_GW_GWR], gwr_res = 0.;
+  o.glo_lake = v[CF_GL_PREV];
+  o.glo_wetl = v[CF_GW_PREV];
+  o.gw = v[CF_A_PREV];
+  o.res = 0.;
+  o.k_release = 0.;
 
   inflow += inflow_up;  // routing.cpp:2623
 
   // global lake, routing.cpp:2630-2804
-  const double lake_area = c.lake_area;
-  if (lake_area > 0.) {
-    const double prev = c.glo_lake, rf = c.red_gl;
-    double use, useMax, evap, st;
-    maxStorage = lake_area * c.lake_depth;
-    evap = ((1.0 - cfa) * ow_prec) + (cfa * ow_pet * rf);
-    if (evap < 0.) evap = 0.;
-    totalInflow = inflow + (ow_prec * (lake_area / 1000000.));
-    if (arid_swb)
-      gwr_glolak = kKswbgw * rf * (lake_area / (cellArea * (cf / 100.)));
-    else
-      gwr_glolak = 0.;
-    use = evap * (lake_area / 1000000.) + gwr_glolak * cellArea * (cf / 100.) / 1000000. + 0.;
-    useMax = totalInflow + maxStorage + prev;
+  if (c.flags & CFL_LAKE) {
+    const double prev = v[CF_GL_PREV], use = v[CF_GL_USE], maxStorage = v[CF_GL_MAX];
+    const double totalInflow = inflow + v[CF_GL_LP];
+    const double useMax = totalInflow + maxStorage + prev;
+    double st;
     if (use > useMax) {
       st = (-1.) * maxStorage;
       outflow = 0.;
       gwr_glolak *= useMax / use;
     } else {
-      st = prev * c.e_sw + c.inv_ksw * (totalInflow - use) * c.ome_sw;
+      st = v[CF_GL_S0E] + c.inv_ksw * (totalInflow - use) * c.ome_sw;
       outflow = totalInflow + prev - st - use;
       if (st > maxStorage) {
         outflow += (st - maxStorage);
@@ -220,16 +172,18 @@ __device__ ChainOut chain_step(const ChainCell &c, double inflow_up, int day, in
   }
 
   // reservoir, routing.cpp:2807-3082
-  const double res_area = c.res_area;
-  if (res_area > 0.) {
-    const double prev = c.res, rf = c.red_res;
-    const double cap = c.stor_cap, qmean = c.mean_outflow;
+  if (c.flags & CFL_RES) {
+    const double res_area = S.reservoir_area[p];
+    const double prev = X.res[mp], rf = X.red_res[mp], cfa = X.cfa[mp];
+    const double cap = S.stor_cap[p], qmean = S.mean_outflow[p];
+    const double ow_prec = v[CF_OW_PREC], ow_pet = v[CF_OW_PET];
+    const double cellArea = c.area, cf = S.contfreq[p];
     double evap, PETgwr, PETgwrMax, st, prov_rel = 0., release;
     const double c_ratio = cap / (qmean * 31536000. / 1000000000.);
-    maxStorage = cap;
+    const double maxStorage = cap;
     evap = ((1.0 - cfa) * ow_prec) + (cfa * (ow_pet * rf));
     if (evap < 0.) evap = 0.;
-    totalInflow = inflow + (ow_prec * (res_area / 1000000.));
+    const double totalInflow = inflow + (ow_prec * (res_area / 1000000.));
     if (arid_swb)
       gwr_res = kKswbgw * rf * (res_area / (cellArea * (cf / 100.)));
     else
@@ -243,8 +197,8 @@ __device__ ChainOut chain_step(const ChainCell &c, double inflow_up, int day, in
       st = prev + totalInflow - PETgwr;
     }
     if (fabs(st) <= kMinStorVol) st = 0.;
-    double krel = c.k_release;
-    if (month == c.start_month - 1) {
+    double krel = X.k_release[mp];
+    if (month == S.start_month[p] - 1) {
       if (day == kFirstDayInMonth[month]) {
         if (st < (cap * 0.1))
           krel = 0.1;
@@ -252,7 +206,8 @@ __device__ ChainOut chain_step(const ChainCell &c, double inflow_up, int day, in
           krel = st / (maxStorage * 0.85);
       }
     }
-    if (c.res_type == 1) {
+    const int rtype = S.res_type[p];
+    if (rtype == 1) {
       // irrigation reservoir without water use: zero demand and zero downstream use
       const double mean_demand = 0.;
       double monthlyUse = 0.0 * kDaysInMonth[month];
@@ -261,7 +216,7 @@ __device__ ChainOut chain_step(const ChainCell &c, double inflow_up, int day, in
         prov_rel = qmean / 2. * (1. + monthlyUse / mean_demand);
       else
         prov_rel = qmean + monthlyUse - mean_demand;
-    } else if (c.res_type == 2) {
+    } else if (rtype == 2) {
       prov_rel = qmean;
     }
     if (c_ratio >= 0.5)
@@ -289,26 +244,17 @@ __device__ ChainOut chain_step(const ChainCell &c, double inflow_up, int day, in
   }
 
   // global wetland, routing.cpp:3178-3297
-  const double glo_wetland = c.glo_wetland;
-  if (glo_wetland > 0) {
-    const double prev = c.glo_wetl, rf = c.red_gw;
-    double evap, PETgwr, PETgwrMax, st;
-    maxStorage = ((glo_wetland) / 100.) * cellArea * c.wetl_depth;
-    evap = ((1.0 - cfa) * (ow_prec * rf)) + (cfa * (ow_pet * rf));
-    if (evap < 0.) evap = 0.;
-    totalInflow = inflow + (ow_prec * rf * (cellArea / 1000000.) * (glo_wetland / 100.));
-    if (arid_swb)
-      gwr_glowet = kKswbgw * rf * glo_wetland / 100. / (cf / 100.);
-    else
-      gwr_glowet = 0.;
-    PETgwr = evap * (cellArea / 1000000.) * ((glo_wetland) / 100.) + gwr_glowet * cellArea * (cf / 100.) / 1000000.;
-    PETgwrMax = totalInflow + prev;
+  if (c.flags & CFL_GWET) {
+    const double prev = v[CF_GW_PREV], PETgwr = v[CF_GW_PET], maxStorage = v[CF_GW_MAX];
+    const double totalInflow = inflow + v[CF_GW_WP];
+    const double PETgwrMax = totalInflow + prev;
+    double st;
     if (PETgwr > PETgwrMax) {
       st = 0.;
       outflow = 0.;
       gwr_glowet *= PETgwrMax / PETgwr;
     } else {
-      st = prev * c.e_sw + c.inv_ksw * (totalInflow - PETgwr) * c.ome_sw;
+      st = v[CF_GW_S0E] + c.inv_ksw * (totalInflow - PETgwr) * c.ome_sw;
       outflow = totalInflow + prev - st - PETgwr;
     }
     if (st > maxStorage) {
@@ -321,12 +267,12 @@ __device__ ChainOut chain_step(const ChainCell &c, double inflow_up, int day, in
   }
 
   // groundwater of (semi-)arid cells, routing.cpp:3305-3386
-  double lgwrir = c.lgwrir;
+  double lgwrir = v[CF_LGWRIR];
   if (arid_swb) {
-    const double gwr_swb = c.gwr_ll + gwr_glolak + c.gwr_lw + gwr_glowet + gwr_res;
-    const double gw_prev = c.gw;
-    const double netGWin = gwr_swb * cellArea * (cf / 100.) / 1000000. + c.recharge_km3;
-    double gw = gw_prev * c.e_gw + c.inv_kgw * netGWin * c.ome_gw;
+    const double gwr_swb = v[CF_A_GWR_LL] + gwr_glolak + v[CF_A_GWR_LW] + gwr_glowet + gwr_res;
+    const double gw_prev = v[CF_A_PREV];
+    const double netGWin = gwr_swb * c.area * c.cf100 / 1000000. + v[CF_A_RECH];
+    double gw = v[CF_A_S0E] + c.inv_kgw * netGWin * c.ome_gw;
     if (fabs(gw) <= kMinStorVol) gw = 0.;
     double gw_runoff = gw_prev - gw + netGWin;
     if (gw_runoff <= 0.) {
@@ -340,21 +286,21 @@ __device__ ChainOut chain_step(const ChainCell &c, double inflow_up, int day, in
 
   // river, routing.cpp:3402-3514
   double riverInflow = inflow;
-  if (ldd >= 0) {
-    riverInflow += c.lrir;
+  if (c.flags & CFL_LDD) {
+    riverInflow += v[CF_LRIR];
     riverInflow += lgwrir;
   }
-  riverInflow += c.rv_prec;
-  const double prev = c.river;
-  const double evapRemUse = 0. + c.rv_evap;
-  const double evapRemUseMax = riverInflow + c.rv_term;
+  riverInflow += v[CF_RV_PREC];
+  const double prev = v[CF_RV_PREV];
+  const double evapRemUse = 0. + v[CF_RV_EVAP];
+  const double evapRemUseMax = riverInflow + v[CF_RV_TERM];
   double tv, st;
   if (evapRemUse > evapRemUseMax) {
     st = 0.;
     tv = riverInflow + prev - evapRemUseMax;
     if (tv < 0.) tv = 0.;
   } else {
-    st = prev * c.rv_e + c.rv_invk * (riverInflow - evapRemUse) * c.rv_ome;
+    st = v[CF_RV_S0E] + v[CF_RV_INVK] * (riverInflow - evapRemUse) * v[CF_RV_OME];
     if (fabs(st) <= kMinStorVol) st = 0.;
     tv = riverInflow + prev - st - evapRemUse;
     if (tv < 0.) tv = 0.;
@@ -362,6 +308,19 @@ __device__ ChainOut chain_step(const ChainCell &c, double inflow_up, int day, in
   o.river = st;
   o.tv = tv;
   return o;
+}
+
+// Writes what chain_step changed.
+__device__ inline void store_chain(const WgState &X, int64_t mp, int flags, const ChainOut &o) {
+  if (flags & CFL_LAKE) X.glo_lake[mp] = o.glo_lake;
+  if (flags & CFL_RES) {
+    X.res[mp] = o.res;
+    X.k_release[mp] = o.k_release;
+  }
+  if (flags & CFL_GWET) X.glo_wetl[mp] = o.glo_wetl;
+  if (flags & CFL_ARID) X.gw[mp] = o.gw;
+  X.river[mp] = o.river;
+  X.tv[mp] = o.tv;
 }
 
 // routing.cpp:7309-7341 getNewRiverVelocity
@@ -1083,16 +1042,24 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   }
 
   // river terms that do not depend on the inflow, routing.cpp:3424-3479
-  ChainCell c;
-  c.inflow = inflow;
-  c.lrir = lrir;
-  c.lgwrir = lgwrir;
-  c.gwr_ll = gwr_loclak;
-  c.gwr_lw = gwr_locwet;
-  c.ow_prec = ow_prec;
-  c.ow_pet = ow_pet;
-  c.recharge_km3 = recharge_km3;
+  // record of the inflow-independent terms of routing.cpp:2630-3514
+  ChainRec c;
+  double *v = c.v;
+  const double e_sw = S.e_sw[p];
+  c.inv_ksw = S.inv_ksw[p];
+  c.ome_sw = S.ome_sw[p];
+  c.inv_kgw = S.inv_kgw[p];
+  c.ome_gw = S.ome_gw[p];
+  c.area = cellArea;
+  c.cf100 = S.cf100[p];
+  c.flags = S.cflags[p];
+  v[CF_INFLOW] = inflow;
+  v[CF_LRIR] = lrir;
+  v[CF_LGWRIR] = lgwrir;
+  v[CF_OW_PREC] = ow_prec;
+  v[CF_OW_PET] = ow_pet;
   {
+    // river velocity and storage terms, routing.cpp:3424-3478
     const double L = S.river_length[p];
     const double prev = X.river[mp];
     const double velocity = new_river_velocity(S.river_slope[p], S.bottom_width[p], S.roughness[p], prev, L,
@@ -1100,20 +1067,72 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     const double K = velocity / L;
     const double eK = wg_exp(-1. * K);
     const double riverAreaFrac = X.river_frac_next[mp];
-    c.rv_e = eK;
-    c.rv_invk = 1. / K;
-    c.rv_ome = 1. - eK;
-    c.rv_term = (K * prev * eK) / (1. - eK);
-    c.rv_evap = ((1.0 - cfa) * (ow_prec) + (cfa * ow_pet)) * riverAreaFrac / 100. * cellArea / 1000000.;
-    c.rv_prec = ow_prec * riverAreaFrac / 100. * cellArea / 1000000.;
+    v[CF_RV_PREV] = prev;
+    v[CF_RV_S0E] = prev * eK;
+    v[CF_RV_INVK] = 1. / K;
+    v[CF_RV_OME] = 1. - eK;
+    v[CF_RV_TERM] = (K * prev * eK) / (1. - eK);
+    v[CF_RV_EVAP] = ((1.0 - cfa) * (ow_prec) + (cfa * ow_pet)) * riverAreaFrac / 100. * cellArea / 1000000.;
+    v[CF_RV_PREC] = ow_prec * riverAreaFrac / 100. * cellArea / 1000000.;
+  }
+  {
+    // global lake, routing.cpp:2633-2676
+    const double lake_area = S.lake_area[p];
+    v[CF_GL_LP] = 0.;
+    v[CF_GL_USE] = 0.;
+    v[CF_GL_MAX] = 0.;
+    v[CF_GL_PREV] = 0.;
+    v[CF_GL_S0E] = 0.;
+    v[CF_GL_GWR] = 0.;
+    if (lake_area > 0.) {
+      const double prev = X.glo_lake[mp], rf = X.red_gl[mp];
+      double evap = ((1.0 - cfa) * ow_prec) + (cfa * ow_pet * rf);
+      if (evap < 0.) evap = 0.;
+      const double gwr = arid_swb ? kKswbgw * rf * (lake_area / (cellArea * (cf / 100.))) : 0.;
+      v[CF_GL_MAX] = lake_area * S.lake_depth[p];
+      v[CF_GL_LP] = (ow_prec * (lake_area / 1000000.));
+      v[CF_GL_GWR] = gwr;
+      v[CF_GL_USE] = evap * (lake_area / 1000000.) + gwr * cellArea * (cf / 100.) / 1000000. + 0.;
+      v[CF_GL_PREV] = prev;
+      v[CF_GL_S0E] = prev * e_sw;
+    }
+  }
+  {
+    // global wetland, routing.cpp:3181-3222
+    const double glo_wetland = S.glo_wetland[p];
+    v[CF_GW_WP] = 0.;
+    v[CF_GW_PET] = 0.;
+    v[CF_GW_MAX] = 0.;
+    v[CF_GW_PREV] = 0.;
+    v[CF_GW_S0E] = 0.;
+    v[CF_GW_GWR] = 0.;
+    if (glo_wetland > 0) {
+      const double prev = X.glo_wetl[mp], rf = X.red_gw[mp];
+      double evap = ((1.0 - cfa) * (ow_prec * rf)) + (cfa * (ow_pet * rf));
+      if (evap < 0.) evap = 0.;
+      const double gwr = arid_swb ? kKswbgw * rf * glo_wetland / 100. / (cf / 100.) : 0.;
+      v[CF_GW_MAX] = ((glo_wetland) / 100.) * cellArea * S.wetl_depth[p];
+      v[CF_GW_WP] = (ow_prec * rf * (cellArea / 1000000.) * (glo_wetland / 100.));
+      v[CF_GW_GWR] = gwr;
+      v[CF_GW_PET] = evap * (cellArea / 1000000.) * ((glo_wetland) / 100.) + gwr * cellArea * (cf / 100.) / 1000000.;
+      v[CF_GW_PREV] = prev;
+      v[CF_GW_S0E] = prev * e_sw;
+    }
+  }
+  {
+    // groundwater of (semi-)arid cells, routing.cpp:3310-3330
+    v[CF_A_GWR_LL] = gwr_loclak;
+    v[CF_A_GWR_LW] = gwr_locwet;
+    v[CF_A_RECH] = recharge_km3;
+    const double gprev = arid_swb ? X.gw[mp] : 0.;
+    v[CF_A_PREV] = gprev;
+    v[CF_A_S0E] = gprev * S.e_gw[p];
   }
   const int n_up = S.up_off[p + 1] - S.up_off[p];
   if (n_up == 0) {
     // no upstream cell: G_riverInflow[n] is still 0 when the cell is routed
-    load_chain_statics(S, p, c);
-    load_chain_state(X, mp, c);
-    const ChainOut o = chain_step(c, 0.0, doy, month);
-    store_chain(X, mp, c, o);
+    const ChainOut o = chain_step(S, X, c, mp, p, 0.0, doy, month);
+    store_chain(X, mp, c.flags, o);
     const int st = S.station[p];
     if (st >= 0) {
       const double q = (ldd >= 0) ? o.tv : 0.;
@@ -1121,52 +1140,40 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
       X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + st] = q;
     }
   } else {
-    X.c_inflow[mp] = c.inflow;
-    X.c_lrir[mp] = c.lrir;
-    X.c_lgwrir[mp] = c.lgwrir;
-    X.c_gwr_ll[mp] = c.gwr_ll;
-    X.c_gwr_lw[mp] = c.gwr_lw;
-    X.c_ow_prec[mp] = c.ow_prec;
-    X.c_ow_pet[mp] = c.ow_pet;
-    X.c_recharge_km3[mp] = c.recharge_km3;
-    X.c_rv_e[mp] = c.rv_e;
-    X.c_rv_invk[mp] = c.rv_invk;
-    X.c_rv_ome[mp] = c.rv_ome;
-    X.c_rv_term[mp] = c.rv_term;
-    X.c_rv_evap[mp] = c.rv_evap;
-    X.c_rv_prec[mp] = c.rv_prec;
+    const int64_t stride = int64_t(S.n_members) * n;
+#pragma unroll
+    for (int f = 0; f < CF_COUNT; f++) X.crec[f * stride + mp] = v[f];
   }
 }
 
 // Inputs of one chain cell, prefetched a level ahead.
 struct ChainSlot {
-  ChainCell c;
-  int p, u0, nu, contcell, station;
+  ChainRec c;
+  int p, u0, nu, station;
   int up[3];
 };
 
-__device__ inline void prefetch_slot(const WgStatic &S, const WgState &X, int64_t base, int p, ChainSlot &q) {
-  const int64_t mp = base + p;
+__device__ inline void prefetch_slot(const WgStatic &S, const WgState &X, int64_t base, int64_t stride, int p,
+                                     ChainSlot &q) {
   q.p = p;
-  q.contcell = S.contcell[p];
   q.station = S.station[p];
   q.u0 = S.up_off[p];
   q.nu = S.up_off[p + 1] - q.u0;
 #pragma unroll
   for (int k = 0; k < 3; k++) q.up[k] = (k < q.nu) ? S.up_idx[q.u0 + k] : 0;
-  load_chain_record(X, mp, q.c);
-  load_chain_statics(S, p, q.c);
-  load_chain_state(X, mp, q.c);
+  load_chain_rec(S, X, base + p, p, stride, q.c);
 }
 
 // One workgroup per (routing group, member): levels 1..depth of the group.  The
 // transported volumes of the group live in LDS (level 0 is copied in from the
-// cell kernel's output); up_idx holds group-local indices.
+// cell kernel's output); up_idx holds group-local indices.  Each thread issues
+// the loads of its next-level cell before computing the current level, so their
+// latency overlaps the computation and the barrier.
 __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, WgTables T,
-                                                        const int32_t *__restrict__ group_level_off,
-                                                        const int32_t *__restrict__ group_first_level,
-                                                        const WgClock *__restrict__ clock, int k,
-                                                        int daily_slot) {
+                                                       const int32_t *__restrict__ group_level_off,
+                                                       const int32_t *__restrict__ group_first_level,
+                                                       const WgClock *__restrict__ clock, int k,
+                                                       int daily_slot) {
   extern __shared__ double s_tv[];
   const int g = blockIdx.x;
   const int m = blockIdx.y;
@@ -1178,52 +1185,53 @@ __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, Wg
   const int gs = group_level_off[l0], ge = group_level_off[l1];
   const int lvl0_end = group_level_off[l0 + 1];
   const int64_t base = int64_t(m) * n;
+  const int64_t stride = int64_t(S.n_members) * n;
   // level 0 was routed by the cell kernel; cells not routed today read as 0
   for (int i = gs + tid; i < ge; i += nt) s_tv[i - gs] = (i < lvl0_end) ? X.tv[base + i] : 0.;
-  ChainSlot next;
-  bool have_next = false;
+  ChainSlot cur, nxt;
+  bool have_cur = false;
   if (l0 + 1 < l1) {
     const int p = group_level_off[l0 + 1] + tid;
     if (p < group_level_off[l0 + 2]) {
-      prefetch_slot(S, X, base, p, next);
-      have_next = true;
+      prefetch_slot(S, X, base, stride, p, cur);
+      have_cur = true;
     }
   }
   __syncthreads();
   for (int l = l0 + 1; l < l1; l++) {
     const int beg = group_level_off[l], end = group_level_off[l + 1];
-    bool first = true;
-    for (int p = beg + tid; p < end; p += nt) {
-      ChainSlot cur;
-      if (first && have_next) {
-        cur = next;
-      } else {
-        prefetch_slot(S, X, base, p, cur);
-      }
-      first = false;
-      if (!cur.contcell) continue;
-      // ordered gather: the serial += of routing.cpp:3955-3958 in routing order
-      double up = 0.;
-      for (int u = 0; u < cur.nu; u++) up += s_tv[(u < 3) ? cur.up[u] : S.up_idx[cur.u0 + u]];
-      const ChainOut o = chain_step(cur.c, up, doy, month);
-      s_tv[p - gs] = o.tv;
-      store_chain(X, base + p, cur.c, o);
-      if (cur.station >= 0) {
-        const double q = (cur.c.ldd >= 0) ? o.tv : 0.;
-        X.st_sum[int64_t(m) * S.n_stations + cur.station] += q;
-        X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + cur.station] = q;
-      }
-    }
-    // inputs of the next level do not depend on this level: load them before the barrier
-    have_next = false;
+    bool have_nxt = false;
     if (l + 1 < l1) {
       const int p = group_level_off[l + 1] + tid;
       if (p < group_level_off[l + 2]) {
-        prefetch_slot(S, X, base, p, next);
-        have_next = true;
+        prefetch_slot(S, X, base, stride, p, nxt);
+        have_nxt = true;
+      }
+    }
+    bool first = true;
+    for (int p = beg + tid; p < end; p += nt) {
+      ChainSlot q;
+      if (first && have_cur)
+        q = cur;
+      else
+        prefetch_slot(S, X, base, stride, p, q);
+      first = false;
+      if (!(q.c.flags & CFL_CONT)) continue;
+      // ordered gather: the serial += of routing.cpp:3955-3958 in routing order
+      double up = 0.;
+      for (int u = 0; u < q.nu; u++) up += s_tv[(u < 3) ? q.up[u] : S.up_idx[q.u0 + u]];
+      const ChainOut o = chain_step(S, X, q.c, base + p, p, up, doy, month);
+      s_tv[p - gs] = o.tv;
+      store_chain(X, base + p, q.c.flags, o);
+      if (q.station >= 0) {
+        const double qq = (q.c.flags & CFL_LDD) ? o.tv : 0.;
+        X.st_sum[int64_t(m) * S.n_stations + q.station] += qq;
+        X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + q.station] = qq;
       }
     }
     __syncthreads();
+    cur = nxt;
+    have_cur = have_nxt;
   }
   (void)T;
 }
