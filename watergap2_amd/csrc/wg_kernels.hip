@@ -528,7 +528,10 @@ __device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p)
 
 
 // ---- one elevation sub-cell of the snow loop, daily.cpp:920-1044 ----
-constexpr int kSnowChunk = 10;  // sub-cell pairs per double-buffered chunk
+#ifndef WG_SNOW_CHUNK
+#define WG_SNOW_CHUNK 10
+#endif
+constexpr int kSnowChunk = WG_SNOW_CHUNK;  // sub-cell pairs per double-buffered chunk
 
 struct SnowDay {
   double tempC, gradnt, scale_num, laf, prec_to_soil, soil_pet, snowfz, snowmt, melt_factor;
@@ -1146,6 +1149,16 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   }
 }
 
+// Workgroup barrier that orders LDS only.  The level walk exchanges data between
+// threads through LDS alone (its global stores are read by later kernels), and a
+// full __syncthreads() would also wait for the next level's prefetch loads
+// (vmcnt counts loads and stores alike on this architecture).
+__device__ inline void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Inputs of one chain cell, prefetched a level ahead.
 struct ChainSlot {
   ChainRec c;
@@ -1197,7 +1210,7 @@ __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, Wg
       have_cur = true;
     }
   }
-  __syncthreads();
+  lds_barrier();
   for (int l = l0 + 1; l < l1; l++) {
     const int beg = group_level_off[l], end = group_level_off[l + 1];
     bool have_nxt = false;
@@ -1229,7 +1242,7 @@ __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, Wg
         X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + q.station] = qq;
       }
     }
-    __syncthreads();
+    lds_barrier();
     cur = nxt;
     have_cur = have_nxt;
   }
