@@ -131,20 +131,22 @@ def test_eager_equals_graph(wg, oracle, regional_world):
 
 
 def test_level_schedule_invariance(wg, regional_world):
-    """Results do not depend on how basins are packed into routing groups or on the
-    chain workgroup size (bit-identical)."""
+    """Results do not depend on how basins are packed into routing groups, how the
+    groups are placed over the XCDs, or on the chain workgroup size (bit-identical)."""
     print("case: schedule invariance", flush=True)
     w = regional_world
     f = w.forcing(seed=3, day0=0, n_days=30)
     states = []
-    for cells, threads in ((1024, 256), (1, 64), (100000, 128)):
+    for cells, threads, xcds in ((1024, 256, 8), (1, 64, 3), (100000, 128, 1)):
         os.environ["WG_GROUP_CELLS"] = str(cells)
         os.environ["WG_CHAIN_THREADS"] = str(threads)
+        os.environ["WG_XCDS"] = str(xcds)
         try:
             m = wg.WaterBalanceModel(w)
         finally:
             del os.environ["WG_GROUP_CELLS"]
             del os.environ["WG_CHAIN_THREADS"]
+            del os.environ["WG_XCDS"]
         m.load_forcing(f)
         m.step_days(0, 50, 30)
         states.append(m.get_state())

@@ -69,6 +69,9 @@ struct wg_handle {
   std::map<int, hipGraphExec_t> graphs;
   int block_threads_cell = 256, block_threads_chain = 256;
   int32_t max_group_cells = 0;
+  int n_xcd = 8;
+  std::vector<int32_t> tile_start, tile_end, block_group;
+  std::vector<WgChainStatic> cstat;
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms[3] = {0, 0, 0};
@@ -260,27 +263,55 @@ int build_schedule(wg_handle *h, const wg_static_fields *st, int group_cells) {
   });
   // greedy packing of basins into routing groups of about group_cells cells
   std::vector<int32_t> group_of_basin(nb, -1);
-  int32_t g = -1, g_cells = 0;
+  int32_t g = -1, cur_cells = 0;
   std::vector<int32_t> g_depth;
   for (int32_t k = 0; k < nb; k++) {
     const int32_t b = border[k];
-    if (g < 0 || g_cells + b_cells[b] > group_cells) {
+    if (g < 0 || cur_cells + b_cells[b] > group_cells) {
       g++;
-      g_cells = 0;
+      cur_cells = 0;
       g_depth.push_back(0);
     }
     group_of_basin[b] = g;
-    g_cells += b_cells[b];
+    cur_cells += b_cells[b];
     g_depth[g] = std::max(g_depth[g], b_depth[b]);
   }
-  h->n_groups = g + 1;
+  const int32_t n_groups0 = g + 1;
+  // XCD-aware placement: bin the groups over the XCDs by cell count (largest
+  // first into the lightest bin) and order them bin by bin, so that a bin's cells
+  // are contiguous and both kernels can run a group's cells on one XCD.
+  std::vector<int32_t> g_cells(n_groups0, 0);
+  for (int32_t i = 0; i < n; i++) g_cells[group_of_basin[bid[outlet[i]]]]++;
+  std::vector<int32_t> gsort(n_groups0);
+  for (int32_t gg = 0; gg < n_groups0; gg++) gsort[gg] = gg;
+  std::stable_sort(gsort.begin(), gsort.end(), [&](int32_t a, int32_t b) { return g_cells[a] > g_cells[b]; });
+  const int nx = h->n_xcd;
+  std::vector<int64_t> bin_load(nx, 0);
+  std::vector<std::vector<int32_t>> bin_groups(nx);
+  for (int32_t gg : gsort) {
+    int best = 0;
+    for (int x = 1; x < nx; x++)
+      if (bin_load[x] < bin_load[best]) best = x;
+    bin_groups[best].push_back(gg);
+    bin_load[best] += g_cells[gg];
+  }
+  std::vector<int32_t> new_gid(n_groups0), bin_of_new;
+  std::vector<int32_t> depth_new;
+  for (int x = 0; x < nx; x++)
+    for (int32_t gg : bin_groups[x]) {
+      new_gid[gg] = int32_t(depth_new.size());
+      depth_new.push_back(g_depth[gg]);
+      bin_of_new.push_back(x);
+    }
+  h->n_groups = n_groups0;
   h->max_depth = 0;
-  for (int32_t d : g_depth) h->max_depth = std::max(h->max_depth, d);
-  // device order: (group, level, routing order)
+  for (int32_t d : depth_new) h->max_depth = std::max(h->max_depth, d);
+  auto gid = [&](int32_t cell) { return new_gid[group_of_basin[bid[outlet[cell]]]]; };
+  // device order: (bin, group, level, routing order)
   std::vector<int32_t> cells(n);
   for (int32_t i = 0; i < n; i++) cells[i] = i;
   std::sort(cells.begin(), cells.end(), [&](int32_t a, int32_t b) {
-    const int32_t ga = group_of_basin[bid[outlet[a]]], gb = group_of_basin[bid[outlet[b]]];
+    const int32_t ga = gid(a), gb = gid(b);
     if (ga != gb) return ga < gb;
     if (level[a] != level[b]) return level[a] < level[b];
     return st->rout_order[a] < st->rout_order[b];
@@ -290,20 +321,49 @@ int build_schedule(wg_handle *h, const wg_static_fields *st, int group_cells) {
   for (int32_t p = 0; p < n; p++) h->inv[cells[p]] = p;
   // level slots
   h->group_first_level.assign(h->n_groups + 1, 0);
-  for (int32_t gg = 0; gg < h->n_groups; gg++) h->group_first_level[gg + 1] = h->group_first_level[gg] + g_depth[gg];
+  for (int32_t gg = 0; gg < h->n_groups; gg++) h->group_first_level[gg + 1] = h->group_first_level[gg] + depth_new[gg];
   const int32_t n_slots = h->group_first_level[h->n_groups];
   h->level_off.assign(n_slots + 1, n);
   std::vector<int32_t> slot_count(n_slots, 0);
-  for (int32_t i = 0; i < n; i++) {
-    const int32_t gg = group_of_basin[bid[outlet[i]]];
-    slot_count[h->group_first_level[gg] + level[i]]++;
-  }
+  for (int32_t i = 0; i < n; i++) slot_count[h->group_first_level[gid(i)] + level[i]]++;
   int32_t acc = 0;
   for (int32_t s = 0; s < n_slots; s++) {
     h->level_off[s] = acc;
     acc += slot_count[s];
   }
   h->level_off[n_slots] = acc;
+  // block tables: block 8j + x runs on XCD x
+  std::vector<int32_t> bin_start(nx, 0), bin_end(nx, 0);
+  {
+    int32_t gg = 0;
+    for (int x = 0; x < nx; x++) {
+      bin_start[x] = h->level_off[h->group_first_level[gg]];
+      gg += int32_t(bin_groups[x].size());
+      bin_end[x] = h->level_off[h->group_first_level[gg]];
+    }
+  }
+  const int T = h->block_threads_cell;
+  int32_t max_tiles = 1, max_groups = 1;
+  for (int x = 0; x < nx; x++) {
+    max_tiles = std::max(max_tiles, (bin_end[x] - bin_start[x] + T - 1) / T);
+    max_groups = std::max(max_groups, int32_t(bin_groups[x].size()));
+  }
+  h->tile_start.assign(size_t(nx) * max_tiles, -1);
+  h->tile_end.assign(size_t(nx) * max_tiles, -1);
+  for (int32_t j = 0; j < max_tiles; j++)
+    for (int x = 0; x < nx; x++) {
+      const int32_t a = bin_start[x] + T * j;
+      if (a < bin_end[x]) {
+        h->tile_start[size_t(j) * nx + x] = a;
+        h->tile_end[size_t(j) * nx + x] = std::min(a + T, bin_end[x]);
+      }
+    }
+  h->block_group.assign(size_t(nx) * max_groups, -1);
+  {
+    int32_t gg = 0;
+    for (int x = 0; x < nx; x++)
+      for (size_t j = 0; j < bin_groups[x].size(); j++) h->block_group[j * nx + x] = gg++;
+  }
   return WG_OK;
 }
 
@@ -337,6 +397,7 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
   e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e != hipSuccess) return WG_E_HIP;
   if (const char *v = getenv("WG_CHAIN_THREADS")) h->block_threads_chain = std::max(64, std::min(256, atoi(v)));
+  if (const char *v = getenv("WG_XCDS")) h->n_xcd = std::max(1, std::min(64, atoi(v)));
   int group_cells = 1024;
   if (const char *v = getenv("WG_GROUP_CELLS")) group_cells = std::max(1, atoi(v));
   int rc = build_schedule(h.get(), st, group_cells);
@@ -569,6 +630,47 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     r_ = upload(H, &d, none);
     if (r_) { *out = h.release(); return r_; }
     S.station = d;
+    // static routing record
+    H->cstat.assign(n, WgChainStatic{});
+    for (int32_t p = 0; p < n; p++) {
+      const int32_t i = H->perm[p];
+      WgChainStatic &c = H->cstat[p];
+      c.inv_ksw = inv_ksw[i];
+      c.ome_sw = ome_sw[i];
+      c.inv_kgw = inv_kgw[i];
+      c.ome_gw = ome_gw[i];
+      c.area = st->cell_area[i];
+      c.cf100 = st->contfreq[i] / 100.;
+      int f = 0;
+      if (st->lake_area[i] > 0.) f |= CFL_LAKE;
+      if (st->reservoir_area[i] > 0.) f |= CFL_RES;
+      if (st->glo_wetland[i] > 0) f |= CFL_GWET;
+      if (st->aindex[i] == 1 && st->ldd[i] >= 0) f |= CFL_ARID;
+      if (st->ldd[i] >= 0) f |= CFL_LDD;
+      if (contcell[i]) f |= CFL_CONT;
+      c.flags = f;
+      c.station = -1;
+      c.u0 = off[p];
+      c.nu = off[p + 1] - off[p];
+      c.up0 = c.nu > 0 ? idx[off[p]] : 0;
+      c.up1 = c.nu > 1 ? idx[off[p] + 1] : 0;
+      c.up2 = c.nu > 2 ? idx[off[p] + 2] : 0;
+      c.pad = 0;
+    }
+    WgChainStatic *dcs;
+    r_ = upload(H, &dcs, H->cstat);
+    if (r_) { *out = h.release(); return r_; }
+    S.cstat = dcs;
+    int32_t *dt;
+    r_ = upload(H, &dt, H->tile_start);
+    if (r_) { *out = h.release(); return r_; }
+    S.tile_start = dt;
+    r_ = upload(H, &dt, H->tile_end);
+    if (r_) { *out = h.release(); return r_; }
+    S.tile_end = dt;
+    r_ = upload(H, &dt, H->block_group);
+    if (r_) { *out = h.release(); return r_; }
+    S.block_group = dt;
     r_ = upload(H, &H->d_level_off, H->level_off);
     if (r_) { *out = h.release(); return r_; }
     r_ = upload(H, &H->d_group_first, H->group_first_level);
@@ -589,7 +691,7 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     rc = dev_alloc(H, p, NM);
     if (rc) { *out = h.release(); return rc; }
   }
-  rc = dev_alloc(H, &X.crec, NM * CF_COUNT);
+  rc = dev_alloc(H, &X.crec, NM);
   if (!rc) rc = dev_alloc(H, &X.lai_days, NM);
   if (!rc) rc = dev_alloc(H, &X.lai_status, NM);
   if (!rc) rc = dev_alloc(H, &X.smax, NM);
@@ -819,6 +921,9 @@ extern "C" int wg_set_stations(wg_handle *h, int32_t n_st, const int32_t *cells)
   }
   HIP_CHECK(h, hipMemcpy(const_cast<int32_t *>(h->S.station), slot.data(), h->n * sizeof(int32_t),
                          hipMemcpyHostToDevice));
+  for (int32_t p = 0; p < h->n; p++) h->cstat[p].station = slot[p];
+  HIP_CHECK(h, hipMemcpy(const_cast<WgChainStatic *>(h->S.cstat), h->cstat.data(),
+                         h->cstat.size() * sizeof(WgChainStatic), hipMemcpyHostToDevice));
   // reallocate the accumulators
   for (auto it = h->bufs.begin(); it != h->bufs.end();) {
     if (it->p == h->X.st_sum || it->p == h->X.st_daily) {
@@ -844,9 +949,9 @@ namespace {
 
 // Enqueues one block of `days` days on the stream (captured or eager).
 int enqueue_block(wg_handle *h, int days, hipEvent_t *evs) {
-  const int32_t n = h->n, M = h->m;
-  const dim3 cgrid((n + h->block_threads_cell - 1) / h->block_threads_cell, M);
-  const dim3 chgrid(h->n_groups, M);
+  const int32_t M = h->m;
+  const dim3 cgrid(uint32_t(h->tile_start.size()), M);
+  const dim3 chgrid(uint32_t(h->block_group.size()), M);
   int ev = 0;
   for (int k = 0; k < days; k++) {
     const int slot = k % WG_DAILY_SLOTS;

@@ -17,6 +17,20 @@ struct alignas(16) WgD2 {
   double x, y;
 };
 
+// Static part of a cell's routing record (device order): the constants of the
+// upstream-dependent routing step, packed for one contiguous load.
+struct alignas(16) WgChainStatic {
+  double inv_ksw, ome_sw, inv_kgw, ome_gw, area, cf100;
+  int32_t flags, station, u0, nu, up0, up1, up2, pad;
+};
+
+// Daily part of a cell's routing record, written by the cell kernel for cells
+// with upstream inflow (field meanings: ChainField in wg_kernels.hip).
+#define WG_DEV_NREC 30
+struct alignas(16) WgChainRec {
+  double v[WG_DEV_NREC];
+};
+
 // Land-cover tables indexed by class-1 (LCT_22.DAT, LAI_22.DAT).
 struct WgTables {
   double albedo[18], albedo_snow[18], ddf[18], emissivity[18];
@@ -48,6 +62,11 @@ struct WgStatic {
   const double *k_sw, *e_sw, *inv_ksw, *ome_sw, *k_gw, *e_gw, *inv_kgw, *ome_gw;
   const double *cf100;              // contfreq / 100
   const int8_t *cflags;             // lake / reservoir / global wetland / arid / LDD >= 0 / continental
+  const WgChainStatic *cstat;       // [N] static routing record
+  // XCD-aware placement: the cell kernel's block b covers cells [tile_start[b],
+  // tile_end[b]) (tile_start -1: idle block); the chain kernel's block i routes
+  // group block_group[i] (-1: idle).  Block i runs on XCD i % 8.
+  const int32_t *tile_start, *tile_end, *block_group;
   // schedule
   const int32_t *up_off, *up_idx;   // CSR of upstream cells sorted by routing order
   const int32_t *station;           // station slot of each cell, -1 = none
@@ -65,9 +84,8 @@ struct WgState {
   double *gamma, *cfa;
   float *smax;
   double *tv;                   // transported volume of the day
-  // routing record written by the cell kernel for cells with upstream inflow:
-  // [field][member][N], see ChainField
-  double *crec;
+  // routing record written by the cell kernel for cells with upstream inflow
+  WgChainRec *crec;             // [M][N]
   // diagnostics of the last day
   double *d_surf, *d_recharge, *d_ow_pet, *d_aet, *d_prec;
   // stations

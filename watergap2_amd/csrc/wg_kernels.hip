@@ -78,8 +78,8 @@ __device__ double irwin_hall(uint64_t seed, uint32_t cell, uint32_t day, uint32_
 // Record of one routing cell: every term of routing.cpp:2623-3958 that does not
 // depend on the water arriving from upstream, computed by the cell kernel with
 // the reference's own expressions, so that the level walk only evaluates the
-// short chain that does.  Stored field-major: crec[field][member][cell].
-enum ChainField {
+// short chain that does.  Stored as one 240-byte record per (member, cell).
+enum ChainField : int {
   CF_INFLOW = 0,  // outflow of the local lake/wetland system (routing.cpp:2616)
   CF_LRIR,        // G_localRunoffIntoRiver
   CF_LGWRIR,      // G_localGWRunoffIntoRiver of humid cells
@@ -111,17 +111,19 @@ struct ChainOut {
   double glo_lake, res, k_release, glo_wetl, gw, river, tv;
 };
 
-__device__ inline void load_chain_rec(const WgStatic &S, const WgState &X, int64_t mp, int p, int64_t stride,
-                                      ChainRec &r) {
+__device__ inline void load_chain_rec(const WgStatic &S, const WgState &X, int64_t mp, int p, ChainRec &r,
+                                      WgChainStatic &cs) {
+  cs = S.cstat[p];
+  const WgChainRec rec = X.crec[mp];
 #pragma unroll
-  for (int f = 0; f < CF_COUNT; f++) r.v[f] = X.crec[f * stride + mp];
-  r.inv_ksw = S.inv_ksw[p];
-  r.ome_sw = S.ome_sw[p];
-  r.inv_kgw = S.inv_kgw[p];
-  r.ome_gw = S.ome_gw[p];
-  r.area = S.cell_area[p];
-  r.cf100 = S.cf100[p];
-  r.flags = S.cflags[p];
+  for (int f = 0; f < CF_COUNT; f++) r.v[f] = rec.v[f];
+  r.inv_ksw = cs.inv_ksw;
+  r.ome_sw = cs.ome_sw;
+  r.inv_kgw = cs.inv_kgw;
+  r.ome_gw = cs.ome_gw;
+  r.area = cs.area;
+  r.cf100 = cs.cf100;
+  r.flags = cs.flags;
 }
 
 // routing.cpp:2623-3958 for one cell from its record (parity profile: no water
@@ -529,7 +531,7 @@ __device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p)
 
 // ---- one elevation sub-cell of the snow loop, daily.cpp:920-1044 ----
 #ifndef WG_SNOW_CHUNK
-#define WG_SNOW_CHUNK 10
+#define WG_SNOW_CHUNK 5
 #endif
 constexpr int kSnowChunk = WG_SNOW_CHUNK;  // sub-cell pairs per double-buffered chunk
 
@@ -603,10 +605,12 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
                                                       const WgClock *__restrict__ clock, int k,
                                                       int do_post, int do_vertical, int write_diag,
                                                       int daily_slot) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t0 = S.tile_start[blockIdx.x];
+  if (t0 < 0) return;
+  const int p = t0 + int(threadIdx.x);
+  if (p >= S.tile_end[blockIdx.x]) return;
   const int m = blockIdx.y;
   const int n = S.n;
-  if (p >= n) return;
   const int64_t mp = int64_t(m) * n + p;
 
   if (do_post) post_day(S, X, mp, p);
@@ -1049,13 +1053,14 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   ChainRec c;
   double *v = c.v;
   const double e_sw = S.e_sw[p];
-  c.inv_ksw = S.inv_ksw[p];
-  c.ome_sw = S.ome_sw[p];
-  c.inv_kgw = S.inv_kgw[p];
-  c.ome_gw = S.ome_gw[p];
-  c.area = cellArea;
-  c.cf100 = S.cf100[p];
-  c.flags = S.cflags[p];
+  const WgChainStatic cs = S.cstat[p];
+  c.inv_ksw = cs.inv_ksw;
+  c.ome_sw = cs.ome_sw;
+  c.inv_kgw = cs.inv_kgw;
+  c.ome_gw = cs.ome_gw;
+  c.area = cs.area;
+  c.cf100 = cs.cf100;
+  c.flags = cs.flags;
   v[CF_INFLOW] = inflow;
   v[CF_LRIR] = lrir;
   v[CF_LGWRIR] = lgwrir;
@@ -1131,21 +1136,22 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     v[CF_A_PREV] = gprev;
     v[CF_A_S0E] = gprev * S.e_gw[p];
   }
-  const int n_up = S.up_off[p + 1] - S.up_off[p];
-  if (n_up == 0) {
+  if (cs.nu == 0) {
     // no upstream cell: G_riverInflow[n] is still 0 when the cell is routed
     const ChainOut o = chain_step(S, X, c, mp, p, 0.0, doy, month);
     store_chain(X, mp, c.flags, o);
-    const int st = S.station[p];
+    const int st = cs.station;
     if (st >= 0) {
       const double q = (ldd >= 0) ? o.tv : 0.;
       X.st_sum[int64_t(m) * S.n_stations + st] += q;
       X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + st] = q;
     }
   } else {
-    const int64_t stride = int64_t(S.n_members) * n;
+    WgChainRec rec;
 #pragma unroll
-    for (int f = 0; f < CF_COUNT; f++) X.crec[f * stride + mp] = v[f];
+    for (int f = 0; f < CF_COUNT; f++) rec.v[f] = v[f];
+    rec.v[WG_DEV_NREC - 1] = 0.;
+    X.crec[mp] = rec;
   }
 }
 
@@ -1162,19 +1168,19 @@ __device__ inline void lds_barrier() {
 // Inputs of one chain cell, prefetched a level ahead.
 struct ChainSlot {
   ChainRec c;
-  int p, u0, nu, station;
+  int u0, nu, station;
   int up[3];
 };
 
-__device__ inline void prefetch_slot(const WgStatic &S, const WgState &X, int64_t base, int64_t stride, int p,
-                                     ChainSlot &q) {
-  q.p = p;
-  q.station = S.station[p];
-  q.u0 = S.up_off[p];
-  q.nu = S.up_off[p + 1] - q.u0;
-#pragma unroll
-  for (int k = 0; k < 3; k++) q.up[k] = (k < q.nu) ? S.up_idx[q.u0 + k] : 0;
-  load_chain_rec(S, X, base + p, p, stride, q.c);
+__device__ inline void prefetch_slot(const WgStatic &S, const WgState &X, int64_t base, int p, ChainSlot &q) {
+  WgChainStatic cs;
+  load_chain_rec(S, X, base + p, p, q.c, cs);
+  q.station = cs.station;
+  q.u0 = cs.u0;
+  q.nu = cs.nu;
+  q.up[0] = cs.up0;
+  q.up[1] = cs.up1;
+  q.up[2] = cs.up2;
 }
 
 // One workgroup per (routing group, member): levels 1..depth of the group.  The
@@ -1188,7 +1194,8 @@ __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, Wg
                                                        const WgClock *__restrict__ clock, int k,
                                                        int daily_slot) {
   extern __shared__ double s_tv[];
-  const int g = blockIdx.x;
+  const int g = S.block_group[blockIdx.x];
+  if (g < 0) return;
   const int m = blockIdx.y;
   const int n = S.n;
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -1198,7 +1205,6 @@ __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, Wg
   const int gs = group_level_off[l0], ge = group_level_off[l1];
   const int lvl0_end = group_level_off[l0 + 1];
   const int64_t base = int64_t(m) * n;
-  const int64_t stride = int64_t(S.n_members) * n;
   // level 0 was routed by the cell kernel; cells not routed today read as 0
   for (int i = gs + tid; i < ge; i += nt) s_tv[i - gs] = (i < lvl0_end) ? X.tv[base + i] : 0.;
   ChainSlot cur, nxt;
@@ -1206,7 +1212,7 @@ __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, Wg
   if (l0 + 1 < l1) {
     const int p = group_level_off[l0 + 1] + tid;
     if (p < group_level_off[l0 + 2]) {
-      prefetch_slot(S, X, base, stride, p, cur);
+      prefetch_slot(S, X, base, p, cur);
       have_cur = true;
     }
   }
@@ -1217,7 +1223,7 @@ __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, Wg
     if (l + 1 < l1) {
       const int p = group_level_off[l + 1] + tid;
       if (p < group_level_off[l + 2]) {
-        prefetch_slot(S, X, base, stride, p, nxt);
+        prefetch_slot(S, X, base, p, nxt);
         have_nxt = true;
       }
     }
@@ -1227,7 +1233,7 @@ __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, Wg
       if (first && have_cur)
         q = cur;
       else
-        prefetch_slot(S, X, base, stride, p, q);
+        prefetch_slot(S, X, base, p, q);
       first = false;
       if (!(q.c.flags & CFL_CONT)) continue;
       // ordered gather: the serial += of routing.cpp:3955-3958 in routing order
