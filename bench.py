@@ -14,8 +14,10 @@ Besides the JSON line's value, the run reports
     the same kernels over a profiling window after the timed region), against
     the 8 TB/s HBM peak, with the HBM traffic per launch when a rocprof counter
     summary is present under profiles/;
-  * cpu_baseline: the CPU oracle (single thread) on a bounded sample of the same
-    workload on this host.
+  * cpu_baseline: the CPU oracle built with the C library's exp/pow, vertical
+    balance on all host threads (up to 16) and serial routing, as the reference
+    runs it, on a bounded sample of the same workload on this host; plus its
+    single-thread rate.
 """
 import argparse
 import json
@@ -46,7 +48,7 @@ def cell_kernel_bytes(n_cells, n_members, level0_cells):
         + 2 * 8 * 3   # fswb fraction, land area fraction, previous land area fraction
         + 8 * 2 + 4   # gamma, CFA, Smax
     )
-    chain_record = 14 * 8  # written for cells with upstream inflow
+    chain_record = 29 * 8  # routing record written for cells with upstream inflow
     statics = (202         # elevations int16 x 101
                + 8 * 36    # FP64 statics (geometry, parameters, exp terms)
                + 4 * 3     # float statics
@@ -56,13 +58,24 @@ def cell_kernel_bytes(n_cells, n_members, level0_cells):
             + n_cells * statics)
 
 
-def traffic_from_profiles():
-    """HBM bytes per cell-kernel launch from the committed rocprof counter summary."""
-    path = os.path.join(ROOT, "profiles", "cell_kernel_hbm_traffic.json")
+def chain_kernel_bytes(n_cells, n_members, level0_cells):
+    """Algorithmic HBM bytes of one chain-kernel launch: per routed cell the routing
+    record (29 FP64 terms), the static record (80 B) and the river-storage and
+    transported-volume stores; the transported volumes of headwater cells are read
+    once.  Storage stores of lake/reservoir/wetland cells are left out (a lower
+    bound)."""
+    n_chain = n_cells - level0_cells
+    return n_members * (n_chain * (29 * 8 + 16) + level0_cells * 8) + n_chain * 80
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprof counter summary
+    (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_global67k.json")
     if not os.path.exists(path):
         return None
     try:
-        return json.load(open(path)).get("bytes_per_launch")
+        return json.load(open(path))["kernels"][kernel]["hbm_bytes_per_launch"]
     except Exception:
         return None
 
@@ -74,8 +87,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1, help="untimed simulated years")
     ap.add_argument("--days-per-step", type=int, default=365)
     ap.add_argument("--profile-days", type=int, default=60)
-    ap.add_argument("--cpu-days", type=int, default=240,
-                    help="days of the CPU-oracle sample (bounded CPU baseline)")
+    ap.add_argument("--cpu-days", type=int, default=150,
+                    help="days of the multi-threaded CPU-oracle sample (bounded CPU baseline)")
+    ap.add_argument("--cpu-days-single", type=int, default=60,
+                    help="days of the single-thread CPU-oracle sample")
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="OpenMP threads of the CPU oracle's vertical loop (capped at the host's)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this size (counter calibration) and exit")
@@ -103,7 +120,7 @@ def main():
     world = wg.World(wg.global_spec(seed=1))
     n = world.n
     dps = args.days_per_step
-    n_days = (args.warmup + args.steps) * dps
+    n_days = max((args.warmup + args.steps) * dps, args.profile_days, 1)
     forcing = world.forcing(seed=1, day0=0, n_days=n_days)
     model = wg.WaterBalanceModel(world)
     model.set_stations(world.basin_outlet)
@@ -131,43 +148,66 @@ def main():
           file=sys.stderr, flush=True)
 
     # per-kernel durations: eager launches with HIP events on the model's stream
-    pd = min(args.profile_days, n_days)
+    pd = max(1, min(args.profile_days, n_days))
     model.step_days(0, 1, pd, profiled=True)
     ms, launches = model.timing()
     cell_ms = ms[0] / max(launches[0], 1)
     chain_ms = ms[1] / max(launches[1], 1)
-    cell_bytes = cell_kernel_bytes(n, 1, info["level0_cells"])
-    cell_gbs = cell_bytes / (cell_ms * 1e-3) / 1e9
-    print(f"# cell kernel {cell_ms * 1e3:.1f} us/launch ({cell_gbs:.0f} GB/s algorithmic), "
-          f"chain kernel {chain_ms * 1e3:.1f} us/launch", file=sys.stderr, flush=True)
-    traffic = traffic_from_profiles()
+    l0 = info["level0_cells"]
+    kern = {
+        "wg_cell_kernel": (cell_ms, cell_kernel_bytes(n, 1, l0)),
+        "wg_chain_kernel": (chain_ms, chain_kernel_bytes(n, 1, l0)),
+    }
+    per_kernel = {}
+    for name, (kms, kb) in kern.items():
+        gbs = kb / (kms * 1e-3) / 1e9
+        per_kernel[name] = {"avg_launch_us": round(kms * 1e3, 2), "algorithmic_bytes_per_launch": kb,
+                            "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                            "share_of_day_time": round(kms / (cell_ms + chain_ms), 3)}
+        print(f"# {name} {kms * 1e3:.1f} us/launch, {kb / 1e6:.1f} MB algorithmic -> {gbs:.0f} GB/s",
+              file=sys.stderr, flush=True)
+    dominant = max(kern, key=lambda k: kern[k][0])
+    dom_ms, dom_bytes = kern[dominant]
+    dom_gbs = dom_bytes / (dom_ms * 1e-3) / 1e9
     roofline = {
-        "kernel": "wg_cell_kernel",
+        "kernel": dominant,
         "bound": "hbm",
-        "achieved": round(cell_gbs, 1),
+        "achieved": round(dom_gbs, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": round(cell_gbs / HBM_PEAK_GBS, 4),
-        "traffic": traffic,
-        "algorithmic_bytes_per_launch": cell_bytes,
-        "avg_launch_us": round(cell_ms * 1e3, 2),
-        "chain_kernel_avg_launch_us": round(chain_ms * 1e3, 2),
-        "chain_kernel_note": "latency-bound level walk; share of day time %.0f%%" %
-                             (100.0 * chain_ms / (cell_ms + chain_ms)),
+        "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
+        "traffic": traffic_from_profiles(dominant),
+        "per_kernel": per_kernel,
+        "note": ("wg_chain_kernel walks the Kahn levels of each routing group with a workgroup "
+                 "barrier per level (104 dependent levels at this grid); its time is latency, "
+                 "not bandwidth"),
     }
 
     cpu = None
     if not args.no_cpu:
         from oracle import wg_oracle
-        cd = min(args.cpu_days, n_days)
-        o = wg_oracle.Oracle(world)
+        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        cd = max(1, min(args.cpu_days, n_days))
+        o = wg_oracle.Oracle(world, variant="glibc")
         o.set_stations(world.basin_outlet)
+        o.set_threads(threads)
         t1 = time.perf_counter()
         o.run(forcing[:cd], 1)
         cpu_s = time.perf_counter() - t1
-        cpu = {"value": round(n * cd / cpu_s, 1), "unit": "cell-days/s", "cores": 1, "kind": "port",
-               "sample": f"CPU oracle, 1 thread, {n} cells x {cd} days of the same forcing ({cpu_s:.1f}s)"}
-        print(f"# cpu oracle {n * cd / cpu_s:.4g} cell-days/s on 1 core", file=sys.stderr, flush=True)
+        cpu_value = n * cd / cpu_s
+        cd1 = max(1, min(args.cpu_days_single, n_days))
+        o1 = wg_oracle.Oracle(world, variant="glibc")
+        t1 = time.perf_counter()
+        o1.run(forcing[:cd1], 1)
+        single_s = time.perf_counter() - t1
+        cpu = {"value": round(cpu_value, 1), "unit": "cell-days/s", "cores": threads, "kind": "port",
+               "sample": (f"CPU oracle (glibc libm, -O2 -ffp-contract=off), calcNewDay over cells on "
+                          f"{threads} OpenMP threads, serial routing; {n} cells x {cd} days of the same "
+                          f"forcing in {cpu_s:.1f}s"),
+               "single_thread_value": round(n * cd1 / single_s, 1),
+               "single_thread_sample": f"{n} cells x {cd1} days in {single_s:.1f}s"}
+        print(f"# cpu oracle {cpu_value:.4g} cell-days/s on {threads} threads, "
+              f"{n * cd1 / single_s:.4g} on 1", file=sys.stderr, flush=True)
 
     line = {
         "metric": "cell-days/s at global 0.5deg (FP64)",

@@ -46,6 +46,7 @@ static const short days_in_month[12] = {31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 
 
 struct wgo_model {
   int32_t n;
+  int threads;             /* OpenMP threads of the vertical loop */
   /* statics (reference types) */
   const wg_static_fields *st;
   const wg_lct_tables *lct;
@@ -176,6 +177,7 @@ int wgo_rout_order(int32_t n, const int32_t *inflow9, const int32_t *outflow, in
 
 wgo_model *wgo_create(int32_t n, const wg_static_fields *st) {
   wgo_model *m = (wgo_model *)calloc(1, sizeof(wgo_model));
+  if (m) m->threads = 1;
   const double *par = st->params;
   m->n = n;
   m->st = st;
@@ -1178,7 +1180,9 @@ static void fractions_cell(wgo_model *m, int32_t i) {
 void wgo_step_day(wgo_model *m, const float *temp, const float *prec, const float *sw, const float *lw,
                   int day, int month) {
   const int32_t n = m->n;
-  /* integrateWGHM.cpp:767-783: calcNewDay for every continental cell */
+  /* integrateWGHM.cpp:767-783: calcNewDay for every continental cell; cells are
+   * independent, so the OpenMP split over cells (as the reference's) changes no bit */
+#pragma omp parallel for schedule(static) num_threads(m->threads)
   for (int32_t i = 0; i < n; i++)
     if (m->contcell[i]) vertical_cell(m, i, temp[i], prec[i], sw[i], lw[i]);
   /* routing.cpp:1775-1811: zero the inflow accumulators */
@@ -1191,6 +1195,8 @@ void wgo_step_day(wgo_model *m, const float *temp, const float *prec, const floa
   for (int32_t i = 0; i < n; i++) fractions_cell(m, i);
   m->started = 1;
 }
+
+void wgo_set_threads(wgo_model *m, int threads) { m->threads = threads > 0 ? threads : 1; }
 
 void wgo_run(wgo_model *m, const float *forcing, int32_t n_days, int32_t first_doy) {
   const size_t n = (size_t)m->n;
@@ -1325,6 +1331,9 @@ void wgo_perturb_forcing(uint64_t seed, int32_t day_index, int32_t n, double pre
     temp_out[i] = (float)((double)temp_in[i] + temp_sigma * zt);
   }
 }
+
+/* One Philox4x32-10 block, exported for the known-answer test. */
+void wgo_philox(uint32_t ctr[4], const uint32_t key[2]) { philox4x32_10(ctr, key[0], key[1]); }
 
 /* The math functions in use, exported for the tests. */
 double wgo_exp(double x) { return EXP(x); }

@@ -45,6 +45,10 @@ void wgo_set_params(wgo_model *m, const double *gamma, const double *cfa, const 
 void wgo_step_day(wgo_model *m, const float *temp, const float *prec, const float *sw,
                   const float *lw, int day, int month);
 
+/* Threads of the vertical loop (OpenMP over cells, as integrateWGHM.cpp:767;
+ * results do not depend on it).  Default 1. */
+void wgo_set_threads(wgo_model *m, int threads);
+
 /* Runs n_days from a [D][4][N] forcing block starting at doy first_doy. */
 void wgo_run(wgo_model *m, const float *forcing, int32_t n_days, int32_t first_doy);
 
@@ -62,6 +66,9 @@ void wgo_reset_station_discharge(wgo_model *m);
 void wgo_perturb_forcing(uint64_t seed, int32_t day_index, int32_t n, double prec_sigma,
                          double temp_sigma, const float *temp_in, const float *prec_in,
                          float *temp_out, float *prec_out);
+
+/* One Philox4x32-10 block: ctr is replaced by the output. */
+void wgo_philox(uint32_t ctr[4], const uint32_t key[2]);
 
 /* The exp, pow and log in use (wg_libm.h, or the C library's in the glibc build). */
 double wgo_exp(double x);

@@ -59,6 +59,10 @@ def load(variant="wg"):
     lib.wgo_perturb_forcing.argtypes = [C.c_uint64, i32, i32, C.c_double, C.c_double, P_f, P_f, P_f, P_f]
     lib.wgo_perturb_forcing.restype = None
     lib.wgo_month_of_day.argtypes = [C.c_int]
+    lib.wgo_set_threads.argtypes = [C.c_void_p, C.c_int]
+    lib.wgo_set_threads.restype = None
+    lib.wgo_philox.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    lib.wgo_philox.restype = None
     for fn in ("wgo_exp", "wgo_log"):
         getattr(lib, fn).argtypes = [C.c_double]
         getattr(lib, fn).restype = C.c_double
@@ -91,6 +95,14 @@ def inflow_outflow(world):
     return inflow9, outflow
 
 
+def philox(ctr, key):
+    """One Philox4x32-10 block of the oracle's generator."""
+    c = (C.c_uint32 * 4)(*[int(v) for v in ctr])
+    k = (C.c_uint32 * 2)(*[int(v) for v in key])
+    load().wgo_philox(c, k)
+    return [int(v) for v in c]
+
+
 def perturb_forcing(seed, day_index, temp, prec, prec_sigma, temp_sigma):
     lib = load()
     temp = np.ascontiguousarray(temp, dtype=np.float32)
@@ -121,6 +133,9 @@ class Oracle:
                 self._m = None
         except Exception:
             pass
+
+    def set_threads(self, threads):
+        self._lib.wgo_set_threads(self._m, int(threads))
 
     def set_params(self, gamma=None, cfa=None, smax=None):
         g = None if gamma is None else np.ascontiguousarray(gamma, dtype=np.float64)
