@@ -142,8 +142,9 @@ def main():
     model.synchronize()
     elapsed = time.perf_counter() - t_start
     timed_days = args.steps * dps
-    value = n * timed_days / elapsed
-    ms_per_step = 1000.0 * elapsed / args.steps
+    # --steps 0 (profiling runs) times nothing
+    value = n * timed_days / elapsed if timed_days else 0.0
+    ms_per_step = 1000.0 * elapsed / args.steps if args.steps else 0.0
     print(f"# timed {timed_days} days in {elapsed:.3f}s -> {value:.4g} cell-days/s",
           file=sys.stderr, flush=True)
 
