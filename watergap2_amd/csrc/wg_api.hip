@@ -613,7 +613,18 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
       for (int32_t p = a; p < b; p++) gstart[p] = a;
       H->max_group_cells = std::max(H->max_group_cells, b - a);
     }
+    if (H->max_group_cells > 32767) {
+      *out = h.release();
+      return fail(*out, WG_E_INPUT, "a routing group of %d cells exceeds the 32767-cell limit of the "
+                  "16-bit upstream indices (largest basin too large)", H->max_group_cells);
+    }
     std::vector<int32_t> off(n + 1, 0), idx;
+    for (int32_t p = 0; p < n; p++)
+      if (ups[p].size() > 8) {
+        *out = h.release();
+        return fail(*out, WG_E_INPUT, "cell %d has %d upstream cells (D8 allows 8)", H->perm[p],
+                    int(ups[p].size()));
+      }
     for (int32_t p = 0; p < n; p++) {
       std::sort(ups[p].begin(), ups[p].end());
       off[p + 1] = off[p] + int32_t(ups[p].size());
@@ -648,13 +659,11 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
       if (st->aindex[i] == 1 && st->ldd[i] >= 0) f |= CFL_ARID;
       if (st->ldd[i] >= 0) f |= CFL_LDD;
       if (contcell[i]) f |= CFL_CONT;
+      if (st->reservoir_area[i] > 0.) f |= (int(st->res_type[i]) & 0xff) << 8 | (int(st->start_month[i]) & 0xff) << 16;
       c.flags = f;
       c.station = -1;
-      c.u0 = off[p];
       c.nu = off[p + 1] - off[p];
-      c.up0 = c.nu > 0 ? idx[off[p]] : 0;
-      c.up1 = c.nu > 1 ? idx[off[p] + 1] : 0;
-      c.up2 = c.nu > 2 ? idx[off[p] + 2] : 0;
+      for (int u = 0; u < 8; u++) c.up[u] = int16_t(u < c.nu ? idx[off[p] + u] : 0);
       c.pad = 0;
     }
     WgChainStatic *dcs;
@@ -965,6 +974,13 @@ int enqueue_block(wg_handle *h, int days, hipEvent_t *evs) {
                        size_t(h->max_group_cells) * sizeof(double), h->stream, h->S, h->X, h->T,
                        h->d_level_off, h->d_group_first, h->d_clock, k, slot);
     HIP_CHECK(h, hipGetLastError());
+    if (h->n_stations > 0 && (slot == WG_DAILY_SLOTS - 1 || k == days - 1)) {
+      // fold the daily station series into the running sums, in day order
+      const int nsm = h->n_stations * M;
+      hipLaunchKernelGGL(wg_station_reduce, dim3((nsm + 255) / 256), dim3(256), 0, h->stream, h->X,
+                         h->n_stations, M, 0, slot + 1);
+      HIP_CHECK(h, hipGetLastError());
+    }
   }
   if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], h->stream));
   // closing pass: finish the last day so that the state is complete

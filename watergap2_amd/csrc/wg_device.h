@@ -18,15 +18,20 @@ struct alignas(16) WgD2 {
 };
 
 // Static part of a cell's routing record (device order): the constants of the
-// upstream-dependent routing step, packed for one contiguous load.
+// upstream-dependent routing step, packed for one contiguous load.  flags holds the
+// chain flags in bits 0-7, the reservoir type in bits 8-15 and the reservoir's
+// start month in bits 16-23; up[] the group-local indices of the (at most 8, D8)
+// upstream cells in routing order.
 struct alignas(16) WgChainStatic {
   double inv_ksw, ome_sw, inv_kgw, ome_gw, area, cf100;
-  int32_t flags, station, u0, nu, up0, up1, up2, pad;
+  int32_t flags, station;
+  int16_t up[8];
+  int32_t nu, pad;
 };
 
 // Daily part of a cell's routing record, written by the cell kernel for cells
 // with upstream inflow (field meanings: ChainField in wg_kernels.hip).
-#define WG_DEV_NREC 30
+#define WG_DEV_NREC 36
 struct alignas(16) WgChainRec {
   double v[WG_DEV_NREC];
 };

@@ -93,13 +93,19 @@ enum ChainField : int {
   CF_GW_PET, CF_GW_MAX, CF_GW_PREV, CF_GW_S0E, CF_GW_GWR,
   CF_A_GWR_LL, CF_A_GWR_LW, CF_A_RECH, CF_A_PREV, CF_A_S0E,
   CF_OW_PREC, CF_OW_PET,
+  // reservoir inputs (routing.cpp:2807-2860): previous storage, evaporation
+  // reduction factor, CFA, release coefficient, capacity, mean outflow, area
+  CF_RES_PREV, CF_RES_RED, CF_RES_CFA, CF_RES_KREL, CF_RES_CAP, CF_RES_QMEAN, CF_RES_AREA,
   CF_COUNT
 };
+static_assert(CF_COUNT == WG_DEV_NREC, "routing record size");
 
-// chain flags (static per cell)
+// chain flags (static per cell); bits 8-15 reservoir type, 16-23 start month
 enum {
   CFL_LAKE = 1, CFL_RES = 2, CFL_GWET = 4, CFL_ARID = 8, CFL_LDD = 16, CFL_CONT = 32
 };
+__device__ inline int flag_res_type(int flags) { return (flags >> 8) & 0xff; }
+__device__ inline int flag_start_month(int flags) { return (flags >> 16) & 0xff; }
 
 struct ChainRec {
   double v[CF_COUNT];
@@ -130,8 +136,7 @@ __device__ inline void load_chain_rec(const WgStatic &S, const WgState &X, int64
 // use, no station correction).  inflow_up is G_riverInflow[n] when the cell is
 // reached.  The reservoir branch (about one cell in two hundred) reads its own
 // inputs here.
-__device__ ChainOut chain_step(const WgStatic &S, const WgState &X, const ChainRec &c, int64_t mp, int p,
-                               double inflow_up, int day, int month) {
+__device__ ChainOut chain_step(const ChainRec &c, double inflow_up, int day, int month) {
   ChainOut o;
   const double *v = c.v;
   const bool arid_swb = (c.flags & CFL_ARID) != 0;
@@ -175,11 +180,11 @@ __device__ ChainOut chain_step(const WgStatic &S, const WgState &X, const ChainR
 
   // reservoir, routing.cpp:2807-3082
   if (c.flags & CFL_RES) {
-    const double res_area = S.reservoir_area[p];
-    const double prev = X.res[mp], rf = X.red_res[mp], cfa = X.cfa[mp];
-    const double cap = S.stor_cap[p], qmean = S.mean_outflow[p];
+    const double res_area = v[CF_RES_AREA];
+    const double prev = v[CF_RES_PREV], rf = v[CF_RES_RED], cfa = v[CF_RES_CFA];
+    const double cap = v[CF_RES_CAP], qmean = v[CF_RES_QMEAN];
     const double ow_prec = v[CF_OW_PREC], ow_pet = v[CF_OW_PET];
-    const double cellArea = c.area, cf = S.contfreq[p];
+    const double cellArea = c.area, cf100 = c.cf100;
     double evap, PETgwr, PETgwrMax, st, prov_rel = 0., release;
     const double c_ratio = cap / (qmean * 31536000. / 1000000000.);
     const double maxStorage = cap;
@@ -187,10 +192,10 @@ __device__ ChainOut chain_step(const WgStatic &S, const WgState &X, const ChainR
     if (evap < 0.) evap = 0.;
     const double totalInflow = inflow + (ow_prec * (res_area / 1000000.));
     if (arid_swb)
-      gwr_res = kKswbgw * rf * (res_area / (cellArea * (cf / 100.)));
+      gwr_res = kKswbgw * rf * (res_area / (cellArea * cf100));
     else
       gwr_res = 0.;
-    PETgwr = evap * (res_area / 1000000.) + gwr_res * cellArea * (cf / 100.) / 1000000.;
+    PETgwr = evap * (res_area / 1000000.) + gwr_res * cellArea * cf100 / 1000000.;
     PETgwrMax = prev + totalInflow;
     if (PETgwr > PETgwrMax) {
       st = prev + totalInflow - PETgwrMax;
@@ -199,8 +204,8 @@ __device__ ChainOut chain_step(const WgStatic &S, const WgState &X, const ChainR
       st = prev + totalInflow - PETgwr;
     }
     if (fabs(st) <= kMinStorVol) st = 0.;
-    double krel = X.k_release[mp];
-    if (month == S.start_month[p] - 1) {
+    double krel = v[CF_RES_KREL];
+    if (month == flag_start_month(c.flags) - 1) {
       if (day == kFirstDayInMonth[month]) {
         if (st < (cap * 0.1))
           krel = 0.1;
@@ -208,7 +213,7 @@ __device__ ChainOut chain_step(const WgStatic &S, const WgState &X, const ChainR
           krel = st / (maxStorage * 0.85);
       }
     }
-    const int rtype = S.res_type[p];
+    const int rtype = flag_res_type(c.flags);
     if (rtype == 1) {
       // irrigation reservoir without water use: zero demand and zero downstream use
       const double mean_demand = 0.;
@@ -616,7 +621,12 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   if (do_post) post_day(S, X, mp, p);
   // calcNewDay and routing run for continental cells only (integrateWGHM.cpp:770,
   // routing.cpp:1876); other cells keep a zero transported volume.
-  if (!do_vertical || !S.contcell[p]) return;
+  if (!do_vertical) return;
+  if (!S.contcell[p]) {
+    const int st = S.station[p];
+    if (st >= 0) X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + st] = 0.;
+    return;
+  }
 
   const int64_t fday = clock->day0 + k;
   const int doy = int((clock->doy0 - 1 + k) % 365) + 1;
@@ -1136,21 +1146,28 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     v[CF_A_PREV] = gprev;
     v[CF_A_S0E] = gprev * S.e_gw[p];
   }
+  {
+    // reservoir, routing.cpp:2807-2860 inputs
+    const bool res = (c.flags & CFL_RES) != 0;
+    v[CF_RES_PREV] = res ? X.res[mp] : 0.;
+    v[CF_RES_RED] = res ? X.red_res[mp] : 0.;
+    v[CF_RES_CFA] = cfa;
+    v[CF_RES_KREL] = res ? X.k_release[mp] : 0.;
+    v[CF_RES_CAP] = res ? S.stor_cap[p] : 0.;
+    v[CF_RES_QMEAN] = res ? S.mean_outflow[p] : 0.;
+    v[CF_RES_AREA] = res ? S.reservoir_area[p] : 0.;
+  }
   if (cs.nu == 0) {
     // no upstream cell: G_riverInflow[n] is still 0 when the cell is routed
-    const ChainOut o = chain_step(S, X, c, mp, p, 0.0, doy, month);
+    const ChainOut o = chain_step(c, 0.0, doy, month);
     store_chain(X, mp, c.flags, o);
     const int st = cs.station;
-    if (st >= 0) {
-      const double q = (ldd >= 0) ? o.tv : 0.;
-      X.st_sum[int64_t(m) * S.n_stations + st] += q;
-      X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + st] = q;
-    }
+    if (st >= 0)
+      X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + st] = (ldd >= 0) ? o.tv : 0.;
   } else {
     WgChainRec rec;
 #pragma unroll
     for (int f = 0; f < CF_COUNT; f++) rec.v[f] = v[f];
-    rec.v[WG_DEV_NREC - 1] = 0.;
     X.crec[mp] = rec;
   }
 }
@@ -1168,26 +1185,67 @@ __device__ inline void lds_barrier() {
 // Inputs of one chain cell, prefetched a level ahead.
 struct ChainSlot {
   ChainRec c;
-  int u0, nu, station;
-  int up[3];
+  int nu, station;
+  int up[4];  // group-local upstream indices, two int16 per word
 };
 
 __device__ inline void prefetch_slot(const WgStatic &S, const WgState &X, int64_t base, int p, ChainSlot &q) {
   WgChainStatic cs;
   load_chain_rec(S, X, base + p, p, q.c, cs);
   q.station = cs.station;
-  q.u0 = cs.u0;
   q.nu = cs.nu;
-  q.up[0] = cs.up0;
-  q.up[1] = cs.up1;
-  q.up[2] = cs.up2;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    q.up[k] = int(uint16_t(cs.up[2 * k])) | (int(uint16_t(cs.up[2 * k + 1])) << 16);
+}
+
+// One chain cell: ordered gather of the upstream transported volumes (the serial
+// += of routing.cpp:3955-3958, in routing order), the upstream-dependent routing
+// step, LDS and global stores, the station's daily discharge.
+__device__ __forceinline__ void chain_cell(const WgStatic &S, const WgState &X, double *s_tv, const ChainSlot &q,
+                                           int64_t base, int p, int gs, int m, int doy, int month,
+                                           int daily_slot) {
+  double qq = 0.;
+  if (q.c.flags & CFL_CONT) {
+    double up = 0.;
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (u < q.nu) up += s_tv[(q.up[u >> 1] >> ((u & 1) * 16)) & 0xffff];
+    const ChainOut o = chain_step(q.c, up, doy, month);
+    s_tv[p - gs] = o.tv;
+    store_chain(X, base + p, q.c.flags, o);
+    qq = (q.c.flags & CFL_LDD) ? o.tv : 0.;
+  }
+  if (q.station >= 0) X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + q.station] = qq;
+}
+
+// One level of a group: the thread's first cell comes from `use` (loaded a level
+// earlier); the first cell of the next level is loaded into `fill` before any
+// computation, so its latency overlaps this level.  `use` and `fill` alternate
+// between two named slots, never copied, so no wait on the new loads is forced.
+// Cells beyond the block size (levels wider than the block) load directly.
+__device__ __forceinline__ void chain_level(const WgStatic &S, const WgState &X, double *s_tv,
+                                            const int32_t *__restrict__ lo, int l, int l1, int64_t base, int gs,
+                                            int m, int doy, int month, int daily_slot, const ChainSlot &use,
+                                            ChainSlot &fill) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int beg = lo[l], end = lo[l + 1];
+  if (l + 1 < l1) {
+    const int p = end + tid;
+    if (p < lo[l + 2]) prefetch_slot(S, X, base, p, fill);
+  }
+  if (beg + tid < end) chain_cell(S, X, s_tv, use, base, beg + tid, gs, m, doy, month, daily_slot);
+  for (int p = beg + tid + nt; p < end; p += nt) {
+    ChainSlot q;
+    prefetch_slot(S, X, base, p, q);
+    chain_cell(S, X, s_tv, q, base, p, gs, m, doy, month, daily_slot);
+  }
+  lds_barrier();
 }
 
 // One workgroup per (routing group, member): levels 1..depth of the group.  The
 // transported volumes of the group live in LDS (level 0 is copied in from the
-// cell kernel's output); up_idx holds group-local indices.  Each thread issues
-// the loads of its next-level cell before computing the current level, so their
-// latency overlaps the computation and the barrier.
+// cell kernel's output); the static record holds group-local upstream indices.
 __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, WgTables T,
                                                        const int32_t *__restrict__ group_level_off,
                                                        const int32_t *__restrict__ group_first_level,
@@ -1201,58 +1259,35 @@ __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, Wg
   const int tid = threadIdx.x, nt = blockDim.x;
   const int doy = int((clock->doy0 - 1 + k) % 365) + 1;
   const int month = month_of_doy(doy);
+  const int32_t *lo = group_level_off;
   const int l0 = group_first_level[g], l1 = group_first_level[g + 1];
-  const int gs = group_level_off[l0], ge = group_level_off[l1];
-  const int lvl0_end = group_level_off[l0 + 1];
+  const int gs = lo[l0], ge = lo[l1];
+  const int lvl0_end = lo[l0 + 1];
   const int64_t base = int64_t(m) * n;
+  ChainSlot a, b;
+  if (l0 + 1 < l1) {
+    const int p = lo[l0 + 1] + tid;
+    if (p < lo[l0 + 2]) prefetch_slot(S, X, base, p, a);
+  }
   // level 0 was routed by the cell kernel; cells not routed today read as 0
   for (int i = gs + tid; i < ge; i += nt) s_tv[i - gs] = (i < lvl0_end) ? X.tv[base + i] : 0.;
-  ChainSlot cur, nxt;
-  bool have_cur = false;
-  if (l0 + 1 < l1) {
-    const int p = group_level_off[l0 + 1] + tid;
-    if (p < group_level_off[l0 + 2]) {
-      prefetch_slot(S, X, base, p, cur);
-      have_cur = true;
-    }
-  }
   lds_barrier();
-  for (int l = l0 + 1; l < l1; l++) {
-    const int beg = group_level_off[l], end = group_level_off[l + 1];
-    bool have_nxt = false;
-    if (l + 1 < l1) {
-      const int p = group_level_off[l + 1] + tid;
-      if (p < group_level_off[l + 2]) {
-        prefetch_slot(S, X, base, p, nxt);
-        have_nxt = true;
-      }
-    }
-    bool first = true;
-    for (int p = beg + tid; p < end; p += nt) {
-      ChainSlot q;
-      if (first && have_cur)
-        q = cur;
-      else
-        prefetch_slot(S, X, base, p, q);
-      first = false;
-      if (!(q.c.flags & CFL_CONT)) continue;
-      // ordered gather: the serial += of routing.cpp:3955-3958 in routing order
-      double up = 0.;
-      for (int u = 0; u < q.nu; u++) up += s_tv[(u < 3) ? q.up[u] : S.up_idx[q.u0 + u]];
-      const ChainOut o = chain_step(S, X, q.c, base + p, p, up, doy, month);
-      s_tv[p - gs] = o.tv;
-      store_chain(X, base + p, q.c.flags, o);
-      if (q.station >= 0) {
-        const double qq = (q.c.flags & CFL_LDD) ? o.tv : 0.;
-        X.st_sum[int64_t(m) * S.n_stations + q.station] += qq;
-        X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + q.station] = qq;
-      }
-    }
-    lds_barrier();
-    cur = nxt;
-    have_cur = have_nxt;
+  for (int l = l0 + 1; l < l1; l += 2) {
+    chain_level(S, X, s_tv, lo, l, l1, base, gs, m, doy, month, daily_slot, a, b);
+    if (l + 1 < l1) chain_level(S, X, s_tv, lo, l + 1, l1, base, gs, m, doy, month, daily_slot, b, a);
   }
   (void)T;
+}
+
+// Adds the daily station discharge of slots [slot0, slot0 + n_slots) to the running
+// sums, day by day in order (the per-day += of routing.cpp:4232-4238).
+__global__ void wg_station_reduce(WgState X, int32_t n_stations, int32_t n_members, int slot0, int n_slots) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_stations * n_members) return;
+  const int m = i / n_stations, st = i - m * n_stations;
+  double s = X.st_sum[i];
+  for (int d = slot0; d < slot0 + n_slots; d++) s += X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + d) * n_stations + st];
+  X.st_sum[i] = s;
 }
 
 // Sets the day clock read by the kernels of a captured block of days; launched
