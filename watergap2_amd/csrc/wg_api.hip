@@ -71,7 +71,7 @@ struct wg_handle {
   int32_t max_group_cells = 0;
   int n_xcd = 8;
   std::vector<int32_t> tile_start, tile_end, block_group;
-  std::vector<WgChainStatic> cstat;
+  std::vector<WgChainLink> clink;
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms[3] = {0, 0, 0};
@@ -641,11 +641,12 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     r_ = upload(H, &d, none);
     if (r_) { *out = h.release(); return r_; }
     S.station = d;
-    // static routing record
-    H->cstat.assign(n, WgChainStatic{});
+    // routing links and outflow constants of the chain storages
+    H->clink.assign(n, WgChainLink{});
+    std::vector<WgChainStatic> cst(n);
     for (int32_t p = 0; p < n; p++) {
       const int32_t i = H->perm[p];
-      WgChainStatic &c = H->cstat[p];
+      WgChainStatic &c = cst[p];
       c.inv_ksw = inv_ksw[i];
       c.ome_sw = ome_sw[i];
       c.inv_kgw = inv_kgw[i];
@@ -660,14 +661,19 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
       if (st->ldd[i] >= 0) f |= CFL_LDD;
       if (contcell[i]) f |= CFL_CONT;
       if (st->reservoir_area[i] > 0.) f |= (int(st->res_type[i]) & 0xff) << 8 | (int(st->start_month[i]) & 0xff) << 16;
-      c.flags = f;
-      c.station = -1;
-      c.nu = off[p + 1] - off[p];
-      for (int u = 0; u < 8; u++) c.up[u] = int16_t(u < c.nu ? idx[off[p] + u] : 0);
-      c.pad = 0;
+      WgChainLink &lk = H->clink[p];
+      lk.flags = f;
+      lk.station = -1;
+      lk.nu = off[p + 1] - off[p];
+      lk.pad = 0;
+      for (int u = 0; u < 8; u++) lk.up[u] = int16_t(u < lk.nu ? idx[off[p] + u] : 0);
     }
+    WgChainLink *dlk;
+    r_ = upload(H, &dlk, H->clink);
+    if (r_) { *out = h.release(); return r_; }
+    S.clink = dlk;
     WgChainStatic *dcs;
-    r_ = upload(H, &dcs, H->cstat);
+    r_ = upload(H, &dcs, cst);
     if (r_) { *out = h.release(); return r_; }
     S.cstat = dcs;
     int32_t *dt;
@@ -930,9 +936,9 @@ extern "C" int wg_set_stations(wg_handle *h, int32_t n_st, const int32_t *cells)
   }
   HIP_CHECK(h, hipMemcpy(const_cast<int32_t *>(h->S.station), slot.data(), h->n * sizeof(int32_t),
                          hipMemcpyHostToDevice));
-  for (int32_t p = 0; p < h->n; p++) h->cstat[p].station = slot[p];
-  HIP_CHECK(h, hipMemcpy(const_cast<WgChainStatic *>(h->S.cstat), h->cstat.data(),
-                         h->cstat.size() * sizeof(WgChainStatic), hipMemcpyHostToDevice));
+  for (int32_t p = 0; p < h->n; p++) h->clink[p].station = slot[p];
+  HIP_CHECK(h, hipMemcpy(const_cast<WgChainLink *>(h->S.clink), h->clink.data(),
+                         h->clink.size() * sizeof(WgChainLink), hipMemcpyHostToDevice));
   // reallocate the accumulators
   for (auto it = h->bufs.begin(); it != h->bufs.end();) {
     if (it->p == h->X.st_sum || it->p == h->X.st_daily) {

@@ -17,20 +17,24 @@ struct alignas(16) WgD2 {
   double x, y;
 };
 
-// Static part of a cell's routing record (device order): the constants of the
-// upstream-dependent routing step, packed for one contiguous load.  flags holds the
-// chain flags in bits 0-7, the reservoir type in bits 8-15 and the reservoir's
-// start month in bits 16-23; up[] the group-local indices of the (at most 8, D8)
-// upstream cells in routing order.
-struct alignas(16) WgChainStatic {
-  double inv_ksw, ome_sw, inv_kgw, ome_gw, area, cf100;
-  int32_t flags, station;
+// Routing links of a cell (device order): chain flags in bits 0-7, reservoir type
+// in bits 8-15 and the reservoir's start month in bits 16-23; the station slot;
+// the number of upstream cells and their group-local indices in routing order
+// (at most 8, D8).
+struct alignas(16) WgChainLink {
+  int32_t flags, station, nu, pad;
   int16_t up[8];
-  int32_t nu, pad;
 };
 
-// Daily part of a cell's routing record, written by the cell kernel for cells
-// with upstream inflow (field meanings: ChainField in wg_kernels.hip).
+// Outflow constants of the storages on the routing chain (global lake / wetland,
+// arid groundwater).
+struct alignas(16) WgChainStatic {
+  double inv_ksw, ome_sw, inv_kgw, ome_gw, area, cf100;
+};
+
+// Routing record of a cell with upstream inflow, written by the cell kernel: a
+// prefix read by every routed cell and a suffix written and read only for cells
+// with a global lake or a reservoir (field meanings: ChainField in wg_kernels.hip).
 #define WG_DEV_NREC 36
 struct alignas(16) WgChainRec {
   double v[WG_DEV_NREC];
@@ -67,7 +71,8 @@ struct WgStatic {
   const double *k_sw, *e_sw, *inv_ksw, *ome_sw, *k_gw, *e_gw, *inv_kgw, *ome_gw;
   const double *cf100;              // contfreq / 100
   const int8_t *cflags;             // lake / reservoir / global wetland / arid / LDD >= 0 / continental
-  const WgChainStatic *cstat;       // [N] static routing record
+  const WgChainLink *clink;         // [N] routing links
+  const WgChainStatic *cstat;       // [N] outflow constants of chain storages
   // XCD-aware placement: the cell kernel's block b covers cells [tile_start[b],
   // tile_end[b]) (tile_start -1: idle block); the chain kernel's block i routes
   // group block_group[i] (-1: idle).  Block i runs on XCD i % 8.

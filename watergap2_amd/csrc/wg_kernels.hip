@@ -80,6 +80,7 @@ __device__ double irwin_hall(uint64_t seed, uint32_t cell, uint32_t day, uint32_
 // the reference's own expressions, so that the level walk only evaluates the
 // short chain that does.  Stored as one 240-byte record per (member, cell).
 enum ChainField : int {
+  // prefix read by every routed cell: river, global wetland, arid groundwater
   CF_INFLOW = 0,  // outflow of the local lake/wetland system (routing.cpp:2616)
   CF_LRIR,        // G_localRunoffIntoRiver
   CF_LGWRIR,      // G_localGWRunoffIntoRiver of humid cells
@@ -87,11 +88,13 @@ enum ChainField : int {
   CF_RV_TERM,     // K S0 e^-K / (1 - e^-K) (routing.cpp:3477-3478)
   CF_RV_EVAP,     // river evaporation (routing.cpp:3449-3453)
   CF_RV_S0E, CF_RV_INVK, CF_RV_OME, CF_RV_PREV,
-  CF_GL_LP,       // precipitation on the global lake (routing.cpp:2664)
-  CF_GL_USE, CF_GL_MAX, CF_GL_PREV, CF_GL_S0E, CF_GL_GWR,
   CF_GW_WP,       // precipitation on the global wetland (routing.cpp:3212-3213)
   CF_GW_PET, CF_GW_MAX, CF_GW_PREV, CF_GW_S0E, CF_GW_GWR,
   CF_A_GWR_LL, CF_A_GWR_LW, CF_A_RECH, CF_A_PREV, CF_A_S0E,
+  CF_PREFIX,
+  // suffix read only by cells with a global lake or a reservoir
+  CF_GL_LP = CF_PREFIX,  // precipitation on the global lake (routing.cpp:2664)
+  CF_GL_USE, CF_GL_MAX, CF_GL_PREV, CF_GL_S0E, CF_GL_GWR,
   CF_OW_PREC, CF_OW_PET,
   // reservoir inputs (routing.cpp:2807-2860): previous storage, evaporation
   // reduction factor, CFA, release coefficient, capacity, mean outflow, area
@@ -117,19 +120,61 @@ struct ChainOut {
   double glo_lake, res, k_release, glo_wetl, gw, river, tv;
 };
 
-__device__ inline void load_chain_rec(const WgStatic &S, const WgState &X, int64_t mp, int p, ChainRec &r,
-                                      WgChainStatic &cs) {
-  cs = S.cstat[p];
-  const WgChainRec rec = X.crec[mp];
+// Suffix of the record of a cell with a global lake or a reservoir.
+__device__ inline void load_chain_suffix(const WgState &X, int64_t mp, ChainRec &r) {
+  const double *src = X.crec[mp].v;
 #pragma unroll
-  for (int f = 0; f < CF_COUNT; f++) r.v[f] = rec.v[f];
-  r.inv_ksw = cs.inv_ksw;
-  r.ome_sw = cs.ome_sw;
-  r.inv_kgw = cs.inv_kgw;
-  r.ome_gw = cs.ome_gw;
-  r.area = cs.area;
-  r.cf100 = cs.cf100;
-  r.flags = cs.flags;
+  for (int f = CF_PREFIX; f < CF_COUNT; f++) r.v[f] = src[f];
+}
+
+struct RiverOut {
+  double st, tv;
+};
+
+// River storage and transported volume, routing.cpp:3402-3514: riverInflow is the
+// outflow of the cell's surface water bodies (or its inflow), plus local runoff
+// and the precipitation on the river.
+__device__ inline RiverOut river_step(double inflow, int flags, double lrir, double lgwrir, double rv_prec,
+                                      double prev, double rv_evap, double rv_term, double s0e, double invk,
+                                      double ome) {
+  double riverInflow = inflow;
+  if (flags & CFL_LDD) {
+    riverInflow += lrir;
+    riverInflow += lgwrir;
+  }
+  riverInflow += rv_prec;
+  const double evapRemUse = 0. + rv_evap;
+  const double evapRemUseMax = riverInflow + rv_term;
+  RiverOut r;
+  if (evapRemUse > evapRemUseMax) {
+    r.st = 0.;
+    r.tv = riverInflow + prev - evapRemUseMax;
+    if (r.tv < 0.) r.tv = 0.;
+  } else {
+    r.st = s0e + invk * (riverInflow - evapRemUse) * ome;
+    if (fabs(r.st) <= kMinStorVol) r.st = 0.;
+    r.tv = riverInflow + prev - r.st - evapRemUse;
+    if (r.tv < 0.) r.tv = 0.;
+  }
+  return r;
+}
+
+// Groundwater of a (semi-)arid cell with recharge below surface water bodies,
+// routing.cpp:3305-3386.  Returns the groundwater runoff; gw receives the storage.
+__device__ inline double arid_groundwater(double gwr_swb, double area, double cf100, double recharge,
+                                         double gw_prev, double s0e, double inv_kgw, double ome_gw,
+                                         double &gw_out) {
+  const double netGWin = gwr_swb * area * cf100 / 1000000. + recharge;
+  double gw = s0e + inv_kgw * netGWin * ome_gw;
+  if (fabs(gw) <= kMinStorVol) gw = 0.;
+  double gw_runoff = gw_prev - gw + netGWin;
+  if (gw_runoff <= 0.) {
+    gw_runoff = 0.;
+    gw = gw_prev + netGWin;
+    if (fabs(gw) <= kMinStorVol) gw = 0.;
+  }
+  gw_out = gw;
+  return gw_runoff;
 }
 
 // routing.cpp:2623-3958 for one cell from its record (parity profile: no water
@@ -277,41 +322,14 @@ __device__ ChainOut chain_step(const ChainRec &c, double inflow_up, int day, int
   double lgwrir = v[CF_LGWRIR];
   if (arid_swb) {
     const double gwr_swb = v[CF_A_GWR_LL] + gwr_glolak + v[CF_A_GWR_LW] + gwr_glowet + gwr_res;
-    const double gw_prev = v[CF_A_PREV];
-    const double netGWin = gwr_swb * c.area * c.cf100 / 1000000. + v[CF_A_RECH];
-    double gw = v[CF_A_S0E] + c.inv_kgw * netGWin * c.ome_gw;
-    if (fabs(gw) <= kMinStorVol) gw = 0.;
-    double gw_runoff = gw_prev - gw + netGWin;
-    if (gw_runoff <= 0.) {
-      gw_runoff = 0.;
-      gw = gw_prev + netGWin;
-      if (fabs(gw) <= kMinStorVol) gw = 0.;
-    }
-    o.gw = gw;
-    lgwrir = gw_runoff;
+    lgwrir = arid_groundwater(gwr_swb, c.area, c.cf100, v[CF_A_RECH], v[CF_A_PREV], v[CF_A_S0E], c.inv_kgw,
+                              c.ome_gw, o.gw);
   }
 
   // river, routing.cpp:3402-3514
-  double riverInflow = inflow;
-  if (c.flags & CFL_LDD) {
-    riverInflow += v[CF_LRIR];
-    riverInflow += lgwrir;
-  }
-  riverInflow += v[CF_RV_PREC];
-  const double prev = v[CF_RV_PREV];
-  const double evapRemUse = 0. + v[CF_RV_EVAP];
-  const double evapRemUseMax = riverInflow + v[CF_RV_TERM];
-  double tv, st;
-  if (evapRemUse > evapRemUseMax) {
-    st = 0.;
-    tv = riverInflow + prev - evapRemUseMax;
-    if (tv < 0.) tv = 0.;
-  } else {
-    st = v[CF_RV_S0E] + v[CF_RV_INVK] * (riverInflow - evapRemUse) * v[CF_RV_OME];
-    if (fabs(st) <= kMinStorVol) st = 0.;
-    tv = riverInflow + prev - st - evapRemUse;
-    if (tv < 0.) tv = 0.;
-  }
+  const RiverOut r = river_step(inflow, c.flags, v[CF_LRIR], lgwrir, v[CF_RV_PREC], v[CF_RV_PREV], v[CF_RV_EVAP],
+                                v[CF_RV_TERM], v[CF_RV_S0E], v[CF_RV_INVK], v[CF_RV_OME]);
+  const double st = r.st, tv = r.tv;
   o.river = st;
   o.tv = tv;
   return o;
@@ -1058,24 +1076,22 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     X.red_lw[mp] = r;
   }
 
-  // river terms that do not depend on the inflow, routing.cpp:3424-3479
-  // record of the inflow-independent terms of routing.cpp:2630-3514
+  // record of the routing terms that do not depend on the inflow from upstream
+  // (routing.cpp:2630-3514)
+  const WgChainLink lk = S.clink[p];
   ChainRec c;
   double *v = c.v;
   const double e_sw = S.e_sw[p];
-  const WgChainStatic cs = S.cstat[p];
-  c.inv_ksw = cs.inv_ksw;
-  c.ome_sw = cs.ome_sw;
-  c.inv_kgw = cs.inv_kgw;
-  c.ome_gw = cs.ome_gw;
-  c.area = cs.area;
-  c.cf100 = cs.cf100;
-  c.flags = cs.flags;
-  v[CF_INFLOW] = inflow;
-  v[CF_LRIR] = lrir;
-  v[CF_LGWRIR] = lgwrir;
-  v[CF_OW_PREC] = ow_prec;
-  v[CF_OW_PET] = ow_pet;
+  {
+    const WgChainStatic cs = S.cstat[p];
+    c.inv_ksw = cs.inv_ksw;
+    c.ome_sw = cs.ome_sw;
+    c.inv_kgw = cs.inv_kgw;
+    c.ome_gw = cs.ome_gw;
+    c.area = cs.area;
+    c.cf100 = cs.cf100;
+    c.flags = lk.flags;
+  }
   {
     // river velocity and storage terms, routing.cpp:3424-3478
     const double L = S.river_length[p];
@@ -1085,6 +1101,9 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     const double K = velocity / L;
     const double eK = wg_exp(-1. * K);
     const double riverAreaFrac = X.river_frac_next[mp];
+    v[CF_INFLOW] = inflow;
+    v[CF_LRIR] = lrir;
+    v[CF_LGWRIR] = lgwrir;
     v[CF_RV_PREV] = prev;
     v[CF_RV_S0E] = prev * eK;
     v[CF_RV_INVK] = 1. / K;
@@ -1093,6 +1112,8 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     v[CF_RV_EVAP] = ((1.0 - cfa) * (ow_prec) + (cfa * ow_pet)) * riverAreaFrac / 100. * cellArea / 1000000.;
     v[CF_RV_PREC] = ow_prec * riverAreaFrac / 100. * cellArea / 1000000.;
   }
+  v[CF_OW_PREC] = ow_prec;
+  v[CF_OW_PET] = ow_pet;
   {
     // global lake, routing.cpp:2633-2676
     const double lake_area = S.lake_area[p];
@@ -1148,7 +1169,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   }
   {
     // reservoir, routing.cpp:2807-2860 inputs
-    const bool res = (c.flags & CFL_RES) != 0;
+    const bool res = (lk.flags & CFL_RES) != 0;
     v[CF_RES_PREV] = res ? X.res[mp] : 0.;
     v[CF_RES_RED] = res ? X.red_res[mp] : 0.;
     v[CF_RES_CFA] = cfa;
@@ -1157,18 +1178,21 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     v[CF_RES_QMEAN] = res ? S.mean_outflow[p] : 0.;
     v[CF_RES_AREA] = res ? S.reservoir_area[p] : 0.;
   }
-  if (cs.nu == 0) {
+  if (lk.nu == 0) {
     // no upstream cell: G_riverInflow[n] is still 0 when the cell is routed
     const ChainOut o = chain_step(c, 0.0, doy, month);
     store_chain(X, mp, c.flags, o);
-    const int st = cs.station;
-    if (st >= 0)
-      X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + st] = (ldd >= 0) ? o.tv : 0.;
+    if (lk.station >= 0)
+      X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + lk.station] = (ldd >= 0) ? o.tv : 0.;
   } else {
-    WgChainRec rec;
+    // the prefix for every routed cell, the suffix for cells with a global lake or reservoir
+    double *dst = X.crec[mp].v;
 #pragma unroll
-    for (int f = 0; f < CF_COUNT; f++) rec.v[f] = v[f];
-    X.crec[mp] = rec;
+    for (int f = 0; f < CF_PREFIX; f++) dst[f] = v[f];
+    if (lk.flags & (CFL_LAKE | CFL_RES)) {
+#pragma unroll
+      for (int f = CF_PREFIX; f < CF_COUNT; f++) dst[f] = v[f];
+    }
   }
 }
 
@@ -1182,21 +1206,34 @@ __device__ inline void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Inputs of one chain cell, prefetched a level ahead.
+// Inputs of one chain cell, prefetched a level ahead: the record prefix, the
+// outflow constants and the links (the record suffix of a cell with a global
+// lake or reservoir is read when the cell is routed).
 struct ChainSlot {
-  ChainRec c;
-  int nu, station;
+  double a[CF_PREFIX];
+  double inv_ksw, ome_sw, inv_kgw, ome_gw, area, cf100;
+  int flags, nu, station;
   int up[4];  // group-local upstream indices, two int16 per word
 };
 
 __device__ inline void prefetch_slot(const WgStatic &S, const WgState &X, int64_t base, int p, ChainSlot &q) {
-  WgChainStatic cs;
-  load_chain_rec(S, X, base + p, p, q.c, cs);
-  q.station = cs.station;
-  q.nu = cs.nu;
+  const WgChainLink lk = S.clink[p];
+  const WgChainStatic cs = S.cstat[p];
+  const double *src = X.crec[base + p].v;
+#pragma unroll
+  for (int f = 0; f < CF_PREFIX; f++) q.a[f] = src[f];
+  q.inv_ksw = cs.inv_ksw;
+  q.ome_sw = cs.ome_sw;
+  q.inv_kgw = cs.inv_kgw;
+  q.ome_gw = cs.ome_gw;
+  q.area = cs.area;
+  q.cf100 = cs.cf100;
+  q.flags = lk.flags;
+  q.station = lk.station;
+  q.nu = lk.nu;
 #pragma unroll
   for (int k = 0; k < 4; k++)
-    q.up[k] = int(uint16_t(cs.up[2 * k])) | (int(uint16_t(cs.up[2 * k + 1])) << 16);
+    q.up[k] = int(uint16_t(lk.up[2 * k])) | (int(uint16_t(lk.up[2 * k + 1])) << 16);
 }
 
 // One chain cell: ordered gather of the upstream transported volumes (the serial
@@ -1206,15 +1243,29 @@ __device__ __forceinline__ void chain_cell(const WgStatic &S, const WgState &X, 
                                            int64_t base, int p, int gs, int m, int doy, int month,
                                            int daily_slot) {
   double qq = 0.;
-  if (q.c.flags & CFL_CONT) {
+  if (q.flags & CFL_CONT) {
     double up = 0.;
 #pragma unroll
     for (int u = 0; u < 8; u++)
       if (u < q.nu) up += s_tv[(q.up[u >> 1] >> ((u & 1) * 16)) & 0xffff];
-    const ChainOut o = chain_step(q.c, up, doy, month);
+    const int64_t mp = base + p;
+    ChainRec c;
+#pragma unroll
+    for (int f = 0; f < CF_PREFIX; f++) c.v[f] = q.a[f];
+#pragma unroll
+    for (int f = CF_PREFIX; f < CF_COUNT; f++) c.v[f] = 0.;
+    if (q.flags & (CFL_LAKE | CFL_RES)) load_chain_suffix(X, mp, c);
+    c.inv_ksw = q.inv_ksw;
+    c.ome_sw = q.ome_sw;
+    c.inv_kgw = q.inv_kgw;
+    c.ome_gw = q.ome_gw;
+    c.area = q.area;
+    c.cf100 = q.cf100;
+    c.flags = q.flags;
+    const ChainOut o = chain_step(c, up, doy, month);
+    store_chain(X, mp, q.flags, o);
     s_tv[p - gs] = o.tv;
-    store_chain(X, base + p, q.c.flags, o);
-    qq = (q.c.flags & CFL_LDD) ? o.tv : 0.;
+    qq = (q.flags & CFL_LDD) ? o.tv : 0.;
   }
   if (q.station >= 0) X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + q.station] = qq;
 }
