@@ -68,6 +68,52 @@ def chain_kernel_bytes(n_cells, n_members, level0_cells):
     return n_members * (n_chain * (29 * 8 + 16) + level0_cells * 8) + n_chain * 80
 
 
+def ensemble_run(wg, world, forcing, members, serial_members):
+    """Batched ensemble: `members` members of the grid for one year (365 days) in
+    one handle, each with its own Philox forcing perturbation (precipitation x
+    lognormal sigma 0.2, temperature + N(0, 1)) and gamma and Smax scaled by
+    U(0.8, 1.2); against the same members run one by one on the same GPU."""
+    import numpy as np
+    n = world.n
+    rng = np.random.default_rng(2024)
+    gamma = np.tile(world.params[wg.PARAM_INDEX["P_GAMRUN_C"]], (members, 1)) * rng.uniform(0.8, 1.2, (members, n))
+    smax = (np.tile(world.smax, (members, 1)) * rng.uniform(0.8, 1.2, (members, n))).astype(np.float32)
+    seeds = np.arange(1000, 1000 + members, dtype=np.uint64)
+    f365 = np.ascontiguousarray(forcing[:365])
+    m = wg.WaterBalanceModel(world, n_members=members)
+    m.set_member_params(gamma=gamma, smax=smax)
+    m.set_forcing_perturbation(seeds, 0.2, 1.0)
+    m.set_stations(world.basin_outlet)
+    m.load_forcing(f365)
+    m.step_days(0, 1, 365)  # warm-up year (captures the graph)
+    m.synchronize()
+    t0 = time.perf_counter()
+    m.step_days(0, 1, 365)
+    m.synchronize()
+    batched_s = time.perf_counter() - t0
+    m.close()
+    serial_s = 0.0
+    for k in range(serial_members):
+        s1 = wg.WaterBalanceModel(world, n_members=1)
+        s1.set_member_params(gamma=gamma[k:k + 1], smax=smax[k:k + 1])
+        s1.set_forcing_perturbation(seeds[k:k + 1], 0.2, 1.0)
+        s1.load_forcing(f365)
+        s1.step_days(0, 1, 365)
+        s1.synchronize()
+        t0 = time.perf_counter()
+        s1.step_days(0, 1, 365)
+        s1.synchronize()
+        serial_s += time.perf_counter() - t0
+        s1.close()
+    serial_per_member = serial_s / serial_members
+    cd = n * 365.0
+    return {"members": members, "days": 365, "cell_days_per_s": round(members * cd / batched_s, 1),
+            "per_member_cell_days_per_s": round(cd / batched_s, 1), "seconds": round(batched_s, 3),
+            "serial_cell_days_per_s": round(cd / serial_per_member, 1),
+            "serial_sample": f"{serial_members} members run one by one, {serial_per_member:.4f}s per member-year",
+            "speedup_vs_serial": round(serial_per_member * members / batched_s, 2)}
+
+
 def traffic_from_profiles(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprof counter summary
     (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration), or None."""
@@ -94,6 +140,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16,
                     help="OpenMP threads of the CPU oracle's vertical loop (capped at the host's)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ensemble-members", type=int, default=256,
+                    help="members of the batched-ensemble line (0 = skip)")
+    ap.add_argument("--serial-members", type=int, default=4,
+                    help="members timed one by one for the ensemble comparison")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this size (counter calibration) and exit")
     args = ap.parse_args()
@@ -210,6 +260,13 @@ def main():
         print(f"# cpu oracle {cpu_value:.4g} cell-days/s on {threads} threads, "
               f"{n * cd1 / single_s:.4g} on 1", file=sys.stderr, flush=True)
 
+    ens = None
+    if args.ensemble_members > 0:
+        ens = ensemble_run(wg, world, forcing if n_days >= 365 else world.forcing(seed=1, day0=0, n_days=365),
+                           args.ensemble_members, max(1, args.serial_members))
+        print(f"# ensemble {ens['members']} members x 1 year: {ens['cell_days_per_s']:.4g} cell-days/s, "
+              f"{ens['speedup_vs_serial']}x the serial member runs", file=sys.stderr, flush=True)
+
     line = {
         "metric": "cell-days/s at global 0.5deg (FP64)",
         "value": round(value, 1),
@@ -229,6 +286,7 @@ def main():
                    "routing_levels": info["max_depth"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "ensemble": ens,
     }
     print(json.dumps(line))
 
