@@ -711,6 +711,7 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
   if (!rc) rc = dev_alloc(H, &X.lai_status, NM);
   if (!rc) rc = dev_alloc(H, &X.smax, NM);
   if (!rc) rc = dev_alloc(H, &X.snow2, NM * WG_DEV_NPAIR);
+  if (!rc) rc = dev_alloc(H, &X.snow_any, NM);
   if (!rc) rc = dev_alloc(H, &X.st_sum, 1);
   if (!rc) rc = dev_alloc(H, &X.st_daily, 1);
   if (!rc) rc = dev_alloc(H, &H->d_clock, 1);
@@ -848,10 +849,16 @@ extern "C" int wg_set_state(wg_handle *h, int32_t member, const wg_state *in) {
       const double *src = in->snow_elev + size_t(h->perm[p]) * WG_NELEV;
       for (int k = 0; k < WG_DEV_NPAIR; k++) se[size_t(k) * n + p] = WgD2{src[1 + 2 * k], src[2 + 2 * k]};
     }
+    // sub-cell storages that are all exactly zero need not be read by the kernel
+    std::vector<uint8_t> any(n, 0);
+    for (int32_t p = 0; p < n; p++)
+      for (int k = 0; k < WG_DEV_NPAIR; k++)
+        if (se[size_t(k) * n + p].x != 0. || se[size_t(k) * n + p].y != 0.) any[p] = 1;
     for (int32_t mm = m0; mm < m1; mm++) {
       for (int32_t i = 0; i < n; i++) h->snow_ch0[size_t(mm) * n + i] = in->snow_elev[size_t(i) * WG_NELEV];
       HIP_CHECK(h, hipMemcpy(h->X.snow2 + size_t(mm) * n * WG_DEV_NPAIR, se.data(), se.size() * sizeof(WgD2),
                              hipMemcpyHostToDevice));
+      HIP_CHECK(h, hipMemcpy(h->X.snow_any + size_t(mm) * n, any.data(), n, hipMemcpyHostToDevice));
     }
   }
   h->started = in->started != 0;

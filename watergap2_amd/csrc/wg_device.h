@@ -88,6 +88,7 @@ struct WgState {
   double *canopy, *soil, *snow, *transfer, *lai_precsum;
   int32_t *lai_days, *lai_status;
   WgD2 *snow2;                  // [M][50][N] sub-cell snow storages 2k+1, 2k+2
+  uint8_t *snow_any;            // [M][N] 0: every sub-cell storage is exactly 0
   double *gw, *loc_lake, *loc_wetl, *glo_lake, *glo_wetl, *res, *river;
   double *red_ll, *red_lw, *red_gl, *red_gw, *red_res, *red_river;
   double *river_frac_next, *river_frac_change, *fswb_next, *laf, *laf_prev, *k_release;

@@ -775,13 +775,18 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   const int elev0 = S.elev0[p];
   WgD2 *sp = X.snow2 + int64_t(m) * WG_DEV_NPAIR * n + p;
   const int32_t *ep = S.elev_pairs + p;
+  // A cell whose sub-cell storages are all exactly zero (snow_any == 0) starts the
+  // loop from zeros without reading them, and stores only storages that become
+  // non-zero: on warm days such a cell moves no snow bytes at all.
+  const bool snow_any = X.snow_any[mp] != 0;
   if (landAreaFrac <= 0.) {
     for (int k = 0; k < WG_DEV_NPAIR; k++) {
-      const WgD2 v = sp[int64_t(k) * n];
+      const WgD2 v = snow_any ? sp[int64_t(k) * n] : WgD2{0., 0.};
       transfer += v.x / 100.;
       transfer += v.y / 100.;
-      sp[int64_t(k) * n] = WgD2{0., 0.};
+      if (snow_any) sp[int64_t(k) * n] = WgD2{0., 0.};
     }
+    X.snow_any[mp] = 0;
   } else {
     SnowDay sd;
     sd.tempC = dailyTempC;
@@ -794,11 +799,12 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     sd.snowfz = T_SNOWFZ;
     sd.snowmt = T_SNOWMT;
     sd.melt_factor = DEGDAY_F * ddf;
+    bool any_after = false;
     WgD2 sa[kSnowChunk], sb[kSnowChunk];
     int32_t ea[kSnowChunk], eb[kSnowChunk];
 #pragma unroll
     for (int j = 0; j < kSnowChunk; j++) {
-      sa[j] = sp[int64_t(j) * n];
+      sa[j] = snow_any ? sp[int64_t(j) * n] : WgD2{0., 0.};
       ea[j] = ep[int64_t(j) * n];
     }
 #pragma unroll 1
@@ -807,7 +813,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
       if (more) {
 #pragma unroll
         for (int j = 0; j < kSnowChunk; j++) {
-          sb[j] = sp[int64_t((c + 1) * kSnowChunk + j) * n];
+          sb[j] = snow_any ? sp[int64_t((c + 1) * kSnowChunk + j) * n] : WgD2{0., 0.};
           eb[j] = ep[int64_t((c + 1) * kSnowChunk + j) * n];
         }
       }
@@ -817,7 +823,9 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
         WgD2 v = sa[j];
         v.x = snow_subcell(v.x, int(int16_t(e2 & 0xffff)), c == 0 && j == 0, sd, acc);
         v.y = snow_subcell(v.y, int(int16_t(e2 >> 16)), 0, sd, acc);
-        sp[int64_t(c * kSnowChunk + j) * n] = v;
+        const bool nz = v.x != 0. || v.y != 0.;
+        any_after |= nz;
+        if (snow_any || nz) sp[int64_t(c * kSnowChunk + j) * n] = v;
       }
       if (more) {
 #pragma unroll
@@ -827,6 +835,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
         }
       }
     }
+    X.snow_any[mp] = any_after ? 1 : 0;
     dailySnowEvapo = acc.evapo;
     acc.snow_sum /= 100.;
     acc.eff /= 100.;
