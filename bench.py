@@ -114,6 +114,63 @@ def ensemble_run(wg, world, forcing, members, serial_members):
             "speedup_vs_serial": round(serial_per_member * members / batched_s, 2)}
 
 
+def calibration_run(args):
+    """Metric 2: basins calibrated per hour at the 1 % criterion, plain search.
+
+    120 synthetic basins (log-uniform 50-5,000 cells), targets from a hidden
+    seeded gamma per basin, 30 years; the reference's gamma search for all basins
+    side by side (one model run of the whole set per search step); and the batched
+    evaluation of 64 (gamma, CFA) candidates per basin in one run."""
+    import numpy as np
+    import watergap2_amd as wg
+    from watergap2_amd.calibration import BasinRunner, make_basins, plain_search
+    t0 = time.time()
+    bs = make_basins(args.cal_basins, seed=2024)
+    rng = np.random.default_rng(77)
+    hidden = rng.uniform(0.1, 5.0, bs.n_basins)
+    years = args.cal_years
+    runner = BasinRunner(bs, n_members=1, forcing_years=3)
+    print(f"# calibration set: {bs.n_basins} basins, {bs.n} cells (largest {np.diff(bs.offsets).max()}), "
+          f"setup {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    t1 = time.perf_counter()
+    measured = runner.run(hidden[None, :], n_years=years)[0].astype(np.float32)
+    target_s = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    res = plain_search(runner, measured, n_years=years)
+    plain_s = time.perf_counter() - t1
+    ok = int((res["status"] == 1).sum())
+    print(f"# plain search: {res['model_runs']} model runs in {plain_s:.1f}s, {ok}/{bs.n_basins} basins "
+          f"within 1 %", file=sys.stderr, flush=True)
+    runner.close()
+    cand = None
+    if args.cal_candidates > 0:
+        K = args.cal_candidates
+        cr = BasinRunner(bs, n_members=K, forcing_years=3)
+        g = rng.uniform(0.1, 5.0, (K, bs.n_basins))
+        c = rng.uniform(0.5, 1.5, (K, bs.n_basins))
+        cr.run(g, c, n_years=1)  # warm-up (graph capture)
+        t1 = time.perf_counter()
+        cr.run(g, c, n_years=years)
+        cand_s = time.perf_counter() - t1
+        cr.close()
+        cand = {"candidates": K, "seconds": round(cand_s, 2),
+                "cell_days_per_s": round(bs.n * K * 365.0 * years / cand_s, 1),
+                "basin_candidate_runs_per_hour": round(bs.n_basins * K / cand_s * 3600, 1)}
+        print(f"# {K} candidates per basin in one run: {cand_s:.1f}s, "
+              f"{cand['cell_days_per_s']:.4g} cell-days/s", file=sys.stderr, flush=True)
+    return {"metric": "basins calibrated/hr at the 1% criterion (plain gamma search)",
+            "value": round(ok / plain_s * 3600.0, 1), "unit": "basins/hr", "higher_is_better": True,
+            "config": {"workload": f"{bs.n_basins} synthetic basins, log-uniform 50-5,000 cells, "
+                                   f"{bs.n} cells, {years} years, targets from hidden gamma",
+                       "basins": bs.n_basins, "cells": bs.n, "years": years},
+            "plain": {"seconds": round(plain_s, 2), "model_runs": res["model_runs"],
+                      "runs_per_basin_mean": round(float(res["runs"].mean()), 2),
+                      "within_1pc": ok, "status_counts": {int(k): int(v) for k, v in
+                                                          zip(*np.unique(res["status"], return_counts=True))},
+                      "target_run_seconds": round(target_s, 2)},
+            "candidates": cand}
+
+
 def traffic_from_profiles(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprof counter summary
     (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration), or None."""
@@ -144,6 +201,11 @@ def main():
                     help="members of the batched-ensemble line (0 = skip)")
     ap.add_argument("--serial-members", type=int, default=4,
                     help="members timed one by one for the ensemble comparison")
+    ap.add_argument("--calibration", action="store_true",
+                    help="measure the calibration metric (basins/hr) instead of the day loop")
+    ap.add_argument("--cal-basins", type=int, default=120)
+    ap.add_argument("--cal-years", type=int, default=30)
+    ap.add_argument("--cal-candidates", type=int, default=64)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this size (counter calibration) and exit")
     args = ap.parse_args()
@@ -156,6 +218,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": a.numel() * 4, "launches": 3}))
+        return
+
+    if args.calibration:
+        print(json.dumps(calibration_run(args)))
         return
 
     if args.gpus != 1:

@@ -266,6 +266,35 @@ int wg_schedule_info(wg_handle *h, int32_t *info);
  * host arrays.  Used to check that the device and the host produce the same bits. */
 int wg_libm_eval(int32_t fn, int32_t n, const double *x, const double *y, double *out);
 
+/* ---- calibration: the per-basin search for the runoff coefficient gamma ---- */
+
+/* State of one basin's search: the function-static variables of
+ * calibGammaClass::findNewGamma (calibration.cpp:275-284) and the members it
+ * updates (calibration.h:13-15, 46-48), one struct per basin so that basins can be
+ * searched side by side. */
+typedef struct wg_gamma_search {
+  float gamma_low, gamma_high;   /* bracket, -99 = not yet found */
+  float sum_diff_old;            /* sumOfDifferences of the previous call */
+  int32_t call_counter;
+  int32_t gamma_cond;            /* 1 found, 2 limit reached (10 % rule / CFA) */
+  int32_t calib_status;          /* 1 gamma, 2 gamma + 10 % rule, 3 + CFA */
+  int32_t cell_corr_factor_ind;  /* 99: correction factor computed */
+  float cell_corr_factor;        /* basin-wide CFA */
+  float gamma_upper_limit, gamma_lower_limit;  /* 5 and 0.1 (calibration.cpp:45-48) */
+  float last_nash_sutcliffe, last_sum_diff, measured_avg_adapt;  /* CALIBRATION.OUT columns */
+} wg_gamma_search;
+
+/* Fresh search state (calibGammaClass constructor + function statics). */
+void wg_gamma_search_init(wg_gamma_search *s);
+/* Start value of gamma from the parameter file (calibration.cpp:196-205). */
+float wg_gamma_start(const wg_gamma_search *s, float gamma_from_file);
+/* calibGammaClass::findNewGamma (calibration.cpp:266-636): compares the annual
+ * simulated and observed discharge of the evaluation years (observed <= -1 = no
+ * data) and returns the next gamma, or -99 when the search ends.  sim_inflow and
+ * sim_water_use may be NULL (zero). */
+float wg_find_new_gamma(wg_gamma_search *s, float gamma_old, int32_t n_years, const float *measured,
+                        const float *simulated, const float *sim_inflow, const float *sim_water_use);
+
 #ifdef __cplusplus
 }
 #endif

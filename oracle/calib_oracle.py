@@ -1,0 +1,154 @@
+"""CPU restatement of the calibration search (test infrastructure only).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, never by the product package.
+
+find_new_gamma restates calibGammaClass::findNewGamma (calibration.cpp:266-636)
+with numpy float32 scalars for the reference's float variables; the function
+statics (calibration.cpp:275-284) live in a GammaSearchState object.
+"""
+import numpy as np
+
+f32 = np.float32
+
+
+class GammaSearchState:
+    """Function statics and members of calibGammaClass used by findNewGamma."""
+
+    def __init__(self):
+        self.gamma_low = f32(-99)            # calibration.cpp:282
+        self.gamma_high = f32(-99)           # calibration.cpp:283
+        self.sum_diff_old = f32(0)           # calibration.cpp:275
+        self.call_counter = 0                # calibration.cpp:278
+        self.gamma_cond = 0                  # calibration.h:14
+        self.calib_status = 0                # calibration.h:15
+        self.cell_corr_factor_ind = 1        # calibration.h:13
+        self.cell_corr_factor = f32(1)       # calibration.h:48
+        self.upper = f32(5.)                 # calibration.cpp:47
+        self.lower = f32(0.1)                # calibration.cpp:48
+
+
+def gamma_start(state, gamma_from_file):
+    """calibration.cpp:196-205."""
+    g = f32(gamma_from_file)
+    if g < 0:
+        return f32(1)
+    if g > state.upper or g < state.lower:
+        return f32(1)
+    return g
+
+
+def find_new_gamma(st, gamma_old, measured, simulated, inflow=None, water_use=None):
+    """calibration.cpp:266-636; returns the next gamma (float32) or -99."""
+    gamma_old = f32(gamma_old)
+    n = len(measured)
+    inflow = np.zeros(n, np.float32) if inflow is None else np.asarray(inflow, np.float32)
+    water_use = np.zeros(n, np.float32) if water_use is None else np.asarray(water_use, np.float32)
+    meas = np.asarray(measured, np.float32)
+    sim = np.asarray(simulated, np.float32)
+    st.call_counter += 1
+    cc = st.call_counter
+    sum_diff = f32(0)
+    meas_sum = f32(0)
+    sim_in = f32(0)
+    sim_wu = f32(0)
+    sim_sum = f32(0)
+    ny = 0
+    for i in range(n):                                    # calibration.cpp:300-313
+        if meas[i] > -1:
+            ny += 1
+            sum_diff = f32(sum_diff + f32(sim[i] - meas[i]))
+            meas_sum = f32(meas_sum + meas[i])
+            sim_sum = f32(sim_sum + sim[i])
+            sim_in = f32(sim_in + inflow[i])
+            sim_wu = f32(sim_wu + water_use[i])
+    meas_avg = f32(meas_sum / f32(ny))                    # calibration.cpp:315
+    crit = abs(float(f32(sum_diff / f32(f32(ny) * meas_avg)))) < 0.01
+    up, lo = st.upper, st.lower
+    gamma = f32(-98)  # an unset local; see calibration.cpp:409, 437
+    if crit:
+        gamma = f32(-99)
+        st.gamma_cond = 1
+        st.calib_status = 1
+    elif cc == 1:                                          # calibration.cpp:318-366
+        if sum_diff > 0:
+            if gamma_old >= up:
+                gamma = f32(-99)
+            else:
+                st.gamma_low = gamma_old
+                gamma = up
+        else:
+            if gamma_old <= lo:
+                gamma = f32(-99)
+            else:
+                st.gamma_high = gamma_old
+                gamma = lo
+    elif cc == 2:                                          # calibration.cpp:367-448
+        if sum_diff > 0:
+            if gamma_old == up:
+                gamma = gamma_old
+                st.gamma_high = up
+            elif gamma_old < up:
+                st.gamma_low = gamma_old
+                if st.gamma_high < 0:
+                    gamma = f32(float(gamma_old) * 2.0)
+                else:
+                    gamma = f32(float(f32(st.gamma_low + st.gamma_high)) / 2.0)
+        else:
+            if gamma_old == lo:
+                gamma = gamma_old
+                st.gamma_low = lo
+            elif gamma_old > lo:
+                if st.gamma_low < 0:
+                    gamma = f32(float(gamma_old) / 2.0)
+                    st.gamma_high = gamma_old
+                else:
+                    st.gamma_high = gamma_old
+                    gamma = f32(float(f32(st.gamma_low + st.gamma_high)) / 2.0)
+    else:                                                  # calibration.cpp:449-523
+        if sum_diff > 0:
+            if gamma_old == up:
+                gamma = f32(-99)
+                st.gamma_cond = 2
+            elif gamma_old < up:
+                st.gamma_low = gamma_old
+                if st.gamma_high < 0:
+                    gamma = f32(float(gamma_old) * 2.0)
+                else:
+                    gamma = f32(float(f32(st.gamma_low + st.gamma_high)) / 2.0)
+        else:
+            if gamma_old == lo:
+                gamma = f32(-99)
+                st.gamma_cond = 2
+            elif gamma_old > lo:
+                if st.gamma_low < 0:
+                    gamma = f32(float(gamma_old) / 2.0)
+                    st.gamma_high = gamma_old
+                else:
+                    st.gamma_high = gamma_old
+                    gamma = f32(float(f32(st.gamma_low + st.gamma_high)) / 2.0)
+    adapt = meas_avg                                        # calibration.cpp:549-597
+    mean_sim = f32(sim_sum / f32(ny))
+    if gamma_old == up and st.gamma_cond == 2:
+        adapt = f32(float(meas_avg) * 1.1)
+        if mean_sim < adapt:
+            st.calib_status = 2
+        else:
+            st.cell_corr_factor_ind = 99
+    if gamma_old == lo and st.gamma_cond == 2:
+        adapt = f32(float(meas_avg) * 0.9)
+        if mean_sim > adapt:
+            st.calib_status = 2
+        else:
+            st.cell_corr_factor_ind = 99
+    if st.cell_corr_factor_ind == 99:
+        num = f32(adapt + f32(f32(sim_wu - sim_in) / f32(ny)))
+        den = f32(f32(f32(sim_sum + sim_wu) - sim_in) / f32(ny))
+        st.cell_corr_factor = f32(num / den)
+    if gamma > 0 and float(gamma) < 0.99 * float(lo) and cc > 2:      # calibration.cpp:613-617
+        gamma = f32(-99)
+    if gamma > 0 and abs(float(f32(f32(st.sum_diff_old / sum_diff) - f32(1)))) < 0.0001:
+        gamma = f32(-99)                                               # calibration.cpp:618-623
+    st.sum_diff_old = sum_diff
+    if gamma < 0 and abs(float(f32(st.cell_corr_factor - f32(1)))) > 0.01:
+        st.calib_status = 3
+    return gamma

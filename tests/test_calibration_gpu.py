@@ -1,0 +1,81 @@
+"""Batched calibration on the GPU against the same search driven by the CPU oracle."""
+import numpy as np
+import pytest
+
+from oracle import calib_oracle as co
+
+pytestmark = pytest.mark.gpu
+
+
+class OracleRunner:
+    """The CPU oracle over a basin set, run like watergap2_amd.calibration.BasinRunner."""
+
+    def __init__(self, oracle, basins, forcing_years=3, seed=11):
+        self.basins = basins
+        self.o = oracle.Oracle(basins)
+        self.o.set_stations(basins.basin_outlet)
+        self.f = basins.forcing(seed=seed, day0=0, n_days=365 * forcing_years)
+        self.forcing_years = forcing_years
+        self.initial = self.o.get_state()
+
+    def run(self, gamma, cfa, n_years):
+        b = self.basins
+        from watergap2_amd import PARAM_INDEX
+        cell_cfa = b.params[PARAM_INDEX["P_CFA"]] if cfa is None else b.per_basin(cfa)
+        self.o.set_params(gamma=b.per_basin(gamma), cfa=cell_cfa)
+        self.o.set_state(self.initial)
+        self.o.reset_station_discharge()
+        out = np.zeros((b.n_basins, n_years))
+        for y in range(n_years):
+            k = y % self.forcing_years
+            self.o.run(self.f[365 * k:365 * (k + 1)], 1)
+            out[:, y] = self.o.station_discharge()
+            self.o.reset_station_discharge()
+        return out
+
+
+def test_candidates_match_oracle(wg, oracle):
+    """K (gamma, CFA) candidates per basin as members of one run: every member's
+    annual station discharge equals the oracle run of that candidate."""
+    from watergap2_amd.calibration import BasinRunner, make_basins
+    print("case: 3 basins x 3 candidates, 3 years", flush=True)
+    bs = make_basins(3, seed=21, min_cells=30, max_cells=150)
+    K, years = 3, 3
+    rng = np.random.default_rng(4)
+    gam = rng.uniform(0.1, 5.0, (K, bs.n_basins))
+    cfa = rng.uniform(0.5, 1.5, (K, bs.n_basins))
+    r = BasinRunner(bs, n_members=K, forcing_years=3)
+    q = r.run(gam, cfa, n_years=years)
+    ref = OracleRunner(oracle, bs, forcing_years=3)
+    for k in range(K):
+        qo = ref.run(gam[k], cfa[k], years)
+        assert np.array_equal(q[k], qo), k
+
+
+def test_plain_search_matches_oracle_driven_search(wg, oracle):
+    """The batched search on the GPU takes the same steps as the reference's search
+    driven by oracle model runs, basin by basin, to the same gamma."""
+    from watergap2_amd.calibration import BasinRunner, make_basins, plain_search
+    print("case: plain gamma search, 4 basins, 3 years", flush=True)
+    bs = make_basins(4, seed=33, min_cells=30, max_cells=150)
+    years = 3
+    hidden = np.array([0.4, 1.7, 3.1, 4.6])
+    ref = OracleRunner(oracle, bs, forcing_years=3)
+    cfa1 = None
+    measured = ref.run(hidden, cfa1, years).astype(np.float32)
+    r = BasinRunner(bs, n_members=1, forcing_years=3)
+    res = plain_search(r, measured, n_years=years)
+    assert res["unfinished"] == 0
+    for b in range(bs.n_basins):
+        st = co.GammaSearchState()
+        g = co.gamma_start(st, -1.0)
+        traj = [float(g)]
+        while True:
+            sim = ref.run(np.full(bs.n_basins, float(g)), cfa1, years)[b].astype(np.float32)
+            g_new = co.find_new_gamma(st, g, measured[b], sim)
+            if g_new < 0:
+                break
+            g = g_new
+            traj.append(float(g))
+        assert traj == res["trajectories"][b], (b, traj, res["trajectories"][b])
+        assert np.float32(res["gamma"][b]) == g

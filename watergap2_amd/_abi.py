@@ -176,6 +176,16 @@ def load_library():
     return lib
 
 
+class GammaSearch(C.Structure):
+    """wg_gamma_search (include/watergap2_amd.h)."""
+    _fields_ = [("gamma_low", C.c_float), ("gamma_high", C.c_float), ("sum_diff_old", C.c_float),
+                ("call_counter", C.c_int32), ("gamma_cond", C.c_int32), ("calib_status", C.c_int32),
+                ("cell_corr_factor_ind", C.c_int32), ("cell_corr_factor", C.c_float),
+                ("gamma_upper_limit", C.c_float), ("gamma_lower_limit", C.c_float),
+                ("last_nash_sutcliffe", C.c_float), ("last_sum_diff", C.c_float),
+                ("measured_avg_adapt", C.c_float)]
+
+
 def _declare(lib):
     i32 = C.c_int32
     lib.wg_last_error.restype = C.c_char_p
@@ -204,6 +214,12 @@ def _declare(lib):
     lib.wg_stream.argtypes = [C.c_void_p]
     lib.wg_stream.restype = C.c_void_p
     lib.wg_last_timing.argtypes = [C.c_void_p, P_d, C.POINTER(C.c_int64)]
+    lib.wg_gamma_search_init.argtypes = [C.POINTER(GammaSearch)]
+    lib.wg_gamma_search_init.restype = None
+    lib.wg_gamma_start.argtypes = [C.POINTER(GammaSearch), C.c_float]
+    lib.wg_gamma_start.restype = C.c_float
+    lib.wg_find_new_gamma.argtypes = [C.POINTER(GammaSearch), C.c_float, i32, P_f, P_f, P_f, P_f]
+    lib.wg_find_new_gamma.restype = C.c_float
     lib.wg_world_create.argtypes = [C.POINTER(WorldSpec), C.POINTER(C.POINTER(WorldStruct))]
     lib.wg_world_destroy.argtypes = [C.POINTER(WorldStruct)]
     lib.wg_world_destroy.restype = None
@@ -221,4 +237,5 @@ ABI_SYMBOLS = [
     "wg_set_forcing_perturbation", "wg_load_forcing", "wg_set_stations", "wg_step_days",
     "wg_get_station_discharge", "wg_reset_station_discharge", "wg_get_day_outputs",
     "wg_synchronize", "wg_stream", "wg_step_days_profiled", "wg_last_timing", "wg_schedule_info", "wg_libm_eval",
+    "wg_gamma_search_init", "wg_gamma_start", "wg_find_new_gamma",
 ]
