@@ -295,6 +295,28 @@ float wg_gamma_start(const wg_gamma_search *s, float gamma_from_file);
 float wg_find_new_gamma(wg_gamma_search *s, float gamma_old, int32_t n_years, const float *measured,
                         const float *simulated, const float *sim_inflow, const float *sim_water_use);
 
+/* ---- calibration surrogate: 16-128-128-128-1 GELU perceptron on BF16 MFMA ----
+ * No counterpart in the reference: it proposes gamma from basin features so that
+ * the plain model only has to verify (the search it replaces is
+ * calibration.cpp:266-636).  Parameters are one flat FP32 vector: W1 [128][16],
+ * b1 [128], W2 [128][128], b2, W3 [128][128], b3, w4 [128], b4; GEMMs run on
+ * BF16 copies with FP32 accumulation. */
+typedef struct wg_mlp wg_mlp;
+int wg_mlp_create(uint64_t seed, wg_mlp **out);
+int wg_mlp_destroy(wg_mlp *h);
+const char *wg_mlp_last_error(const wg_mlp *h);
+int64_t wg_mlp_n_params(void);
+int wg_mlp_get_params(wg_mlp *h, float *out);
+int wg_mlp_set_params(wg_mlp *h, const float *in);
+/* y[r] for rows x[r][16] (host arrays). */
+int wg_mlp_forward(wg_mlp *h, int32_t rows, const float *x, float *y);
+/* Mean-squared-error loss and its gradient (flat, like the parameters) over rows. */
+int wg_mlp_gradient(wg_mlp *h, int32_t rows, const float *x, const float *t, float *loss, float *grad);
+/* Adam training over shuffled mini-batches (batch a multiple of 32); mean batch
+ * loss per epoch into epoch_loss[epochs] (may be NULL). */
+int wg_mlp_train(wg_mlp *h, int32_t n_samples, const float *x, const float *t, int32_t epochs, int32_t batch,
+                 float lr, uint64_t seed, float *epoch_loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -220,6 +220,17 @@ def _declare(lib):
     lib.wg_gamma_start.restype = C.c_float
     lib.wg_find_new_gamma.argtypes = [C.POINTER(GammaSearch), C.c_float, i32, P_f, P_f, P_f, P_f]
     lib.wg_find_new_gamma.restype = C.c_float
+    lib.wg_mlp_create.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
+    lib.wg_mlp_destroy.argtypes = [C.c_void_p]
+    lib.wg_mlp_last_error.argtypes = [C.c_void_p]
+    lib.wg_mlp_last_error.restype = C.c_char_p
+    lib.wg_mlp_n_params.argtypes = []
+    lib.wg_mlp_n_params.restype = C.c_int64
+    lib.wg_mlp_get_params.argtypes = [C.c_void_p, P_f]
+    lib.wg_mlp_set_params.argtypes = [C.c_void_p, P_f]
+    lib.wg_mlp_forward.argtypes = [C.c_void_p, i32, P_f, P_f]
+    lib.wg_mlp_gradient.argtypes = [C.c_void_p, i32, P_f, P_f, P_f, P_f]
+    lib.wg_mlp_train.argtypes = [C.c_void_p, i32, P_f, P_f, i32, i32, C.c_float, C.c_uint64, P_f]
     lib.wg_world_create.argtypes = [C.POINTER(WorldSpec), C.POINTER(C.POINTER(WorldStruct))]
     lib.wg_world_destroy.argtypes = [C.POINTER(WorldStruct)]
     lib.wg_world_destroy.restype = None
@@ -238,4 +249,6 @@ ABI_SYMBOLS = [
     "wg_get_station_discharge", "wg_reset_station_discharge", "wg_get_day_outputs",
     "wg_synchronize", "wg_stream", "wg_step_days_profiled", "wg_last_timing", "wg_schedule_info", "wg_libm_eval",
     "wg_gamma_search_init", "wg_gamma_start", "wg_find_new_gamma",
+    "wg_mlp_create", "wg_mlp_destroy", "wg_mlp_last_error", "wg_mlp_n_params", "wg_mlp_get_params",
+    "wg_mlp_set_params", "wg_mlp_forward", "wg_mlp_gradient", "wg_mlp_train",
 ]

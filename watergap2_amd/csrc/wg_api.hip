@@ -29,7 +29,6 @@
 
 namespace {
 
-constexpr int kNsub = WG_DEV_NSUB;
 
 struct DevBuf {
   void *p = nullptr;
