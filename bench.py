@@ -171,6 +171,110 @@ def calibration_run(args):
             "candidates": cand}
 
 
+def surrogate_run(args):
+    """Metric 2 with the learned part: basins calibrated per hour with surrogate
+    proposals (verified by the plain model, plain search as the fallback) against
+    the plain search alone, on unseen basins, at the same 1 % criterion."""
+    import numpy as np
+    from watergap2_amd.calibration import BasinRunner, BasinSet, make_basins, plain_search, within_1pc
+    from watergap2_amd import surrogate as sg
+
+    def log(msg):
+        print("# " + msg, file=sys.stderr, flush=True)
+
+    out = {}
+    t0 = time.perf_counter()
+    if args.sur_data and all(os.path.exists(p) for p in args.sur_data.split(",")):
+        parts = [np.load(p) for p in args.sur_data.split(",")]
+        x = np.concatenate([d["x"] for d in parts])
+        t = np.concatenate([d["t"] for d in parts])
+        log(f"training set loaded from {args.sur_data}: {len(t)} samples")
+        gen_s = None
+    else:
+        x, t = sg.generate_training_set(args.sur_train_basins, args.sur_candidates, seed=args.sur_seed, log=log)
+        gen_s = time.perf_counter() - t0
+        if args.sur_data:
+            os.makedirs(os.path.dirname(os.path.abspath(args.sur_data)), exist_ok=True)
+            np.savez(args.sur_data, x=x, t=t)
+    out["training_set"] = {"samples": int(len(t)), "basins": args.sur_train_basins,
+                           "candidates_per_basin": args.sur_candidates, "generation_seconds":
+                           None if gen_s is None else round(gen_s, 1)}
+    if args.sur_generate_only:
+        return out
+    # hold out 5 % of the samples for validation
+    rng = np.random.default_rng(5)
+    perm = rng.permutation(len(t))
+    n_val = len(t) // 20
+    val, tr = perm[:n_val], perm[n_val:]
+    t1 = time.perf_counter()
+    model, losses = sg.Surrogate.fit(x[tr], t[tr], epochs=(args.sur_epochs, args.sur_epochs // 2), log=log)
+    train_s = time.perf_counter() - t1
+    pv = model.predict(x[val])
+    val_rmse = float(np.sqrt(np.mean((pv - t[val]) ** 2)))
+    out["training"] = {"seconds": round(train_s, 2), "final_loss": float(losses[-1]),
+                       "validation_rmse_log_runoff": round(val_rmse, 5)}
+    log(f"trained in {train_s:.1f}s, validation RMSE of log runoff {val_rmse:.4f}")
+
+    # unseen basins, targets from a hidden gamma
+    bs = make_basins(args.sur_eval_basins, seed=424242)
+    forcing = bs.forcing(seed=11, day0=0, n_days=365 * 3)
+    feats = sg.basin_features(bs, forcing)
+    hidden = np.random.default_rng(99).uniform(0.1, 5.0, bs.n_basins)
+    years = args.cal_years
+    runner = BasinRunner(bs, n_members=1, forcing_years=3, forcing=forcing)
+    measured = runner.run(hidden[None, :], n_years=years)[0].astype(np.float32)
+    log(f"evaluation set: {bs.n_basins} basins, {bs.n} cells")
+
+    # plain search
+    t1 = time.perf_counter()
+    res = plain_search(runner, measured, n_years=years)
+    plain_s = time.perf_counter() - t1
+    plain_ok = int((res["status"] == 1).sum())
+    log(f"plain search: {res['model_runs']} runs in {plain_s:.1f}s, {plain_ok} within 1 %")
+
+    # surrogate proposal + verification + plain fallback for the misses
+    t1 = time.perf_counter()
+    target = sg.runoff_target(measured.astype(np.float64), bs)
+    g_star = model.propose_gamma(feats, target)
+    prop_s = time.perf_counter() - t1
+    q = runner.run(g_star[None, :], n_years=years)[0]
+    hit = np.array([within_1pc(q[b], measured[b]) for b in range(bs.n_basins)])
+    verify_s = time.perf_counter() - t1
+    miss = np.flatnonzero(~hit)
+    fb_ok = 0
+    fb_runs = 0
+    if len(miss):
+        sub = BasinSet([bs.worlds[i] for i in miss])
+        cols = np.concatenate([np.arange(bs.offsets[i], bs.offsets[i + 1]) for i in miss])
+        sr = BasinRunner(sub, n_members=1, forcing_years=3, forcing=np.ascontiguousarray(forcing[:, :, cols]))
+        fres = plain_search(sr, measured[miss], n_years=years)
+        sr.close()
+        fb_ok = int((fres["status"] == 1).sum())
+        fb_runs = fres["model_runs"]
+    sur_s = time.perf_counter() - t1
+    sur_ok = int(hit.sum()) + fb_ok
+    runner.close()
+    log(f"surrogate: {int(hit.sum())}/{bs.n_basins} first proposals within 1 %, fallback {len(miss)} basins "
+        f"({fb_runs} runs), {sur_s:.1f}s")
+    plain_rate = plain_ok / plain_s * 3600.0
+    sur_rate = sur_ok / sur_s * 3600.0
+    out.update({
+        "metric": "basins calibrated/hr at the 1% criterion, surrogate proposals vs plain search",
+        "value": round(sur_rate, 1), "unit": "basins/hr", "higher_is_better": True,
+        "plain_basins_per_hr": round(plain_rate, 1), "gain": round(sur_rate / max(plain_rate, 1e-9), 2),
+        "config": {"workload": f"{bs.n_basins} unseen synthetic basins (log-uniform 50-5,000 cells), {bs.n} "
+                               f"cells, {years} years, targets from a hidden gamma U(0.1, 5)",
+                   "basins": bs.n_basins, "cells": bs.n, "years": years},
+        "plain": {"seconds": round(plain_s, 2), "model_runs": res["model_runs"], "within_1pc": plain_ok,
+                  "runs_per_basin_mean": round(float(res["runs"].mean()), 2)},
+        "surrogate": {"seconds": round(sur_s, 2), "proposal_seconds": round(prop_s, 3),
+                      "verify_seconds": round(verify_s, 2), "first_proposal_hits": int(hit.sum()),
+                      "fallback_basins": int(len(miss)), "fallback_runs": fb_runs, "within_1pc": sur_ok,
+                      "median_abs_gamma_error": round(float(np.median(np.abs(g_star - hidden))), 4)},
+    })
+    return out
+
+
 def traffic_from_profiles(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprof counter summary
     (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration), or None."""
@@ -206,6 +310,15 @@ def main():
     ap.add_argument("--cal-basins", type=int, default=120)
     ap.add_argument("--cal-years", type=int, default=30)
     ap.add_argument("--cal-candidates", type=int, default=64)
+    ap.add_argument("--surrogate", action="store_true",
+                    help="measure basins/hr with surrogate proposals against the plain search")
+    ap.add_argument("--sur-data", default="", help="training-set .npz to load, or to write when absent")
+    ap.add_argument("--sur-generate-only", action="store_true")
+    ap.add_argument("--sur-train-basins", type=int, default=5000)
+    ap.add_argument("--sur-seed", type=int, default=9001, help="seed of the training basins")
+    ap.add_argument("--sur-candidates", type=int, default=40)
+    ap.add_argument("--sur-epochs", type=int, default=30)
+    ap.add_argument("--sur-eval-basins", type=int, default=500)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this size (counter calibration) and exit")
     args = ap.parse_args()
@@ -222,6 +335,9 @@ def main():
 
     if args.calibration:
         print(json.dumps(calibration_run(args)))
+        return
+    if args.surrogate:
+        print(json.dumps(surrogate_run(args)))
         return
 
     if args.gpus != 1:

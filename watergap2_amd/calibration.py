@@ -108,13 +108,16 @@ class BasinRunner:
     """One model handle over a basin set: resettable runs of n_years years with
     per-member, per-basin gamma and CFA; returns annual station discharge."""
 
-    def __init__(self, basins, n_members=1, forcing_years=3, seed=11):
+    def __init__(self, basins, n_members=1, forcing_years=3, seed=11, forcing=None):
         self.basins = basins
         self.n_members = n_members
         self.forcing_years = forcing_years
         self.model = WaterBalanceModel(basins, n_members=n_members)
         self.model.set_stations(basins.basin_outlet)
-        self.model.load_forcing(basins.forcing(seed=seed, day0=0, n_days=365 * forcing_years))
+        if forcing is None:
+            forcing = basins.forcing(seed=seed, day0=0, n_days=365 * forcing_years)
+        assert forcing.shape[0] == 365 * forcing_years
+        self.model.load_forcing(forcing)
         self.initial = self.model.get_state(0)
         self.base_gamma = basins.params[PARAM_INDEX["P_GAMRUN_C"]].copy()
         self.base_cfa = basins.params[PARAM_INDEX["P_CFA"]].copy()

@@ -78,3 +78,136 @@ class MLP:
         self._c(self._lib.wg_mlp_train(self._h, len(x), _abi.ptr(x, _abi.P_f), _abi.ptr(t, _abi.P_f), epochs, batch,
                                        lr, int(seed), _abi.ptr(losses, _abi.P_f)))
         return losses
+
+
+# ---------------------------------------------------------------------------
+# basin features, training data, proposals
+# ---------------------------------------------------------------------------
+
+FEATURES = ["log10_area", "precip", "precip_cv", "temp", "shortwave", "arid_frac", "smax", "lake_pct",
+            "wetland_pct", "snow_frac", "log10_slope", "elevation", "elev_spread", "gw_factor", "gamma", "cfa"]
+
+
+def basin_features(basins, forcing):
+    """[n_basins, 14] climate and land features of each basin (without gamma and
+    CFA) from the statics and the forcing cycle [days, 4, N]."""
+    nb = basins.n_basins
+    out = np.zeros((nb, 14))
+    temp = forcing[:, 0, :]
+    prec = forcing[:, 1, :]
+    sw = forcing[:, 2, :]
+    years = forcing.shape[0] / 365.0
+    for b in range(nb):
+        a, e = int(basins.offsets[b]), int(basins.offsets[b + 1])
+        area = basins.cell_area[a:e] * basins.contfreq[a:e] / 100.0
+        w = area / area.sum()
+        p_b = prec[:, a:e].astype(np.float64)
+        daily = p_b @ w
+        cold = temp[:, a:e] < 0.0
+        elev = basins.elevation[a:e].astype(np.float64)
+        out[b] = [
+            np.log10(area.sum()),
+            float((p_b.sum(0) / years) @ w) / 1000.0,
+            float(daily.std() / max(daily.mean(), 1e-9)),
+            float(temp[:, a:e].mean(0) @ w) / 10.0,
+            float(sw[:, a:e].mean(0) @ w) / 100.0,
+            float((basins.aindex[a:e] == 1) @ w),
+            float(basins.smax[a:e] @ w) / 100.0,
+            float((basins.loc_lake[a:e] + basins.glo_lake[a:e]) @ w),
+            float((basins.loc_wetland[a:e] + basins.glo_wetland[a:e]) @ w),
+            float((p_b * cold).sum() / max(p_b.sum(), 1e-9)),
+            float(np.log10(np.maximum(basins.river_slope[a:e], 1e-6)) @ w),
+            float(elev[:, 0] @ w) / 1000.0,
+            float(elev[:, 1:].std(axis=1) @ w) / 100.0,
+            float(basins.gw_factor[a:e] @ w),
+        ]
+    return out
+
+
+def runoff_target(annual, basins):
+    """log of the long-term mean runoff (mm/yr over the basin area) from annual
+    discharge [..., n_basins, years] in km3/yr."""
+    area = np.array([(basins.cell_area[a:e] * basins.contfreq[a:e] / 100.0).sum()
+                     for a, e in zip(basins.offsets[:-1], basins.offsets[1:])])
+    q = np.asarray(annual).mean(axis=-1)
+    return np.log(np.maximum(q, 1e-12) * 1e6 / area)
+
+
+def thirty_year_mean_from_six(annual6):
+    """Mean of 30 years from the first 6 of a run whose forcing repeats every 3
+    years: years 1-3 as run, years 4-30 as nine cycles of years 4-6."""
+    a = np.asarray(annual6)
+    return (a[..., :3].sum(-1) + 9.0 * a[..., 3:6].sum(-1)) / 30.0
+
+
+def generate_training_set(n_basins, candidates, seed, chunk=250, log=None):
+    """Samples (features + gamma + CFA -> log long-term runoff) from batched GPU
+    runs: chunks of basins, `candidates` (gamma, CFA) members per basin, 6 years
+    each (the 30-year mean from the 3-year forcing cycle)."""
+    from .calibration import BasinRunner, make_basins
+    xs, ts = [], []
+    rng = np.random.default_rng(seed)
+    done = 0
+    while done < n_basins:
+        nb = min(chunk, n_basins - done)
+        bs = make_basins(nb, seed=seed * 1000 + done)
+        forcing = bs.forcing(seed=seed + done, day0=0, n_days=365 * 3)
+        feats = basin_features(bs, forcing)
+        g = rng.uniform(0.1, 5.0, (candidates, nb))
+        c = rng.uniform(0.5, 1.5, (candidates, nb))
+        r = BasinRunner(bs, n_members=candidates, forcing_years=3, forcing=forcing)
+        annual = r.run(g, c, n_years=6)
+        r.close()
+        q30 = thirty_year_mean_from_six(annual)            # [M][B]
+        t = runoff_target(q30[..., None], bs)               # [M][B]
+        for m in range(candidates):
+            xs.append(np.concatenate([feats, g[m][:, None], c[m][:, None]], axis=1))
+            ts.append(t[m])
+        done += nb
+        if log:
+            log(f"training set: {done}/{n_basins} basins, {done * candidates} samples")
+    return np.concatenate(xs).astype(np.float32), np.concatenate(ts).astype(np.float32)
+
+
+class Surrogate:
+    """Trained network plus the feature standardisation it was trained with."""
+
+    def __init__(self, mlp, mean, std):
+        self.mlp, self.mean, self.std = mlp, np.asarray(mean), np.asarray(std)
+
+    @classmethod
+    def fit(cls, x, t, epochs=(30, 15), lrs=(2e-3, 5e-4), batch=1024, seed=0, log=None):
+        mean = x.mean(0)
+        std = x.std(0) + 1e-6
+        mlp = MLP(seed=seed)
+        xn = ((x - mean) / std).astype(np.float32)
+        losses = []
+        for ep, lr in zip(epochs, lrs):
+            losses.extend(mlp.train(xn, t, epochs=ep, batch=batch, lr=lr, seed=seed + len(losses)).tolist())
+            if log:
+                log(f"surrogate training: epoch {len(losses)}, loss {losses[-1]:.5f}")
+        return cls(mlp, mean, std), np.array(losses)
+
+    def predict(self, x):
+        return self.mlp.forward(((np.asarray(x) - self.mean) / self.std).astype(np.float32))
+
+    def propose_gamma(self, feats, target_log_runoff, cfa=None, grid=None):
+        """gamma per basin whose predicted runoff matches the target: the network
+        on a gamma grid, the crossing interpolated linearly."""
+        grid = np.linspace(0.1, 5.0, 491) if grid is None else np.asarray(grid)
+        nb = len(feats)
+        cfa = np.ones(nb) if cfa is None else np.asarray(cfa)
+        rows = np.concatenate([np.repeat(feats, len(grid), axis=0), np.tile(grid, nb)[:, None],
+                               np.repeat(cfa, len(grid))[:, None]], axis=1)
+        pred = self.predict(rows).reshape(nb, len(grid)).astype(np.float64)
+        out = np.zeros(nb)
+        for b in range(nb):
+            d = pred[b] - target_log_runoff[b]
+            best = grid[int(np.argmin(np.abs(d)))]
+            # the crossing nearest to the best grid point, interpolated linearly
+            cross = np.flatnonzero((d[:-1] * d[1:] <= 0) & (d[:-1] != d[1:]))
+            if len(cross):
+                k = cross[np.argmin(np.abs(grid[cross] - best))]
+                best = grid[k] + (grid[k + 1] - grid[k]) * d[k] / (d[k] - d[k + 1])
+            out[b] = best
+        return out
