@@ -196,7 +196,13 @@ def surrogate_run(args):
         if args.sur_data:
             os.makedirs(os.path.dirname(os.path.abspath(args.sur_data)), exist_ok=True)
             np.savez(args.sur_data, x=x, t=t)
-    out["training_set"] = {"samples": int(len(t)), "basins": args.sur_train_basins,
+    # The reference's land-area update can drive a cell's land fraction to ~0 and
+    # back; with CFA > 1 the rescaled soil storage then turns slightly negative and
+    # pow() of it gives NaN (the oracle does the same): such runs are dropped.
+    finite = np.isfinite(t) & np.isfinite(x).all(axis=1)
+    n_dropped = int((~finite).sum())
+    x, t = x[finite], t[finite]
+    out["training_set"] = {"samples": int(len(t)), "dropped_non_finite": n_dropped, "basins": args.sur_train_basins,
                            "candidates_per_basin": args.sur_candidates, "generation_seconds":
                            None if gen_s is None else round(gen_s, 1)}
     if args.sur_generate_only:
