@@ -124,26 +124,34 @@ def basin_features(basins, forcing):
     return out
 
 
-def runoff_target(annual, basins):
-    """log of the long-term mean runoff (mm/yr over the basin area) from annual
-    discharge [..., n_basins, years] in km3/yr."""
-    area = np.array([(basins.cell_area[a:e] * basins.contfreq[a:e] / 100.0).sum()
+def basin_areas(basins):
+    """Land area (km2) of each basin."""
+    return np.array([(basins.cell_area[a:e] * basins.contfreq[a:e] / 100.0).sum()
                      for a, e in zip(basins.offsets[:-1], basins.offsets[1:])])
+
+
+def runoff_target(annual, basins, idx=None):
+    """log of the long-term mean runoff (mm/yr over the basin area) from annual
+    discharge [..., n_basins, years] in km3/yr (of the basins `idx` if given)."""
+    area = basin_areas(basins)
+    if idx is not None:
+        area = area[np.asarray(idx)]
     q = np.asarray(annual).mean(axis=-1)
     return np.log(np.maximum(q, 1e-12) * 1e6 / area)
 
 
-def thirty_year_mean_from_six(annual6):
-    """Mean of 30 years from the first 6 of a run whose forcing repeats every 3
-    years: years 1-3 as run, years 4-30 as nine cycles of years 4-6."""
-    a = np.asarray(annual6)
-    return (a[..., :3].sum(-1) + 9.0 * a[..., 3:6].sum(-1)) / 30.0
+def thirty_year_mean(annual9):
+    """Mean of 30 years from the first 9 of a run whose forcing repeats every 3
+    years: years 1-9 as run, years 10-30 as seven more cycles of years 7-9 (the
+    storages have settled by then: the cycle sums change by < 0.2 % after it)."""
+    a = np.asarray(annual9)
+    return (a[..., :9].sum(-1) + 7.0 * a[..., 6:9].sum(-1)) / 30.0
 
 
 def generate_training_set(n_basins, candidates, seed, chunk=250, log=None):
     """Samples (features + gamma + CFA -> log long-term runoff) from batched GPU
-    runs: chunks of basins, `candidates` (gamma, CFA) members per basin, 6 years
-    each (the 30-year mean from the 3-year forcing cycle)."""
+    runs: chunks of basins, `candidates` (gamma, CFA) members per basin, 9 years
+    each (the 30-year mean from the 3-year forcing cycle, see thirty_year_mean)."""
     from .calibration import BasinRunner, make_basins
     xs, ts = [], []
     rng = np.random.default_rng(seed)
@@ -156,9 +164,9 @@ def generate_training_set(n_basins, candidates, seed, chunk=250, log=None):
         g = rng.uniform(0.1, 5.0, (candidates, nb))
         c = rng.uniform(0.5, 1.5, (candidates, nb))
         r = BasinRunner(bs, n_members=candidates, forcing_years=3, forcing=forcing)
-        annual = r.run(g, c, n_years=6)
+        annual = r.run(g, c, n_years=9)
         r.close()
-        q30 = thirty_year_mean_from_six(annual)            # [M][B]
+        q30 = thirty_year_mean(annual)                      # [M][B]
         t = runoff_target(q30[..., None], bs)               # [M][B]
         for m in range(candidates):
             xs.append(np.concatenate([feats, g[m][:, None], c[m][:, None]], axis=1))
@@ -188,18 +196,28 @@ class Surrogate:
                 log(f"surrogate training: epoch {len(losses)}, loss {losses[-1]:.5f}")
         return cls(mlp, mean, std), np.array(losses)
 
+    def predict_at(self, feats, gamma, cfa=None):
+        """Predicted log runoff of each basin at its own gamma."""
+        nb = len(feats)
+        cfa = np.ones(nb) if cfa is None else np.asarray(cfa)
+        rows = np.concatenate([feats, np.asarray(gamma)[:, None], cfa[:, None]], axis=1)
+        return self.predict(rows).astype(np.float64)
+
     def predict(self, x):
         return self.mlp.forward(((np.asarray(x) - self.mean) / self.std).astype(np.float32))
 
-    def propose_gamma(self, feats, target_log_runoff, cfa=None, grid=None):
-        """gamma per basin whose predicted runoff matches the target: the network
-        on a gamma grid, the crossing interpolated linearly."""
+    def propose_gamma(self, feats, target_log_runoff, cfa=None, grid=None, offset=None):
+        """gamma per basin whose predicted runoff (plus `offset`, the verified
+        miss of an earlier proposal) matches the target: the network on a gamma
+        grid, the crossing interpolated linearly."""
         grid = np.linspace(0.1, 5.0, 491) if grid is None else np.asarray(grid)
         nb = len(feats)
         cfa = np.ones(nb) if cfa is None else np.asarray(cfa)
         rows = np.concatenate([np.repeat(feats, len(grid), axis=0), np.tile(grid, nb)[:, None],
                                np.repeat(cfa, len(grid))[:, None]], axis=1)
         pred = self.predict(rows).reshape(nb, len(grid)).astype(np.float64)
+        if offset is not None:
+            pred += np.asarray(offset, dtype=np.float64)[:, None]
         out = np.zeros(nb)
         for b in range(nb):
             d = pred[b] - target_log_runoff[b]
