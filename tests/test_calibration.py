@@ -162,3 +162,25 @@ def test_basin_set_equals_separate_basins(wg, oracle):
         for key in ("soil", "river", "groundwater", "glo_wetl", "snow"):
             assert np.array_equal(whole[key][a:e], sb[key]), (b, key)
         assert q_whole[b] == ob.station_discharge()[0]
+
+
+def test_thirty_year_mean_from_nine_years():
+    from watergap2_amd import surrogate as sg
+    # years 1-9 as run, then seven more copies of the last cycle (years 7-9)
+    a = np.arange(1.0, 10.0)
+    expect = (a.sum() + 7 * (7 + 8 + 9)) / 30.0
+    assert sg.thirty_year_mean(a) == expect
+    # a settled periodic series keeps its mean
+    p = np.tile([2.0, 3.0, 7.0], 3)
+    assert abs(sg.thirty_year_mean(p) - 4.0) < 1e-15
+
+
+def test_runoff_target_units(wg):
+    from watergap2_amd import surrogate as sg
+    from watergap2_amd.calibration import make_basins
+    bs = make_basins(2, seed=3, min_cells=20, max_cells=40)
+    area = sg.basin_areas(bs)
+    # 1 km3/yr over A km2 is 1e6/A mm/yr
+    q = np.ones((2, 5))
+    assert np.allclose(np.exp(sg.runoff_target(q, bs)), 1e6 / area)
+    assert np.allclose(sg.runoff_target(q[1:], bs, idx=[1]), np.log(1e6 / area[1]))

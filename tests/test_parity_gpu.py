@@ -194,6 +194,41 @@ def test_global_grid_20_days(wg, oracle, global_world):
     assert info["groups"] >= 1 and info["max_depth"] == global_world.max_level + 1
 
 
+def test_launch_configs_and_autotune_are_bit_identical(wg, regional_world):
+    """Every launch configuration the autotuner chooses from gives the same bits,
+    and the autotuner leaves the model state as it found it."""
+    print("case: launch configurations and autotune, regional 30 days", flush=True)
+    w = regional_world
+    f = w.forcing(seed=8, day0=0, n_days=30)
+    ref = None
+    for cell, chain, chunk in ((256, 256, 5), (128, 64, 10), (192, 128, 5), (64, 256, 10)):
+        m = wg.WaterBalanceModel(w)
+        m.load_forcing(f)
+        m.set_launch_config(cell, chain, chunk)
+        m.step_days(0, 1, 30)
+        st = m.get_state()
+        if ref is None:
+            ref = st
+        else:
+            for k in st:
+                if k != "started":
+                    assert np.array_equal(st[k], ref[k]), (cell, chain, chunk, k)
+    m = wg.WaterBalanceModel(w)
+    m.load_forcing(f)
+    m.step_days(0, 1, 10)
+    before = m.get_state()
+    cfg, ms = m.autotune(n_days=5)
+    assert cfg[0] in (128, 256) and cfg[1] in (64, 128, 256) and cfg[2] in (5, 10) and ms > 0
+    after = m.get_state()
+    for k in before:
+        assert np.array_equal(np.asarray(before[k]), np.asarray(after[k])), k
+    m.step_days(10, 11, 20)
+    st = m.get_state()
+    for k in st:
+        if k != "started":
+            assert np.array_equal(st[k], ref[k]), ("after autotune", k)
+
+
 def test_vertical_closure_on_gpu(wg):
     """Per-cell water balance of the GPU day loop, 120 days, CFA = 1 and no built-up
     area: storage change after the land-area rescaling equals precipitation minus

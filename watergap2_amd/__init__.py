@@ -248,6 +248,18 @@ class WaterBalanceModel:
         fn = self._lib.wg_step_days_profiled if profiled else self._lib.wg_step_days
         self._c(fn(self._h, forcing_day0, first_doy, n_days))
 
+    def autotune(self, n_days=5):
+        """Picks the fastest launch configuration on n_days of the loaded forcing
+        (state restored afterwards); returns (cell threads, chain threads, snow
+        chunk) and the best time per day in ms."""
+        cfg = (C.c_int32 * 3)()
+        ms = C.c_double()
+        self._c(self._lib.wg_autotune(self._h, n_days, cfg, C.byref(ms)))
+        return tuple(cfg), ms.value
+
+    def set_launch_config(self, cell_threads=256, chain_threads=256, snow_chunk=5):
+        self._c(self._lib.wg_set_launch_config(self._h, cell_threads, chain_threads, snow_chunk))
+
     def timing(self):
         ms = (C.c_double * 3)()
         cnt = (C.c_int64 * 3)()

@@ -214,6 +214,8 @@ def _declare(lib):
     lib.wg_stream.argtypes = [C.c_void_p]
     lib.wg_stream.restype = C.c_void_p
     lib.wg_last_timing.argtypes = [C.c_void_p, P_d, C.POINTER(C.c_int64)]
+    lib.wg_autotune.argtypes = [C.c_void_p, i32, P_i32, P_d]
+    lib.wg_set_launch_config.argtypes = [C.c_void_p, i32, i32, i32]
     lib.wg_gamma_search_init.argtypes = [C.POINTER(GammaSearch)]
     lib.wg_gamma_search_init.restype = None
     lib.wg_gamma_start.argtypes = [C.POINTER(GammaSearch), C.c_float]
@@ -248,6 +250,7 @@ ABI_SYMBOLS = [
     "wg_set_forcing_perturbation", "wg_load_forcing", "wg_set_stations", "wg_step_days",
     "wg_get_station_discharge", "wg_reset_station_discharge", "wg_get_day_outputs",
     "wg_synchronize", "wg_stream", "wg_step_days_profiled", "wg_last_timing", "wg_schedule_info", "wg_libm_eval",
+    "wg_autotune", "wg_set_launch_config",
     "wg_gamma_search_init", "wg_gamma_start", "wg_find_new_gamma",
     "wg_mlp_create", "wg_mlp_destroy", "wg_mlp_last_error", "wg_mlp_n_params", "wg_mlp_get_params",
     "wg_mlp_set_params", "wg_mlp_forward", "wg_mlp_gradient", "wg_mlp_train",

@@ -67,9 +67,12 @@ struct wg_handle {
   // graphs keyed by block length
   std::map<int, hipGraphExec_t> graphs;
   int block_threads_cell = 256, block_threads_chain = 256;
+  int snow_chunk = 5;             // sub-cell pairs per double-buffered chunk of the cell kernel
   int32_t max_group_cells = 0;
   int n_xcd = 8;
+  std::vector<int32_t> bin_start, bin_end;   // device cell range of each XCD bin
   std::vector<int32_t> tile_start, tile_end, block_group;
+  int32_t tile_capacity = 0;      // entries of the device tile tables
   std::vector<WgChainLink> clink;
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -211,6 +214,23 @@ extern "C" const char *wg_last_error(const wg_handle *h) { return h ? h->err.c_s
 
 namespace {
 
+// Cell-kernel tiles of block_threads_cell cells: block 8j + x covers tile j of XCD bin x.
+void build_tiles(wg_handle *h) {
+  const int nx = h->n_xcd, T = h->block_threads_cell;
+  int32_t max_tiles = 1;
+  for (int x = 0; x < nx; x++) max_tiles = std::max(max_tiles, (h->bin_end[x] - h->bin_start[x] + T - 1) / T);
+  h->tile_start.assign(size_t(nx) * max_tiles, -1);
+  h->tile_end.assign(size_t(nx) * max_tiles, -1);
+  for (int32_t j = 0; j < max_tiles; j++)
+    for (int x = 0; x < nx; x++) {
+      const int32_t a = h->bin_start[x] + T * j;
+      if (a < h->bin_end[x]) {
+        h->tile_start[size_t(j) * nx + x] = a;
+        h->tile_end[size_t(j) * nx + x] = std::min(a + T, h->bin_end[x]);
+      }
+    }
+}
+
 // Builds the level schedule and the device order.
 int build_schedule(wg_handle *h, const wg_static_fields *st, int group_cells) {
   const int32_t n = h->n;
@@ -341,28 +361,26 @@ int build_schedule(wg_handle *h, const wg_static_fields *st, int group_cells) {
       bin_end[x] = h->level_off[h->group_first_level[gg]];
     }
   }
-  const int T = h->block_threads_cell;
-  int32_t max_tiles = 1, max_groups = 1;
-  for (int x = 0; x < nx; x++) {
-    max_tiles = std::max(max_tiles, (bin_end[x] - bin_start[x] + T - 1) / T);
-    max_groups = std::max(max_groups, int32_t(bin_groups[x].size()));
-  }
-  h->tile_start.assign(size_t(nx) * max_tiles, -1);
-  h->tile_end.assign(size_t(nx) * max_tiles, -1);
-  for (int32_t j = 0; j < max_tiles; j++)
-    for (int x = 0; x < nx; x++) {
-      const int32_t a = bin_start[x] + T * j;
-      if (a < bin_end[x]) {
-        h->tile_start[size_t(j) * nx + x] = a;
-        h->tile_end[size_t(j) * nx + x] = std::min(a + T, bin_end[x]);
-      }
-    }
+  h->bin_start = bin_start;
+  h->bin_end = bin_end;
+  int32_t max_groups = 1;
+  for (int x = 0; x < nx; x++) max_groups = std::max(max_groups, int32_t(bin_groups[x].size()));
+  build_tiles(h);
   h->block_group.assign(size_t(nx) * max_groups, -1);
   {
     int32_t gg = 0;
     for (int x = 0; x < nx; x++)
       for (size_t j = 0; j < bin_groups[x].size(); j++) h->block_group[j * nx + x] = gg++;
   }
+  return WG_OK;
+}
+
+int upload_tiles(wg_handle *h) {
+  if (int32_t(h->tile_start.size()) > h->tile_capacity) return fail(h, WG_E_ARG, "tile table too large");
+  HIP_CHECK(h, hipMemcpy(const_cast<int32_t *>(h->S.tile_start), h->tile_start.data(),
+                         h->tile_start.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_CHECK(h, hipMemcpy(const_cast<int32_t *>(h->S.tile_end), h->tile_end.data(),
+                         h->tile_end.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   return WG_OK;
 }
 
@@ -675,13 +693,21 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     r_ = upload(H, &dcs, cst);
     if (r_) { *out = h.release(); return r_; }
     S.cstat = dcs;
+    // tile tables sized for the smallest cell block the tuner tries (64 threads)
     int32_t *dt;
-    r_ = upload(H, &dt, H->tile_start);
+    {
+      int32_t cap = 1;
+      for (int x = 0; x < H->n_xcd; x++) cap = std::max(cap, (H->bin_end[x] - H->bin_start[x] + 63) / 64);
+      H->tile_capacity = cap * H->n_xcd;
+    }
+    r_ = dev_alloc(H, &dt, size_t(H->tile_capacity));
     if (r_) { *out = h.release(); return r_; }
     S.tile_start = dt;
-    r_ = upload(H, &dt, H->tile_end);
+    r_ = dev_alloc(H, &dt, size_t(H->tile_capacity));
     if (r_) { *out = h.release(); return r_; }
     S.tile_end = dt;
+    r_ = upload_tiles(H);
+    if (r_) { *out = h.release(); return r_; }
     r_ = upload(H, &dt, H->block_group);
     if (r_) { *out = h.release(); return r_; }
     S.block_group = dt;
@@ -968,6 +994,16 @@ extern "C" int wg_set_stations(wg_handle *h, int32_t n_st, const int32_t *cells)
 
 namespace {
 
+// Launches the cell kernel with the handle's snow chunk.
+void launch_cell(wg_handle *h, dim3 grid, int k, int do_post, int do_vertical, int write_diag, int slot) {
+  if (h->snow_chunk == 10)
+    hipLaunchKernelGGL(wg_cell_kernel<10>, grid, dim3(h->block_threads_cell), 0, h->stream, h->S, h->X, h->T,
+                       h->d_forcing, h->n_forcing_days, h->d_clock, k, do_post, do_vertical, write_diag, slot);
+  else
+    hipLaunchKernelGGL(wg_cell_kernel<5>, grid, dim3(h->block_threads_cell), 0, h->stream, h->S, h->X, h->T,
+                       h->d_forcing, h->n_forcing_days, h->d_clock, k, do_post, do_vertical, write_diag, slot);
+}
+
 // Enqueues one block of `days` days on the stream (captured or eager).
 int enqueue_block(wg_handle *h, int days, hipEvent_t *evs) {
   const int32_t M = h->m;
@@ -977,9 +1013,7 @@ int enqueue_block(wg_handle *h, int days, hipEvent_t *evs) {
   for (int k = 0; k < days; k++) {
     const int slot = k % WG_DAILY_SLOTS;
     if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], h->stream));
-    hipLaunchKernelGGL(wg_cell_kernel, cgrid, dim3(h->block_threads_cell), 0, h->stream, h->S, h->X, h->T,
-                       h->d_forcing, h->n_forcing_days, h->d_clock, k, k > 0 ? 1 : 0, 1, k == days - 1 ? 1 : 0,
-                       slot);
+    launch_cell(h, cgrid, k, k > 0 ? 1 : 0, 1, k == days - 1 ? 1 : 0, slot);
     HIP_CHECK(h, hipGetLastError());
     if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], h->stream));
     hipLaunchKernelGGL(wg_chain_kernel, chgrid, dim3(h->block_threads_chain),
@@ -996,8 +1030,7 @@ int enqueue_block(wg_handle *h, int days, hipEvent_t *evs) {
   }
   if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], h->stream));
   // closing pass: finish the last day so that the state is complete
-  hipLaunchKernelGGL(wg_cell_kernel, cgrid, dim3(h->block_threads_cell), 0, h->stream, h->S, h->X, h->T,
-                     h->d_forcing, h->n_forcing_days, h->d_clock, 0, 1, 0, 0, 0);
+  launch_cell(h, cgrid, 0, 1, 0, 0, 0);
   HIP_CHECK(h, hipGetLastError());
   if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], h->stream));
   return WG_OK;
@@ -1106,6 +1139,101 @@ extern "C" int wg_step_days_profiled(wg_handle *h, int32_t forcing_day0, int32_t
   h->last_launches[2] = 2 * int64_t(n_days) + 1;
   h->last_daily_days = std::min(n_days, WG_DAILY_SLOTS);
   return WG_OK;
+}
+
+namespace {
+
+void clear_graphs(wg_handle *h) {
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
+}
+
+int apply_config(wg_handle *h, int cell_threads, int chain_threads, int chunk) {
+  h->block_threads_cell = cell_threads;
+  h->block_threads_chain = chain_threads;
+  h->snow_chunk = chunk;
+  build_tiles(h);
+  clear_graphs(h);
+  return upload_tiles(h);
+}
+
+}  // namespace
+
+// Tries cell-kernel tiles of 128 and 256 cells, chain workgroups of 64, 128 and 256
+// threads and snow chunks of 5 and 10 sub-cell pairs on n_days days of the loaded
+// forcing, each from the same saved state, and keeps the fastest.  Results do not
+// depend on the choice (every configuration is bit-identical).
+extern "C" int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double *ms_per_day) {
+  if (!h || n_days <= 0) return WG_E_ARG;
+  if (n_days > h->n_forcing_days) return fail(h, WG_E_ARG, "autotune needs %d loaded forcing days", n_days);
+  HIP_CHECK(h, hipSetDevice(h->device));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  size_t total = 0;
+  for (const auto &b : h->bufs) total += b.bytes;
+  char *snap = nullptr;
+  if (hipMalloc(&snap, total) != hipSuccess) return fail(h, WG_E_NOMEM, "autotune snapshot of %zu bytes", total);
+  auto copy_all = [&](bool save) -> hipError_t {
+    size_t off = 0;
+    for (const auto &b : h->bufs) {
+      hipError_t e = save ? hipMemcpyAsync(snap + off, b.p, b.bytes, hipMemcpyDeviceToDevice, h->stream)
+                          : hipMemcpyAsync(b.p, snap + off, b.bytes, hipMemcpyDeviceToDevice, h->stream);
+      if (e != hipSuccess) return e;
+      off += b.bytes;
+    }
+    return hipStreamSynchronize(h->stream);
+  };
+  const bool started = h->started;
+  const int32_t daily = h->last_daily_days;
+  int rc = WG_OK;
+  if (copy_all(true) != hipSuccess) rc = fail(h, WG_E_HIP, "autotune snapshot copy");
+  const int cells[] = {128, 256}, chains[] = {64, 128, 256}, chunks[] = {5, 10};
+  double best = 1e300;
+  int bc = h->block_threads_cell, bch = h->block_threads_chain, bk = h->snow_chunk;
+  for (int c : cells)
+    for (int ch : chains)
+      for (int k : chunks) {
+        if (rc) break;
+        rc = apply_config(h, c, ch, k);
+        double ms = 0;
+        for (int rep = 0; rep < 2 && !rc; rep++) {  // the first pass loads the kernels
+          if (copy_all(false) != hipSuccess) rc = fail(h, WG_E_HIP, "autotune restore");
+          if (!rc) rc = upload_tiles(h);
+          h->started = started;
+          if (!rc) rc = wg_step_days_profiled(h, 0, 1, n_days);
+          ms = h->last_ms[2];
+        }
+        if (!rc && ms < best) {
+          best = ms;
+          bc = c;
+          bch = ch;
+          bk = k;
+        }
+      }
+  if (!rc) rc = apply_config(h, bc, bch, bk);
+  if (!rc && copy_all(false) != hipSuccess) rc = fail(h, WG_E_HIP, "autotune restore");
+  if (!rc) rc = upload_tiles(h);
+  h->started = started;
+  h->last_daily_days = daily;
+  (void)hipFree(snap);
+  if (rc) return rc;
+  if (config) {
+    config[0] = bc;
+    config[1] = bch;
+    config[2] = bk;
+  }
+  if (ms_per_day) *ms_per_day = best / n_days;
+  return WG_OK;
+}
+
+// Sets the launch configuration directly (cell tile 64-256 in steps of 64, chain
+// workgroup 64-256, snow chunk 5 or 10).
+extern "C" int wg_set_launch_config(wg_handle *h, int32_t cell_threads, int32_t chain_threads, int32_t snow_chunk) {
+  if (!h || cell_threads < 64 || cell_threads > 256 || cell_threads % 64 || chain_threads < 64 ||
+      chain_threads > 256 || (snow_chunk != 5 && snow_chunk != 10))
+    return WG_E_ARG;
+  HIP_CHECK(h, hipSetDevice(h->device));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  return apply_config(h, cell_threads, chain_threads, snow_chunk);
 }
 
 extern "C" int wg_get_station_discharge(wg_handle *h, double *sum, double *daily, int32_t max_days) {

@@ -553,10 +553,6 @@ __device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p)
 
 
 // ---- one elevation sub-cell of the snow loop, daily.cpp:920-1044 ----
-#ifndef WG_SNOW_CHUNK
-#define WG_SNOW_CHUNK 5
-#endif
-constexpr int kSnowChunk = WG_SNOW_CHUNK;  // sub-cell pairs per double-buffered chunk
 
 struct SnowDay {
   double tempC, gradnt, scale_num, laf, prec_to_soil, soil_pet, snowfz, snowmt, melt_factor;
@@ -622,7 +618,9 @@ struct WgClock {
   int32_t started;
 };
 
-// One lane per (member, cell).  Grid: x over cells, y over members.
+// One lane per (member, cell).  Grid: x over cell tiles, y over members.
+// kSnowChunk: sub-cell pairs per double-buffered chunk of the snow loop (5 or 10).
+template <int kSnowChunk>
 __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgTables T,
                                                       const float *__restrict__ forcing, int64_t n_forcing_days,
                                                       const WgClock *__restrict__ clock, int k,

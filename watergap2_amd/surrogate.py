@@ -148,7 +148,7 @@ def thirty_year_mean(annual9):
     return (a[..., :9].sum(-1) + 7.0 * a[..., 6:9].sum(-1)) / 30.0
 
 
-def generate_training_set(n_basins, candidates, seed, chunk=250, log=None):
+def generate_training_set(n_basins, candidates, seed, chunk=250, log=None, min_cells=50, max_cells=5000):
     """Samples (features + gamma + CFA -> log long-term runoff) from batched GPU
     runs: chunks of basins, `candidates` (gamma, CFA) members per basin, 9 years
     each (the 30-year mean from the 3-year forcing cycle, see thirty_year_mean)."""
@@ -158,7 +158,7 @@ def generate_training_set(n_basins, candidates, seed, chunk=250, log=None):
     done = 0
     while done < n_basins:
         nb = min(chunk, n_basins - done)
-        bs = make_basins(nb, seed=seed * 1000 + done)
+        bs = make_basins(nb, seed=seed * 1000 + done, min_cells=min_cells, max_cells=max_cells)
         forcing = bs.forcing(seed=seed + done, day0=0, n_days=365 * 3)
         feats = basin_features(bs, forcing)
         g = rng.uniform(0.1, 5.0, (candidates, nb))
