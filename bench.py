@@ -386,6 +386,12 @@ def main():
           f"{info['groups']} routing groups, depth {info['max_depth']} levels, "
           f"{info['level0_cells']} headwater cells", file=sys.stderr, flush=True)
 
+    # launch configuration chosen by the library's autotuner (state restored after)
+    tuned, tuned_ms = model.autotune(n_days=5)
+    print(f"# autotune: cell tile {tuned[0]}, chain workgroup {tuned[1]}, snow chunk {tuned[2]}, "
+          f"chain mode {'dataflow' if tuned[3] else 'level'} ({tuned_ms * 1e3:.1f} us/day eager)",
+          file=sys.stderr, flush=True)
+
     # warmup (also builds the graphs)
     for s in range(args.warmup):
         model.step_days(s * dps, 1, dps)
@@ -489,7 +495,9 @@ def main():
         "config": {"workload": "global 0.5deg synthetic land mask, 67,420 cells, seed 1, 100 elevation "
                                "sub-cells, 10 years of daily steps, hipGraph-captured days",
                    "cells": n, "days_per_step": dps, "members": 1, "basins": world.n_basins,
-                   "routing_levels": info["max_depth"]},
+                   "routing_levels": info["max_depth"],
+                   "launch": {"cell_tile": tuned[0], "chain_workgroup": tuned[1], "snow_chunk": tuned[2],
+                              "chain_kernel": "dataflow" if tuned[3] else "level-synchronous"}},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "ensemble": ens,

@@ -259,15 +259,18 @@ int wg_last_timing(wg_handle *h, double *ms, int64_t *n_launches);
 
 /* Autotuner, no counterpart in the reference: times the launch configurations
  * (cell-kernel tile 128 or 256 cells, chain workgroup 64, 128 or 256 threads,
- * snow chunk 5 or 10 sub-cell pairs) on n_days days of the loaded forcing, each
- * from the same saved state, keeps the fastest and restores the state.  Needs
- * device memory for one copy of the handle's buffers.  config receives [cell
- * threads, chain threads, snow chunk]; ms_per_day the best day time.  Results are
- * identical under every configuration. */
+ * snow chunk 5 or 10 sub-cell pairs, level-synchronous or dataflow chain kernel)
+ * on n_days days of the loaded forcing, each from the same saved state, keeps the
+ * fastest and restores the state.  Needs device memory for one copy of the
+ * handle's buffers.  config receives [cell threads, chain threads, snow chunk,
+ * chain mode]; ms_per_day the best day time.  Results are identical under every
+ * configuration. */
 int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double *ms_per_day);
 /* Sets the launch configuration directly (cell threads 64..256 in steps of 64,
- * chain threads 64..256, snow chunk 5 or 10). */
-int wg_set_launch_config(wg_handle *h, int32_t cell_threads, int32_t chain_threads, int32_t snow_chunk);
+ * chain threads 64..256, snow chunk 5 or 10, chain mode 0 level-synchronous or
+ * 1 dataflow). */
+int wg_set_launch_config(wg_handle *h, int32_t cell_threads, int32_t chain_threads, int32_t snow_chunk,
+                         int32_t chain_mode);
 
 /* Schedule facts: [0] routing groups, [1] deepest group (levels), [2] level
  * slots, [3] cells routed by the cell kernel (no upstream cell). */

@@ -201,10 +201,11 @@ def test_launch_configs_and_autotune_are_bit_identical(wg, regional_world):
     w = regional_world
     f = w.forcing(seed=8, day0=0, n_days=30)
     ref = None
-    for cell, chain, chunk in ((256, 256, 5), (128, 64, 10), (192, 128, 5), (64, 256, 10)):
+    for cell, chain, chunk, mode in ((256, 256, 5, 0), (128, 64, 10, 0), (192, 128, 5, 0), (64, 256, 10, 0),
+                                     (256, 256, 5, 1), (128, 64, 10, 1), (256, 128, 5, 1)):
         m = wg.WaterBalanceModel(w)
         m.load_forcing(f)
-        m.set_launch_config(cell, chain, chunk)
+        m.set_launch_config(cell, chain, chunk, mode)
         m.step_days(0, 1, 30)
         st = m.get_state()
         if ref is None:
@@ -212,13 +213,13 @@ def test_launch_configs_and_autotune_are_bit_identical(wg, regional_world):
         else:
             for k in st:
                 if k != "started":
-                    assert np.array_equal(st[k], ref[k]), (cell, chain, chunk, k)
+                    assert np.array_equal(st[k], ref[k]), (cell, chain, chunk, mode, k)
     m = wg.WaterBalanceModel(w)
     m.load_forcing(f)
     m.step_days(0, 1, 10)
     before = m.get_state()
     cfg, ms = m.autotune(n_days=5)
-    assert cfg[0] in (128, 256) and cfg[1] in (64, 128, 256) and cfg[2] in (5, 10) and ms > 0
+    assert cfg[0] in (128, 256) and cfg[1] in (64, 128, 256) and cfg[2] in (5, 10) and cfg[3] in (0, 1) and ms > 0
     after = m.get_state()
     for k in before:
         assert np.array_equal(np.asarray(before[k]), np.asarray(after[k])), k

@@ -251,14 +251,14 @@ class WaterBalanceModel:
     def autotune(self, n_days=5):
         """Picks the fastest launch configuration on n_days of the loaded forcing
         (state restored afterwards); returns (cell threads, chain threads, snow
-        chunk) and the best time per day in ms."""
-        cfg = (C.c_int32 * 3)()
+        chunk, chain mode) and the best time per day in ms."""
+        cfg = (C.c_int32 * 4)()
         ms = C.c_double()
         self._c(self._lib.wg_autotune(self._h, n_days, cfg, C.byref(ms)))
         return tuple(cfg), ms.value
 
-    def set_launch_config(self, cell_threads=256, chain_threads=256, snow_chunk=5):
-        self._c(self._lib.wg_set_launch_config(self._h, cell_threads, chain_threads, snow_chunk))
+    def set_launch_config(self, cell_threads=256, chain_threads=256, snow_chunk=5, chain_mode=0):
+        self._c(self._lib.wg_set_launch_config(self._h, cell_threads, chain_threads, snow_chunk, chain_mode))
 
     def timing(self):
         ms = (C.c_double * 3)()

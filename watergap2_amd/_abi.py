@@ -215,7 +215,7 @@ def _declare(lib):
     lib.wg_stream.restype = C.c_void_p
     lib.wg_last_timing.argtypes = [C.c_void_p, P_d, C.POINTER(C.c_int64)]
     lib.wg_autotune.argtypes = [C.c_void_p, i32, P_i32, P_d]
-    lib.wg_set_launch_config.argtypes = [C.c_void_p, i32, i32, i32]
+    lib.wg_set_launch_config.argtypes = [C.c_void_p, i32, i32, i32, i32]
     lib.wg_gamma_search_init.argtypes = [C.POINTER(GammaSearch)]
     lib.wg_gamma_search_init.restype = None
     lib.wg_gamma_start.argtypes = [C.POINTER(GammaSearch), C.c_float]

@@ -68,6 +68,7 @@ struct wg_handle {
   std::map<int, hipGraphExec_t> graphs;
   int block_threads_cell = 256, block_threads_chain = 256;
   int snow_chunk = 5;             // sub-cell pairs per double-buffered chunk of the cell kernel
+  int chain_mode = 0;             // 0: level-synchronous chain kernel, 1: dataflow chain kernel
   int32_t max_group_cells = 0;
   int n_xcd = 8;
   std::vector<int32_t> bin_start, bin_end;   // device cell range of each XCD bin
@@ -415,6 +416,7 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
   if (e != hipSuccess) return WG_E_HIP;
   if (const char *v = getenv("WG_CHAIN_THREADS")) h->block_threads_chain = std::max(64, std::min(256, atoi(v)));
   if (const char *v = getenv("WG_XCDS")) h->n_xcd = std::max(1, std::min(64, atoi(v)));
+  if (const char *v = getenv("WG_CHAIN_MODE")) h->chain_mode = atoi(v) == 1 ? 1 : 0;
   int group_cells = 1024;
   if (const char *v = getenv("WG_GROUP_CELLS")) group_cells = std::max(1, atoi(v));
   int rc = build_schedule(h.get(), st, group_cells);
@@ -422,6 +424,7 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     *out = h.release();
     return rc;
   }
+  if (h->max_depth > WG_MAX_GROUP_DEPTH) h->chain_mode = 0;  // the dataflow walk keeps the levels in LDS
   wg_handle *H = h.get();
   const int32_t M = n_members;
   {
@@ -737,6 +740,7 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
   if (!rc) rc = dev_alloc(H, &X.smax, NM);
   if (!rc) rc = dev_alloc(H, &X.snow2, NM * WG_DEV_NPAIR);
   if (!rc) rc = dev_alloc(H, &X.snow_any, NM);
+  if (!rc) rc = dev_alloc(H, &X.flow_error, 1);
   if (!rc) rc = dev_alloc(H, &X.st_sum, 1);
   if (!rc) rc = dev_alloc(H, &X.st_daily, 1);
   if (!rc) rc = dev_alloc(H, &H->d_clock, 1);
@@ -1016,9 +1020,14 @@ int enqueue_block(wg_handle *h, int days, hipEvent_t *evs) {
     launch_cell(h, cgrid, k, k > 0 ? 1 : 0, 1, k == days - 1 ? 1 : 0, slot);
     HIP_CHECK(h, hipGetLastError());
     if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], h->stream));
-    hipLaunchKernelGGL(wg_chain_kernel, chgrid, dim3(h->block_threads_chain),
-                       size_t(h->max_group_cells) * sizeof(double), h->stream, h->S, h->X, h->T,
-                       h->d_level_off, h->d_group_first, h->d_clock, k, slot);
+    if (h->chain_mode == 1)
+      hipLaunchKernelGGL(wg_chain_flow_kernel, chgrid, dim3(h->block_threads_chain),
+                         size_t(h->max_group_cells) * sizeof(double), h->stream, h->S, h->X, h->T,
+                         h->d_level_off, h->d_group_first, h->d_clock, k, slot);
+    else
+      hipLaunchKernelGGL(wg_chain_kernel, chgrid, dim3(h->block_threads_chain),
+                         size_t(h->max_group_cells) * sizeof(double), h->stream, h->S, h->X, h->T,
+                         h->d_level_off, h->d_group_first, h->d_clock, k, slot);
     HIP_CHECK(h, hipGetLastError());
     if (h->n_stations > 0 && (slot == WG_DAILY_SLOTS - 1 || k == days - 1)) {
       // fold the daily station series into the running sums, in day order
@@ -1067,6 +1076,20 @@ int check_step_args(wg_handle *h, int32_t forcing_day0, int32_t first_doy, int32
   return WG_OK;
 }
 
+// The dataflow chain kernel flags a walk that hit its spin limit (it cannot for a
+// valid schedule); the day's routing is then incomplete and the call fails.
+int check_flow_error(wg_handle *h) {
+  if (h->chain_mode != 1) return WG_OK;
+  int32_t flag = 0;
+  HIP_CHECK(h, hipMemcpy(&flag, h->X.flow_error, sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (flag) {
+    const int32_t zero = 0;
+    HIP_CHECK(h, hipMemcpy(h->X.flow_error, &zero, sizeof(int32_t), hipMemcpyHostToDevice));
+    return fail(h, WG_E_STATE, "dataflow routing walk exceeded its spin limit");
+  }
+  return WG_OK;
+}
+
 }  // namespace
 
 extern "C" int wg_step_days(wg_handle *h, int32_t forcing_day0, int32_t first_doy, int32_t n_days) {
@@ -1102,7 +1125,7 @@ extern "C" int wg_step_days(wg_handle *h, int32_t forcing_day0, int32_t first_do
   h->last_ms[2] = ms;
   h->last_launches[2] = launches;
   h->last_daily_days = std::min(WG_DAILY_SLOTS, (n_days - 1) % block + 1);
-  return WG_OK;
+  return check_flow_error(h);
 }
 
 // Eager variant with an event pair around every kernel: per-kernel durations for
@@ -1138,7 +1161,7 @@ extern "C" int wg_step_days_profiled(wg_handle *h, int32_t forcing_day0, int32_t
   h->last_launches[1] = n_days;
   h->last_launches[2] = 2 * int64_t(n_days) + 1;
   h->last_daily_days = std::min(n_days, WG_DAILY_SLOTS);
-  return WG_OK;
+  return check_flow_error(h);
 }
 
 namespace {
@@ -1148,10 +1171,13 @@ void clear_graphs(wg_handle *h) {
   h->graphs.clear();
 }
 
-int apply_config(wg_handle *h, int cell_threads, int chain_threads, int chunk) {
+int apply_config(wg_handle *h, int cell_threads, int chain_threads, int chunk, int chain_mode) {
+  if (chain_mode == 1 && h->max_depth > WG_MAX_GROUP_DEPTH)
+    return fail(h, WG_E_ARG, "dataflow chain kernel supports groups of up to %d levels", WG_MAX_GROUP_DEPTH);
   h->block_threads_cell = cell_threads;
   h->block_threads_chain = chain_threads;
   h->snow_chunk = chunk;
+  h->chain_mode = chain_mode;
   build_tiles(h);
   clear_graphs(h);
   return upload_tiles(h);
@@ -1186,14 +1212,16 @@ extern "C" int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double
   const int32_t daily = h->last_daily_days;
   int rc = WG_OK;
   if (copy_all(true) != hipSuccess) rc = fail(h, WG_E_HIP, "autotune snapshot copy");
-  const int cells[] = {128, 256}, chains[] = {64, 128, 256}, chunks[] = {5, 10};
+  const int cells[] = {128, 256}, chains[] = {64, 128, 256}, chunks[] = {5, 10}, modes[] = {0, 1};
   double best = 1e300;
-  int bc = h->block_threads_cell, bch = h->block_threads_chain, bk = h->snow_chunk;
+  int bc = h->block_threads_cell, bch = h->block_threads_chain, bk = h->snow_chunk, bm = h->chain_mode;
   for (int c : cells)
     for (int ch : chains)
-      for (int k : chunks) {
+      for (int k : chunks)
+        for (int md : modes) {
         if (rc) break;
-        rc = apply_config(h, c, ch, k);
+        if (md == 1 && h->max_depth > WG_MAX_GROUP_DEPTH) continue;
+        rc = apply_config(h, c, ch, k, md);
         double ms = 0;
         for (int rep = 0; rep < 2 && !rc; rep++) {  // the first pass loads the kernels
           if (copy_all(false) != hipSuccess) rc = fail(h, WG_E_HIP, "autotune restore");
@@ -1207,9 +1235,10 @@ extern "C" int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double
           bc = c;
           bch = ch;
           bk = k;
+          bm = md;
         }
       }
-  if (!rc) rc = apply_config(h, bc, bch, bk);
+  if (!rc) rc = apply_config(h, bc, bch, bk, bm);
   if (!rc && copy_all(false) != hipSuccess) rc = fail(h, WG_E_HIP, "autotune restore");
   if (!rc) rc = upload_tiles(h);
   h->started = started;
@@ -1220,6 +1249,7 @@ extern "C" int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double
     config[0] = bc;
     config[1] = bch;
     config[2] = bk;
+    config[3] = bm;
   }
   if (ms_per_day) *ms_per_day = best / n_days;
   return WG_OK;
@@ -1227,13 +1257,14 @@ extern "C" int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double
 
 // Sets the launch configuration directly (cell tile 64-256 in steps of 64, chain
 // workgroup 64-256, snow chunk 5 or 10).
-extern "C" int wg_set_launch_config(wg_handle *h, int32_t cell_threads, int32_t chain_threads, int32_t snow_chunk) {
+extern "C" int wg_set_launch_config(wg_handle *h, int32_t cell_threads, int32_t chain_threads, int32_t snow_chunk,
+                                    int32_t chain_mode) {
   if (!h || cell_threads < 64 || cell_threads > 256 || cell_threads % 64 || chain_threads < 64 ||
-      chain_threads > 256 || (snow_chunk != 5 && snow_chunk != 10))
+      chain_threads > 256 || (snow_chunk != 5 && snow_chunk != 10) || (chain_mode != 0 && chain_mode != 1))
     return WG_E_ARG;
   HIP_CHECK(h, hipSetDevice(h->device));
   HIP_CHECK(h, hipStreamSynchronize(h->stream));
-  return apply_config(h, cell_threads, chain_threads, snow_chunk);
+  return apply_config(h, cell_threads, chain_threads, snow_chunk, chain_mode);
 }
 
 extern "C" int wg_get_station_discharge(wg_handle *h, double *sum, double *daily, int32_t max_days) {

@@ -102,6 +102,7 @@ struct WgState {
   // stations
   double *st_sum;               // [M][n_stations]
   double *st_daily;             // [M][WG_DAILY_SLOTS][n_stations]
+  int32_t *flow_error;           // [1] set when the dataflow chain walk hits its spin limit
   // forcing perturbation
   const uint64_t *member_seed;
   double prec_sigma, temp_sigma;
@@ -109,5 +110,6 @@ struct WgState {
 };
 
 #define WG_DAILY_SLOTS 366
+#define WG_MAX_GROUP_DEPTH 4096   // Kahn levels of one routing group (dataflow chain kernel)
 
 #endif

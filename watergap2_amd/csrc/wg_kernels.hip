@@ -1337,6 +1337,124 @@ __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, Wg
   (void)T;
 }
 
+// Dataflow walk of a routing group, without a barrier per level.  Each thread owns
+// the cells of the group's levels at positions tid, tid + nt, ... (so its cells
+// come in level order) and routes each one as soon as the transported volumes of
+// all its upstream cells are in LDS; LDS holds a sentinel bit pattern until a
+// cell's volume is written.  The unfinished cell of lowest level always has its
+// inputs ready and its owner at it, so the walk cannot stall; a spin limit that
+// every lane reaches ends the kernel (with an error flag) should that ever fail.
+constexpr unsigned long long kTvPending = 0xFFF4A11DDEADBEEFull;  // a NaN no arithmetic produces
+constexpr uint32_t kFlowSpinLimit = 1u << 24;
+
+__device__ inline bool flow_next(const int32_t *s_lo, int l1_rel, int nt, int &l, int &p) {
+  // advance (l, p) to this thread's next cell: same level + nt, else the next level
+  // with a cell at this thread's position (l relative to the group's first level)
+  p += nt;
+  const int tid = threadIdx.x;
+  while (l < l1_rel && p >= s_lo[l + 1]) {
+    l++;
+    if (l < l1_rel) p = s_lo[l] + tid;
+  }
+  return l < l1_rel;
+}
+
+__global__ __launch_bounds__(256) void wg_chain_flow_kernel(WgStatic S, WgState X, WgTables T,
+                                                            const int32_t *__restrict__ group_level_off,
+                                                            const int32_t *__restrict__ group_first_level,
+                                                            const WgClock *__restrict__ clock, int k,
+                                                            int daily_slot) {
+  extern __shared__ double s_tv[];
+  __shared__ int32_t s_lo[WG_MAX_GROUP_DEPTH + 1];
+  const int g = S.block_group[blockIdx.x];
+  if (g < 0) return;
+  const int m = blockIdx.y;
+  const int n = S.n;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int doy = int((clock->doy0 - 1 + k) % 365) + 1;
+  const int month = month_of_doy(doy);
+  const int l0 = group_first_level[g], l1 = group_first_level[g + 1];
+  const int depth = l1 - l0;
+  const int gs = group_level_off[l0], ge = group_level_off[l1];
+  const int lvl0_end = group_level_off[l0 + 1];
+  const int64_t base = int64_t(m) * n;
+  for (int i = tid; i <= depth; i += nt) s_lo[i] = group_level_off[l0 + i];
+  volatile unsigned long long *vtv = reinterpret_cast<volatile unsigned long long *>(s_tv);
+  for (int i = gs + tid; i < ge; i += nt) {
+    double v;
+    if (i < lvl0_end)
+      v = X.tv[base + i];  // level 0 was routed by the cell kernel
+    else
+      v = (S.clink[i].flags & CFL_CONT) ? __longlong_as_double(kTvPending) : 0.;
+    s_tv[i - gs] = v;
+  }
+  __syncthreads();
+  int l = 1, p = s_lo[1] + tid;  // level relative to l0
+  while (l < depth && p >= s_lo[l + 1]) {
+    l++;
+    if (l < depth) p = s_lo[l] + tid;
+  }
+  bool have = l < depth;
+  ChainSlot cur, nxt;
+  int nl = l, np = p;
+  bool have_nxt = false;
+  if (have) {
+    prefetch_slot(S, X, base, p, cur);
+    have_nxt = flow_next(s_lo, depth, nt, nl, np);
+    if (have_nxt) prefetch_slot(S, X, base, np, nxt);
+  }
+  uint32_t spins = 0;
+  while (have) {
+    if (++spins > kFlowSpinLimit) {
+      X.flow_error[0] = 1;
+      break;
+    }
+    bool ready = true;
+    if (cur.flags & CFL_CONT) {
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (u < cur.nu && vtv[(cur.up[u >> 1] >> ((u & 1) * 16)) & 0xffff] == kTvPending) ready = false;
+    }
+    if (!ready) continue;
+    double qq = 0.;
+    if (cur.flags & CFL_CONT) {
+      double up = 0.;
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (u < cur.nu) up += __longlong_as_double(vtv[(cur.up[u >> 1] >> ((u & 1) * 16)) & 0xffff]);
+      const int64_t mp = base + p;
+      ChainRec c;
+#pragma unroll
+      for (int f = 0; f < CF_PREFIX; f++) c.v[f] = cur.a[f];
+#pragma unroll
+      for (int f = CF_PREFIX; f < CF_COUNT; f++) c.v[f] = 0.;
+      if (cur.flags & (CFL_LAKE | CFL_RES)) load_chain_suffix(X, mp, c);
+      c.inv_ksw = cur.inv_ksw;
+      c.ome_sw = cur.ome_sw;
+      c.inv_kgw = cur.inv_kgw;
+      c.ome_gw = cur.ome_gw;
+      c.area = cur.area;
+      c.cf100 = cur.cf100;
+      c.flags = cur.flags;
+      const ChainOut o = chain_step(c, up, doy, month);
+      vtv[p - gs] = __double_as_longlong(o.tv);
+      store_chain(X, mp, cur.flags, o);
+      qq = (cur.flags & CFL_LDD) ? o.tv : 0.;
+    }
+    if (cur.station >= 0)
+      X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + cur.station] = qq;
+    have = have_nxt;
+    if (have) {
+      cur = nxt;
+      l = nl;
+      p = np;
+      have_nxt = flow_next(s_lo, depth, nt, nl, np);
+      if (have_nxt) prefetch_slot(S, X, base, np, nxt);
+    }
+  }
+  (void)T;
+}
+
 // Adds the daily station discharge of slots [slot0, slot0 + n_slots) to the running
 // sums, day by day in order (the per-day += of routing.cpp:4232-4238).
 __global__ void wg_station_reduce(WgState X, int32_t n_stations, int32_t n_members, int slot0, int n_slots) {
