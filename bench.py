@@ -238,47 +238,18 @@ def surrogate_run(args):
     plain_ok = int((res["status"] == 1).sum())
     log(f"plain search: {res['model_runs']} runs in {plain_s:.1f}s, {plain_ok} within 1 %")
 
-    # surrogate proposals, each verified by the plain model; a second proposal for
-    # the basins the first one misses (the network's curve shifted by the verified
-    # miss); the plain search for what is still missed
-    def sub_runner(idx):
-        sub = BasinSet([bs.worlds[i] for i in idx])
-        cols = np.concatenate([np.arange(bs.offsets[i], bs.offsets[i + 1]) for i in idx])
-        return BasinRunner(sub, n_members=1, forcing_years=3, forcing=np.ascontiguousarray(forcing[:, :, cols]))
-
+    # surrogate-guided calibration: proposals verified by the plain model, a second
+    # round of corrected proposals, the plain search for what still misses
     t1 = time.perf_counter()
-    target = sg.runoff_target(measured.astype(np.float64), bs)
-    g_star = model.propose_gamma(feats, target)
-    prop_s = time.perf_counter() - t1
-    q = runner.run(g_star[None, :], n_years=years)[0]
-    hit = np.array([within_1pc(q[b], measured[b]) for b in range(bs.n_basins)])
-    verify_s = time.perf_counter() - t1
-    first_hits = int(hit.sum())
-    miss = np.flatnonzero(~hit)
-    second_hits = 0
-    if len(miss):
-        offset = sg.runoff_target(q[miss], bs, idx=miss) - model.predict_at(feats[miss], g_star[miss])
-        g2 = model.propose_gamma(feats[miss], target[miss], offset=offset)
-        sr = sub_runner(miss)
-        q2 = sr.run(g2[None, :], n_years=years)[0]
-        sr.close()
-        h2 = np.array([within_1pc(q2[j], measured[i]) for j, i in enumerate(miss)])
-        second_hits = int(h2.sum())
-        hit[miss[h2]] = True
-        miss = np.flatnonzero(~hit)
-    fb_ok = 0
-    fb_runs = 0
-    if len(miss):
-        sr = sub_runner(miss)
-        fres = plain_search(sr, measured[miss], n_years=years)
-        sr.close()
-        fb_ok = int((fres["status"] == 1).sum())
-        fb_runs = fres["model_runs"]
+    sres = sg.calibrate_with_surrogate(model, runner, feats, measured, log=log)
     sur_s = time.perf_counter() - t1
-    sur_ok = int(hit.sum()) + fb_ok
+    sur_ok = int(sres["within_1pc"].sum())
     runner.close()
-    log(f"surrogate: {first_hits}/{bs.n_basins} first proposals within 1 %, {second_hits} second proposals, "
-        f"fallback {len(miss)} basins ({fb_runs} runs), {sur_s:.1f}s")
+    first_hits = int((sres["round"] == 1).sum())
+    second_hits = int((sres["round"] == 2).sum())
+    fb = int((sres["round"] == 3).sum())
+    log(f"surrogate: {first_hits} first-round, {second_hits} second-round, {fb} fallback basins "
+        f"({sres['fallback_runs']} runs), {sur_s:.1f}s")
     plain_rate = plain_ok / plain_s * 3600.0
     sur_rate = sur_ok / sur_s * 3600.0
     out.update({
@@ -290,11 +261,11 @@ def surrogate_run(args):
                    "basins": bs.n_basins, "cells": bs.n, "years": years},
         "plain": {"seconds": round(plain_s, 2), "model_runs": res["model_runs"], "within_1pc": plain_ok,
                   "runs_per_basin_mean": round(float(res["runs"].mean()), 2)},
-        "surrogate": {"seconds": round(sur_s, 2), "proposal_seconds": round(prop_s, 3),
-                      "verify_seconds": round(verify_s, 2), "first_proposal_hits": first_hits,
-                      "second_proposal_hits": second_hits,
-                      "fallback_basins": int(len(miss)), "fallback_runs": fb_runs, "within_1pc": sur_ok,
-                      "median_abs_gamma_error": round(float(np.median(np.abs(g_star - hidden))), 4)},
+        "surrogate": {"seconds": round(sur_s, 2), "first_round_hits": first_hits,
+                      "second_round_hits": second_hits, "fallback_basins": fb,
+                      "fallback_runs": sres["fallback_runs"], "within_1pc": sur_ok,
+                      "runs_per_basin_mean": round(float(sres["runs"].mean()), 2),
+                      "median_abs_gamma_error": round(float(np.median(np.abs(sres["gamma"] - hidden))), 4)},
     })
     return out
 

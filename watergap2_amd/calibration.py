@@ -117,6 +117,7 @@ class BasinRunner:
         if forcing is None:
             forcing = basins.forcing(seed=seed, day0=0, n_days=365 * forcing_years)
         assert forcing.shape[0] == 365 * forcing_years
+        self.forcing = forcing
         self.model.load_forcing(forcing)
         self.initial = self.model.get_state(0)
         self.base_gamma = basins.params[PARAM_INDEX["P_GAMRUN_C"]].copy()

@@ -177,6 +177,83 @@ def generate_training_set(n_basins, candidates, seed, chunk=250, log=None, min_c
     return np.concatenate(xs).astype(np.float32), np.concatenate(ts).astype(np.float32)
 
 
+def long_term_mean(annual):
+    """The 30-year mean discharge the network predicts, from a run's annual series:
+    the plain mean of a 30-year run, or thirty_year_mean of a 9-year one."""
+    a = np.asarray(annual, dtype=np.float64)
+    return thirty_year_mean(a) if a.shape[-1] == 9 else a.mean(-1)
+
+
+def calibrate_with_surrogate(model, runner, feats, measured, spread=0.006, log=None):
+    """Surrogate-guided calibration of every basin of the runner's set.
+
+    1. The network's proposal for each basin, verified by one model run of the set.
+    2. For the misses, the network's curve shifted by the verified miss, aimed at
+       the target and at the target +- `spread` (log runoff): three proposals per
+       basin, verified as three members of one run of just those basins.
+    3. The plain search (the reference's findNewGamma) for what still misses.
+
+    Returns per-basin round (1, 2, or 3 = plain fallback), gamma, model runs per
+    basin, whether it meets the 1 % criterion, and the fallback's lock-step runs."""
+    from .calibration import BasinRunner, BasinSet, plain_search, within_1pc
+    bs = runner.basins
+    nb = bs.n_basins
+    measured = np.asarray(measured)
+    n_years = measured.shape[-1]
+    areas = basin_areas(bs)
+
+    def log_runoff(annual, idx):
+        return np.log(np.maximum(long_term_mean(annual), 1e-12) * 1e6 / areas[idx])
+
+    every = np.arange(nb)
+    target = log_runoff(measured, every)
+    gam = model.propose_gamma(feats, target)
+    q = runner.run(gam[None, :], n_years=n_years)[0]
+    rnd = np.zeros(nb, dtype=np.int32)
+    runs = np.ones(nb, dtype=np.int32)
+    hit = np.array([within_1pc(q[b], measured[b]) for b in range(nb)])
+    rnd[hit] = 1
+    miss = np.flatnonzero(~hit)
+    if log:
+        log(f"surrogate round 1: {int(hit.sum())}/{nb} within 1 %")
+
+    def sub_runner(idx, members):
+        cols = np.concatenate([np.arange(bs.offsets[i], bs.offsets[i + 1]) for i in idx])
+        return BasinRunner(BasinSet([bs.worlds[i] for i in idx]), n_members=members,
+                           forcing_years=runner.forcing_years,
+                           forcing=np.ascontiguousarray(runner.forcing[:, :, cols]))
+
+    if len(miss):
+        offset = log_runoff(q[miss], miss) - model.predict_at(feats[miss], gam[miss])
+        shifts = (0.0, -spread, spread)
+        g2 = np.stack([model.propose_gamma(feats[miss], target[miss] + s_, offset=offset) for s_ in shifts])
+        sr = sub_runner(miss, len(shifts))
+        q2 = sr.run(g2, n_years=n_years)
+        sr.close()
+        runs[miss] += 1
+        for j, b in enumerate(miss):
+            for k in range(len(shifts)):
+                if within_1pc(q2[k, j], measured[b]):
+                    hit[b] = True
+                    rnd[b] = 2
+                    gam[b] = g2[k, j]
+                    break
+        miss = np.flatnonzero(~hit)
+        if log:
+            log(f"surrogate round 2: {int((rnd == 2).sum())} more within 1 %, {len(miss)} to the plain search")
+    fallback_runs = 0
+    if len(miss):
+        sr = sub_runner(miss, 1)
+        res = plain_search(sr, measured[miss], n_years=n_years)
+        sr.close()
+        fallback_runs = res["model_runs"]
+        runs[miss] += res["runs"]
+        rnd[miss] = 3
+        gam[miss] = res["gamma"]
+        hit[miss] = res["status"] == 1
+    return {"round": rnd, "gamma": gam, "runs": runs, "within_1pc": hit, "fallback_runs": fallback_runs}
+
+
 class Surrogate:
     """Trained network plus the feature standardisation it was trained with."""
 
@@ -186,7 +263,9 @@ class Surrogate:
     @classmethod
     def fit(cls, x, t, epochs=(30, 15), lrs=(2e-3, 5e-4), batch=1024, seed=0, log=None):
         mean = x.mean(0)
-        std = x.std(0) + 1e-6
+        # a floor of 1 % of the mean keeps nearly constant features (the synthetic
+        # climate varies little between basins) from amplifying small differences
+        std = np.sqrt(x.std(0) ** 2 + (0.01 * np.abs(mean)) ** 2 + 1e-8)
         mlp = MLP(seed=seed)
         xn = ((x - mean) / std).astype(np.float32)
         losses = []
