@@ -21,8 +21,9 @@ def test_surrogate_needs_fewer_runs_than_the_plain_search(wg):
     feats = sg.basin_features(bs, forcing)
     hidden = np.random.default_rng(3).uniform(0.3, 4.5, bs.n_basins)
     r = BasinRunner(bs, n_members=1, forcing_years=3, forcing=forcing)
-    measured = r.run(hidden[None, :], n_years=9)[0].astype(np.float32)
-    plain = plain_search(r, measured, n_years=9)
+    # 30 years, the period the network's training targets stand for
+    measured = r.run(hidden[None, :], n_years=30)[0].astype(np.float32)
+    plain = plain_search(r, measured, n_years=30)
     sur = sg.calibrate_with_surrogate(model, r, feats, measured)
     r.close()
     print(f"rounds {np.bincount(sur['round'], minlength=4)[1:].tolist()}; model runs per basin: surrogate "
