@@ -11,8 +11,8 @@ pytestmark = pytest.mark.gpu
 def test_surrogate_needs_fewer_runs_than_the_plain_search(wg):
     from watergap2_amd import surrogate as sg
     from watergap2_amd.calibration import BasinRunner, make_basins, plain_search
-    print("case: surrogate on 16 unseen basins (trained on 240 small basins)", flush=True)
-    x, t = sg.generate_training_set(240, 24, seed=31, chunk=120, min_cells=30, max_cells=200)
+    print("case: surrogate on 16 unseen basins (trained on 800 small basins)", flush=True)
+    x, t = sg.generate_training_set(800, 12, seed=31, chunk=400, min_cells=30, max_cells=200)
     ok = np.isfinite(t)
     model, losses = sg.Surrogate.fit(x[ok], t[ok], epochs=(40, 20), batch=256)
     assert losses[-1] < 0.2 * losses[0]
@@ -26,7 +26,7 @@ def test_surrogate_needs_fewer_runs_than_the_plain_search(wg):
     plain = plain_search(r, measured, n_years=30)
     sur = sg.calibrate_with_surrogate(model, r, feats, measured)
     r.close()
-    print(f"rounds {np.bincount(sur['round'], minlength=4)[1:].tolist()}; model runs per basin: surrogate "
+    print(f"rounds {np.bincount(sur['round'], minlength=5)[1:].tolist()}; model runs per basin: surrogate "
           f"{sur['runs'].mean():.2f}, plain {plain['runs'].mean():.2f}", flush=True)
     assert sur["within_1pc"][plain["status"] == 1].all()
     assert sur["runs"].mean() < plain["runs"].mean()

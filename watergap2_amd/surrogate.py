@@ -184,17 +184,21 @@ def long_term_mean(annual):
     return thirty_year_mean(a) if a.shape[-1] == 9 else a.mean(-1)
 
 
-def calibrate_with_surrogate(model, runner, feats, measured, spread=0.006, log=None):
+def calibrate_with_surrogate(model, runner, feats, measured, spread=0.006, rounds=3, log=None):
     """Surrogate-guided calibration of every basin of the runner's set.
 
-    1. The network's proposal for each basin, verified by one model run of the set.
-    2. For the misses, the network's curve shifted by the verified miss, aimed at
-       the target and at the target +- `spread` (log runoff): three proposals per
-       basin, verified as three members of one run of just those basins.
-    3. The plain search (the reference's findNewGamma) for what still misses.
+    Round 1: the network's proposal for each basin, verified by one model run of
+    the set.  Rounds 2..`rounds`, for the basins still missing the 1 % criterion:
+    the network's curve corrected through the verified points (shifted by the
+    miss after one point; shifted and scaled through the two closest points after
+    two or more), aimed at the target and at the target +- `spread` (log runoff),
+    three proposals per basin verified as three members of one run of just those
+    basins.  Then the plain search (the reference's findNewGamma) for what still
+    misses.
 
-    Returns per-basin round (1, 2, or 3 = plain fallback), gamma, model runs per
-    basin, whether it meets the 1 % criterion, and the fallback's lock-step runs."""
+    Returns per-basin round (1..rounds, or rounds + 1 = plain fallback), gamma,
+    model runs per basin, whether it meets the 1 % criterion, and the fallback's
+    lock-step runs."""
     from .calibration import BasinRunner, BasinSet, plain_search, within_1pc
     bs = runner.basins
     nb = bs.n_basins
@@ -205,6 +209,12 @@ def calibrate_with_surrogate(model, runner, feats, measured, spread=0.006, log=N
     def log_runoff(annual, idx):
         return np.log(np.maximum(long_term_mean(annual), 1e-12) * 1e6 / areas[idx])
 
+    def sub_runner(idx, members):
+        cols = np.concatenate([np.arange(bs.offsets[i], bs.offsets[i + 1]) for i in idx])
+        return BasinRunner(BasinSet([bs.worlds[i] for i in idx]), n_members=members,
+                           forcing_years=runner.forcing_years,
+                           forcing=np.ascontiguousarray(runner.forcing[:, :, cols]))
+
     every = np.arange(nb)
     target = log_runoff(measured, every)
     gam = model.propose_gamma(feats, target)
@@ -213,34 +223,47 @@ def calibrate_with_surrogate(model, runner, feats, measured, spread=0.006, log=N
     runs = np.ones(nb, dtype=np.int32)
     hit = np.array([within_1pc(q[b], measured[b]) for b in range(nb)])
     rnd[hit] = 1
-    miss = np.flatnonzero(~hit)
+    # verified points per basin: (gamma, log runoff), kept sorted by distance to the target
+    points = [[(float(gam[b]), float(log_runoff(q[b:b + 1], [b])[0]))] for b in range(nb)]
     if log:
         log(f"surrogate round 1: {int(hit.sum())}/{nb} within 1 %")
-
-    def sub_runner(idx, members):
-        cols = np.concatenate([np.arange(bs.offsets[i], bs.offsets[i + 1]) for i in idx])
-        return BasinRunner(BasinSet([bs.worlds[i] for i in idx]), n_members=members,
-                           forcing_years=runner.forcing_years,
-                           forcing=np.ascontiguousarray(runner.forcing[:, :, cols]))
-
-    if len(miss):
-        offset = log_runoff(q[miss], miss) - model.predict_at(feats[miss], gam[miss])
-        shifts = (0.0, -spread, spread)
-        g2 = np.stack([model.propose_gamma(feats[miss], target[miss] + s_, offset=offset) for s_ in shifts])
+    shifts = (0.0, -spread, spread)
+    for r in range(2, rounds + 1):
+        miss = np.flatnonzero(~hit)
+        if not len(miss):
+            break
+        best2 = [sorted(points[b], key=lambda pt: abs(pt[1] - target[b]))[:2] for b in miss]
+        g1 = np.array([p_[0][0] for p_ in best2])
+        y1 = np.array([p_[0][1] for p_ in best2])
+        f1 = model.predict_at(feats[miss], g1)
+        scale = np.ones(len(miss))
+        shift = y1 - f1
+        two = np.array([len(p_) == 2 for p_ in best2])
+        if two.any():
+            g2 = np.array([p_[1][0] if len(p_) == 2 else p_[0][0] for p_ in best2])
+            y2 = np.array([p_[1][1] if len(p_) == 2 else p_[0][1] for p_ in best2])
+            f2 = model.predict_at(feats[miss], g2)
+            ok = two & (np.abs(f1 - f2) > 1e-6) & ((y1 - y2) * (f1 - f2) > 0)
+            scale[ok] = (y1[ok] - y2[ok]) / (f1[ok] - f2[ok])
+            shift[ok] = y1[ok] - scale[ok] * f1[ok]
+        # network values that the corrected curves map to the aimed runoff
+        props = np.stack([model.propose_gamma(feats[miss], (target[miss] + s_ - shift) / scale) for s_ in shifts])
         sr = sub_runner(miss, len(shifts))
-        q2 = sr.run(g2, n_years=n_years)
+        q2 = sr.run(props, n_years=n_years)
         sr.close()
         runs[miss] += 1
+        n_new = 0
         for j, b in enumerate(miss):
             for k in range(len(shifts)):
-                if within_1pc(q2[k, j], measured[b]):
+                points[b].append((float(props[k, j]), float(log_runoff(q2[k, j:j + 1], [b])[0])))
+                if not hit[b] and within_1pc(q2[k, j], measured[b]):
                     hit[b] = True
-                    rnd[b] = 2
-                    gam[b] = g2[k, j]
-                    break
-        miss = np.flatnonzero(~hit)
+                    rnd[b] = r
+                    gam[b] = props[k, j]
+                    n_new += 1
         if log:
-            log(f"surrogate round 2: {int((rnd == 2).sum())} more within 1 %, {len(miss)} to the plain search")
+            log(f"surrogate round {r}: {n_new} more within 1 %, {int((~hit).sum())} left")
+    miss = np.flatnonzero(~hit)
     fallback_runs = 0
     if len(miss):
         sr = sub_runner(miss, 1)
@@ -248,7 +271,7 @@ def calibrate_with_surrogate(model, runner, feats, measured, spread=0.006, log=N
         sr.close()
         fallback_runs = res["model_runs"]
         runs[miss] += res["runs"]
-        rnd[miss] = 3
+        rnd[miss] = rounds + 1
         gam[miss] = res["gamma"]
         hit[miss] = res["status"] == 1
     return {"round": rnd, "gamma": gam, "runs": runs, "within_1pc": hit, "fallback_runs": fallback_runs}
