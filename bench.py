@@ -360,7 +360,7 @@ def main():
     # launch configuration chosen by the library's autotuner (state restored after)
     tuned, tuned_ms = model.autotune(n_days=5)
     print(f"# autotune: cell tile {tuned[0]}, chain workgroup {tuned[1]}, snow chunk {tuned[2]}, "
-          f"chain mode {'dataflow' if tuned[3] else 'level'} ({tuned_ms * 1e3:.1f} us/day eager)",
+          f"chain mode {tuned[3]} ({tuned_ms * 1e3:.1f} us/day eager)",
           file=sys.stderr, flush=True)
 
     # warmup (also builds the graphs)
@@ -468,7 +468,8 @@ def main():
                    "cells": n, "days_per_step": dps, "members": 1, "basins": world.n_basins,
                    "routing_levels": info["max_depth"],
                    "launch": {"cell_tile": tuned[0], "chain_workgroup": tuned[1], "snow_chunk": tuned[2],
-                              "chain_kernel": "dataflow" if tuned[3] else "level-synchronous"}},
+                              "chain_kernel": ["level walk, 1 level ahead", "dataflow walk",
+                                               "level walk, 2 levels ahead"][tuned[3]]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "ensemble": ens,

@@ -259,7 +259,7 @@ int wg_last_timing(wg_handle *h, double *ms, int64_t *n_launches);
 
 /* Autotuner, no counterpart in the reference: times the launch configurations
  * (cell-kernel tile 128 or 256 cells, chain workgroup 64, 128 or 256 threads,
- * snow chunk 5 or 10 sub-cell pairs, level-synchronous or dataflow chain kernel)
+ * snow chunk 5 or 10 sub-cell pairs, chain kernel mode 0, 1 or 2)
  * on n_days days of the loaded forcing, each from the same saved state, keeps the
  * fastest and restores the state.  Needs device memory for one copy of the
  * handle's buffers.  config receives [cell threads, chain threads, snow chunk,
@@ -267,8 +267,9 @@ int wg_last_timing(wg_handle *h, double *ms, int64_t *n_launches);
  * configuration. */
 int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double *ms_per_day);
 /* Sets the launch configuration directly (cell threads 64..256 in steps of 64,
- * chain threads 64..256, snow chunk 5 or 10, chain mode 0 level-synchronous or
- * 1 dataflow). */
+ * chain threads 64..256, snow chunk 5 or 10, chain mode 0 level walk loading
+ * one level ahead, 1 dataflow walk without per-level barriers, 2 level walk
+ * loading two levels ahead). */
 int wg_set_launch_config(wg_handle *h, int32_t cell_threads, int32_t chain_threads, int32_t snow_chunk,
                          int32_t chain_mode);
 

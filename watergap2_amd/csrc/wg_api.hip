@@ -68,7 +68,8 @@ struct wg_handle {
   std::map<int, hipGraphExec_t> graphs;
   int block_threads_cell = 256, block_threads_chain = 256;
   int snow_chunk = 5;             // sub-cell pairs per double-buffered chunk of the cell kernel
-  int chain_mode = 0;             // 0: level-synchronous chain kernel, 1: dataflow chain kernel
+  int chain_mode = 0;             // chain kernel: 0 level walk, loads one level ahead; 1 dataflow walk;
+                                  // 2 level walk, loads two levels ahead
   int32_t max_group_cells = 0;
   int n_xcd = 8;
   std::vector<int32_t> bin_start, bin_end;   // device cell range of each XCD bin
@@ -416,7 +417,7 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
   if (e != hipSuccess) return WG_E_HIP;
   if (const char *v = getenv("WG_CHAIN_THREADS")) h->block_threads_chain = std::max(64, std::min(256, atoi(v)));
   if (const char *v = getenv("WG_XCDS")) h->n_xcd = std::max(1, std::min(64, atoi(v)));
-  if (const char *v = getenv("WG_CHAIN_MODE")) h->chain_mode = atoi(v) == 1 ? 1 : 0;
+  if (const char *v = getenv("WG_CHAIN_MODE")) h->chain_mode = std::max(0, std::min(2, atoi(v)));
   int group_cells = 1024;
   if (const char *v = getenv("WG_GROUP_CELLS")) group_cells = std::max(1, atoi(v));
   int rc = build_schedule(h.get(), st, group_cells);
@@ -424,7 +425,7 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     *out = h.release();
     return rc;
   }
-  if (h->max_depth > WG_MAX_GROUP_DEPTH) h->chain_mode = 0;  // the dataflow walk keeps the levels in LDS
+  if (h->max_depth > WG_MAX_GROUP_DEPTH && h->chain_mode == 1) h->chain_mode = 0;  // dataflow keeps levels in LDS
   wg_handle *H = h.get();
   const int32_t M = n_members;
   {
@@ -1024,8 +1025,12 @@ int enqueue_block(wg_handle *h, int days, hipEvent_t *evs) {
       hipLaunchKernelGGL(wg_chain_flow_kernel, chgrid, dim3(h->block_threads_chain),
                          size_t(h->max_group_cells) * sizeof(double), h->stream, h->S, h->X, h->T,
                          h->d_level_off, h->d_group_first, h->d_clock, k, slot);
+    else if (h->chain_mode == 2)
+      hipLaunchKernelGGL(wg_chain_kernel<2>, chgrid, dim3(h->block_threads_chain),
+                         size_t(h->max_group_cells) * sizeof(double), h->stream, h->S, h->X, h->T,
+                         h->d_level_off, h->d_group_first, h->d_clock, k, slot);
     else
-      hipLaunchKernelGGL(wg_chain_kernel, chgrid, dim3(h->block_threads_chain),
+      hipLaunchKernelGGL(wg_chain_kernel<1>, chgrid, dim3(h->block_threads_chain),
                          size_t(h->max_group_cells) * sizeof(double), h->stream, h->S, h->X, h->T,
                          h->d_level_off, h->d_group_first, h->d_clock, k, slot);
     HIP_CHECK(h, hipGetLastError());
@@ -1212,7 +1217,7 @@ extern "C" int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double
   const int32_t daily = h->last_daily_days;
   int rc = WG_OK;
   if (copy_all(true) != hipSuccess) rc = fail(h, WG_E_HIP, "autotune snapshot copy");
-  const int cells[] = {128, 256}, chains[] = {64, 128, 256}, chunks[] = {5, 10}, modes[] = {0, 1};
+  const int cells[] = {128, 256}, chains[] = {64, 128, 256}, chunks[] = {5, 10}, modes[] = {0, 1, 2};
   double best = 1e300;
   int bc = h->block_threads_cell, bch = h->block_threads_chain, bk = h->snow_chunk, bm = h->chain_mode;
   for (int c : cells)
@@ -1260,7 +1265,7 @@ extern "C" int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double
 extern "C" int wg_set_launch_config(wg_handle *h, int32_t cell_threads, int32_t chain_threads, int32_t snow_chunk,
                                     int32_t chain_mode) {
   if (!h || cell_threads < 64 || cell_threads > 256 || cell_threads % 64 || chain_threads < 64 ||
-      chain_threads > 256 || (snow_chunk != 5 && snow_chunk != 10) || (chain_mode != 0 && chain_mode != 1))
+      chain_threads > 256 || (snow_chunk != 5 && snow_chunk != 10) || chain_mode < 0 || chain_mode > 2)
     return WG_E_ARG;
   HIP_CHECK(h, hipSetDevice(h->device));
   HIP_CHECK(h, hipStreamSynchronize(h->stream));

@@ -1282,15 +1282,16 @@ __device__ __forceinline__ void chain_cell(const WgStatic &S, const WgState &X, 
 // computation, so its latency overlaps this level.  `use` and `fill` alternate
 // between two named slots, never copied, so no wait on the new loads is forced.
 // Cells beyond the block size (levels wider than the block) load directly.
+template <int kAhead>
 __device__ __forceinline__ void chain_level(const WgStatic &S, const WgState &X, double *s_tv,
                                             const int32_t *__restrict__ lo, int l, int l1, int64_t base, int gs,
                                             int m, int doy, int month, int daily_slot, const ChainSlot &use,
                                             ChainSlot &fill) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int beg = lo[l], end = lo[l + 1];
-  if (l + 1 < l1) {
-    const int p = end + tid;
-    if (p < lo[l + 2]) prefetch_slot(S, X, base, p, fill);
+  if (l + kAhead < l1) {
+    const int p = lo[l + kAhead] + tid;
+    if (p < lo[l + kAhead + 1]) prefetch_slot(S, X, base, p, fill);
   }
   if (beg + tid < end) chain_cell(S, X, s_tv, use, base, beg + tid, gs, m, doy, month, daily_slot);
   for (int p = beg + tid + nt; p < end; p += nt) {
@@ -1304,6 +1305,9 @@ __device__ __forceinline__ void chain_level(const WgStatic &S, const WgState &X,
 // One workgroup per (routing group, member): levels 1..depth of the group.  The
 // transported volumes of the group live in LDS (level 0 is copied in from the
 // cell kernel's output); the static record holds group-local upstream indices.
+// Each thread's first cell of a level is loaded kAhead levels early (1 or 2), so
+// that the load latency overlaps the levels in between; the slots rotate by name.
+template <int kAhead>
 __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, WgTables T,
                                                        const int32_t *__restrict__ group_level_off,
                                                        const int32_t *__restrict__ group_first_level,
@@ -1322,17 +1326,27 @@ __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, Wg
   const int gs = lo[l0], ge = lo[l1];
   const int lvl0_end = lo[l0 + 1];
   const int64_t base = int64_t(m) * n;
-  ChainSlot a, b;
-  if (l0 + 1 < l1) {
-    const int p = lo[l0 + 1] + tid;
-    if (p < lo[l0 + 2]) prefetch_slot(S, X, base, p, a);
+  ChainSlot a, b, c;
+  for (int j = 1; j <= kAhead; j++) {
+    if (l0 + j < l1) {
+      const int p = lo[l0 + j] + tid;
+      if (p < lo[l0 + j + 1]) prefetch_slot(S, X, base, p, j == 1 ? a : b);
+    }
   }
   // level 0 was routed by the cell kernel; cells not routed today read as 0
   for (int i = gs + tid; i < ge; i += nt) s_tv[i - gs] = (i < lvl0_end) ? X.tv[base + i] : 0.;
   lds_barrier();
-  for (int l = l0 + 1; l < l1; l += 2) {
-    chain_level(S, X, s_tv, lo, l, l1, base, gs, m, doy, month, daily_slot, a, b);
-    if (l + 1 < l1) chain_level(S, X, s_tv, lo, l + 1, l1, base, gs, m, doy, month, daily_slot, b, a);
+  if (kAhead == 1) {
+    for (int l = l0 + 1; l < l1; l += 2) {
+      chain_level<1>(S, X, s_tv, lo, l, l1, base, gs, m, doy, month, daily_slot, a, b);
+      if (l + 1 < l1) chain_level<1>(S, X, s_tv, lo, l + 1, l1, base, gs, m, doy, month, daily_slot, b, a);
+    }
+  } else {
+    for (int l = l0 + 1; l < l1; l += 3) {
+      chain_level<2>(S, X, s_tv, lo, l, l1, base, gs, m, doy, month, daily_slot, a, c);
+      if (l + 1 < l1) chain_level<2>(S, X, s_tv, lo, l + 1, l1, base, gs, m, doy, month, daily_slot, b, a);
+      if (l + 2 < l1) chain_level<2>(S, X, s_tv, lo, l + 2, l1, base, gs, m, doy, month, daily_slot, c, b);
+    }
   }
   (void)T;
 }
