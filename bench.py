@@ -412,6 +412,11 @@ def main():
         "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
         "traffic": traffic_from_profiles(dominant),
         "per_kernel": per_kernel,
+        # the whole simulated day against the per-cell-day algorithmic bytes of the
+        # path (state 2,176 B + statics 555 B shared by the members)
+        "day": {"bytes_per_cell_day": 2176 + 555, "us_per_day": round((cell_ms + chain_ms) * 1e3, 2),
+                "achieved_GBs": round((2176 + 555) * n / ((cell_ms + chain_ms) * 1e-3) / 1e9, 1),
+                "frac": round((2176 + 555) * n / ((cell_ms + chain_ms) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "note": ("wg_chain_kernel walks the Kahn levels of each routing group with a workgroup "
                  "barrier per level (104 dependent levels at this grid); its time is latency, "
                  "not bandwidth"),
