@@ -1333,8 +1333,21 @@ __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, Wg
       if (p < lo[l0 + j + 1]) prefetch_slot(S, X, base, p, j == 1 ? a : b);
     }
   }
-  // level 0 was routed by the cell kernel; cells not routed today read as 0
-  for (int i = gs + tid; i < ge; i += nt) s_tv[i - gs] = (i < lvl0_end) ? X.tv[base + i] : 0.;
+  // level 0 was routed by the cell kernel; cells not routed today read as 0.
+  // Eight loads in flight per thread before their LDS stores.
+  for (int i0 = gs + tid; i0 < ge; i0 += 8 * nt) {
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int i = i0 + j * nt;
+      v[j] = (i < lvl0_end) ? X.tv[base + i] : 0.;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int i = i0 + j * nt;
+      if (i < ge) s_tv[i - gs] = v[j];
+    }
+  }
   lds_barrier();
   if (kAhead == 1) {
     for (int l = l0 + 1; l < l1; l += 2) {
