@@ -296,6 +296,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16,
                     help="OpenMP threads of the CPU oracle's vertical loop (capped at the host's)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch configuration (profiling runs)")
     ap.add_argument("--ensemble-members", type=int, default=256,
                     help="members of the batched-ensemble line (0 = skip)")
     ap.add_argument("--serial-members", type=int, default=4,
@@ -358,7 +360,10 @@ def main():
           f"{info['level0_cells']} headwater cells", file=sys.stderr, flush=True)
 
     # launch configuration chosen by the library's autotuner (state restored after)
-    tuned, tuned_ms = model.autotune(n_days=5)
+    if args.no_autotune:
+        tuned, tuned_ms = (256, 256, 5, 0), 0.0
+    else:
+        tuned, tuned_ms = model.autotune(n_days=5)
     print(f"# autotune: cell tile {tuned[0]}, chain workgroup {tuned[1]}, snow chunk {tuned[2]}, "
           f"chain mode {tuned[3]} ({tuned_ms * 1e3:.1f} us/day eager)",
           file=sys.stderr, flush=True)
