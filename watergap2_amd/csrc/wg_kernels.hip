@@ -1489,7 +1489,16 @@ __global__ void wg_station_reduce(WgState X, int32_t n_stations, int32_t n_membe
   if (i >= n_stations * n_members) return;
   const int m = i / n_stations, st = i - m * n_stations;
   double s = X.st_sum[i];
-  for (int d = slot0; d < slot0 + n_slots; d++) s += X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + d) * n_stations + st];
+  const double *q = X.st_daily + (int64_t(m) * WG_DAILY_SLOTS + slot0) * n_stations + st;
+  int d = 0;
+  for (; d + 8 <= n_slots; d += 8) {  // eight loads in flight, added in day order
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) v[j] = q[int64_t(d + j) * n_stations];
+#pragma unroll
+    for (int j = 0; j < 8; j++) s += v[j];
+  }
+  for (; d < n_slots; d++) s += q[int64_t(d) * n_stations];
   X.st_sum[i] = s;
 }
 
