@@ -201,7 +201,7 @@ int wg_rout_order(int32_t n_cells, const int32_t *inflow9, const int32_t *outflo
 
 /* Creates a model of n_members members sharing the static fields.  Reservoirs
  * with reservoir_area > 0 are in operation from the first day (resYearOpt = 1,
- * routing.cpp:5647-5720). */
+ * routing.cpp:5647-5720).  1 <= n_cells < 2^28, 1 <= n_members <= 65535. */
 int wg_create(int32_t n_cells, const wg_static_fields *st, const wg_options_profile *opt,
               int32_t n_members, wg_handle **out);
 int wg_destroy(wg_handle *h);
@@ -259,7 +259,7 @@ int wg_last_timing(wg_handle *h, double *ms, int64_t *n_launches);
 
 /* Autotuner, no counterpart in the reference: times the launch configurations
  * (cell-kernel tile 128 or 256 cells, chain workgroup 64, 128 or 256 threads,
- * snow chunk 5 or 10 sub-cell pairs, chain kernel mode 0, 1 or 2)
+ * snow chunk 5 or 10 sub-cell pairs, chain kernel mode 0, 1, 2 or 3)
  * on n_days days of the loaded forcing, each from the same saved state, keeps the
  * fastest and restores the state.  Needs device memory for one copy of the
  * handle's buffers.  config receives [cell threads, chain threads, snow chunk,
@@ -269,7 +269,8 @@ int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double *ms_per_da
 /* Sets the launch configuration directly (cell threads 64..256 in steps of 64,
  * chain threads 64..256, snow chunk 5 or 10, chain mode 0 level walk loading
  * one level ahead, 1 dataflow walk without per-level barriers, 2 level walk
- * loading two levels ahead). */
+ * loading two levels ahead, 3 wave walk: one wavefront per routing group in
+ * batches of 64 cells, the default; chain threads do not apply to mode 3). */
 int wg_set_launch_config(wg_handle *h, int32_t cell_threads, int32_t chain_threads, int32_t snow_chunk,
                          int32_t chain_mode);
 

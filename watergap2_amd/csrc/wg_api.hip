@@ -66,10 +66,10 @@ struct wg_handle {
   int32_t last_daily_days = 0;
   // graphs keyed by block length
   std::map<int, hipGraphExec_t> graphs;
-  int block_threads_cell = 256, block_threads_chain = 256;
+  int block_threads_cell = 256, block_threads_chain = 64;
   int snow_chunk = 5;             // sub-cell pairs per double-buffered chunk of the cell kernel
-  int chain_mode = 0;             // chain kernel: 0 level walk, loads one level ahead; 1 dataflow walk;
-                                  // 2 level walk, loads two levels ahead
+  int chain_mode = 3;             // chain kernel: 0 level walk, loads one level ahead; 1 dataflow walk;
+                                  // 2 level walk, loads two levels ahead; 3 wave walk in batches
   int32_t max_group_cells = 0;
   int n_xcd = 8;
   std::vector<int32_t> bin_start, bin_end;   // device cell range of each XCD bin
@@ -328,13 +328,18 @@ int build_schedule(wg_handle *h, const wg_static_fields *st, int group_cells) {
   h->max_depth = 0;
   for (int32_t d : depth_new) h->max_depth = std::max(h->max_depth, d);
   auto gid = [&](int32_t cell) { return new_gid[group_of_basin[bid[outlet[cell]]]]; };
-  // device order: (bin, group, level, routing order)
+  // device order: (bin, group, level, cells with a global lake or reservoir last,
+  // routing order).  The cells of one level are independent; putting the few with
+  // a lake or reservoir together keeps the general routing step, which only they
+  // need on every day, to few wavefronts.
+  auto lakeres = [&](int32_t c) { return (st->lake_area[c] > 0. || st->reservoir_area[c] > 0.) ? 1 : 0; };
   std::vector<int32_t> cells(n);
   for (int32_t i = 0; i < n; i++) cells[i] = i;
   std::sort(cells.begin(), cells.end(), [&](int32_t a, int32_t b) {
     const int32_t ga = gid(a), gb = gid(b);
     if (ga != gb) return ga < gb;
     if (level[a] != level[b]) return level[a] < level[b];
+    if (lakeres(a) != lakeres(b)) return lakeres(a) < lakeres(b);
     return st->rout_order[a] < st->rout_order[b];
   });
   h->perm = cells;
@@ -392,7 +397,9 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
                          wg_handle **out) {
   if (!out) return WG_E_ARG;
   *out = nullptr;
-  if (n <= 0 || !st || !opt || n_members <= 0 || n_members > 65535) return WG_E_ARG;
+  // rows of fewer than 2^28 doubles: the level walk addresses a member's row with
+  // 32-bit byte offsets below 2^31
+  if (n <= 0 || n >= (1 << 28) || !st || !opt || n_members <= 0 || n_members > 65535) return WG_E_ARG;
   const void *req[] = {st->row, st->cell_area, st->contfreq, st->aindex, st->ldd, st->land_cover, st->built_up,
                        st->smax, st->gw_factor, st->rgmax, st->texture, st->elevation, st->params, st->glo_lake,
                        st->loc_lake, st->glo_wetland, st->loc_wetland, st->lake_area, st->reservoir_area,
@@ -417,7 +424,7 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
   if (e != hipSuccess) return WG_E_HIP;
   if (const char *v = getenv("WG_CHAIN_THREADS")) h->block_threads_chain = std::max(64, std::min(256, atoi(v)));
   if (const char *v = getenv("WG_XCDS")) h->n_xcd = std::max(1, std::min(64, atoi(v)));
-  if (const char *v = getenv("WG_CHAIN_MODE")) h->chain_mode = std::max(0, std::min(2, atoi(v)));
+  if (const char *v = getenv("WG_CHAIN_MODE")) h->chain_mode = std::max(0, std::min(3, atoi(v)));
   int group_cells = 1024;
   if (const char *v = getenv("WG_GROUP_CELLS")) group_cells = std::max(1, atoi(v));
   int rc = build_schedule(h.get(), st, group_cells);
@@ -618,7 +625,8 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     if (r_) { *out = h.release(); return r_; }
     S.elev_pairs = dpairs;
   }
-  // upstream CSR in device indices, sorted (device order follows routing order within a group)
+  // upstream CSR in device indices, each list in routing order (the order of the
+  // reference's serial += into the downstream cell's inflow)
   {
     std::vector<std::vector<int32_t>> ups(n);
     for (int32_t i = 0; i < n; i++) {
@@ -647,7 +655,8 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
                     int(ups[p].size()));
       }
     for (int32_t p = 0; p < n; p++) {
-      std::sort(ups[p].begin(), ups[p].end());
+      std::sort(ups[p].begin(), ups[p].end(),
+                [&](int32_t a, int32_t b) { return st->rout_order[H->perm[a]] < st->rout_order[H->perm[b]]; });
       off[p + 1] = off[p] + int32_t(ups[p].size());
       for (int32_t u : ups[p]) idx.push_back(u - gstart[p]);
     }
@@ -665,6 +674,11 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     // routing links and outflow constants of the chain storages
     H->clink.assign(n, WgChainLink{});
     std::vector<WgChainStatic> cst(n);
+    // Kahn level of every device cell counted from its group's first level
+    std::vector<int32_t> rel_level(n, 0);
+    for (int32_t gg = 0; gg < H->n_groups; gg++)
+      for (int32_t l = H->group_first_level[gg]; l < H->group_first_level[gg + 1]; l++)
+        for (int32_t q = H->level_off[l]; q < H->level_off[l + 1]; q++) rel_level[q] = l - H->group_first_level[gg];
     for (int32_t p = 0; p < n; p++) {
       const int32_t i = H->perm[p];
       WgChainStatic &c = cst[p];
@@ -685,9 +699,15 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
       WgChainLink &lk = H->clink[p];
       lk.flags = f;
       lk.station = -1;
-      lk.nu = off[p + 1] - off[p];
-      lk.pad = 0;
+      lk.nu = int16_t(off[p + 1] - off[p]);
+      lk.level = rel_level[p];
       for (int u = 0; u < 8; u++) lk.up[u] = int16_t(u < lk.nu ? idx[off[p] + u] : 0);
+    }
+    // the largest upstream count of each level
+    for (int32_t l = 0; l + 1 < int32_t(H->level_off.size()); l++) {
+      int16_t mx = 0;
+      for (int32_t q = H->level_off[l]; q < H->level_off[l + 1]; q++) mx = std::max(mx, H->clink[q].nu);
+      for (int32_t q = H->level_off[l]; q < H->level_off[l + 1]; q++) H->clink[q].nu_level = mx;
     }
     WgChainLink *dlk;
     r_ = upload(H, &dlk, H->clink);
@@ -1021,7 +1041,10 @@ int enqueue_block(wg_handle *h, int days, hipEvent_t *evs) {
     launch_cell(h, cgrid, k, k > 0 ? 1 : 0, 1, k == days - 1 ? 1 : 0, slot);
     HIP_CHECK(h, hipGetLastError());
     if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], h->stream));
-    if (h->chain_mode == 1)
+    if (h->chain_mode == 3)
+      hipLaunchKernelGGL(wg_chain_wave_kernel, chgrid, dim3(64), size_t(h->max_group_cells) * sizeof(double),
+                         h->stream, h->S, h->X, h->T, h->d_level_off, h->d_group_first, h->d_clock, k, slot);
+    else if (h->chain_mode == 1)
       hipLaunchKernelGGL(wg_chain_flow_kernel, chgrid, dim3(h->block_threads_chain),
                          size_t(h->max_group_cells) * sizeof(double), h->stream, h->S, h->X, h->T,
                          h->d_level_off, h->d_group_first, h->d_clock, k, slot);
@@ -1217,7 +1240,7 @@ extern "C" int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double
   const int32_t daily = h->last_daily_days;
   int rc = WG_OK;
   if (copy_all(true) != hipSuccess) rc = fail(h, WG_E_HIP, "autotune snapshot copy");
-  const int cells[] = {128, 256}, chains[] = {64, 128, 256}, chunks[] = {5, 10}, modes[] = {0, 1, 2};
+  const int cells[] = {128, 256}, chains[] = {64, 128, 256}, chunks[] = {5, 10}, modes[] = {0, 1, 2, 3};
   double best = 1e300;
   int bc = h->block_threads_cell, bch = h->block_threads_chain, bk = h->snow_chunk, bm = h->chain_mode;
   for (int c : cells)
@@ -1226,6 +1249,7 @@ extern "C" int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double
         for (int md : modes) {
         if (rc) break;
         if (md == 1 && h->max_depth > WG_MAX_GROUP_DEPTH) continue;
+        if (md == 3 && ch != 64) continue;  // the wave walk runs one wavefront per group
         rc = apply_config(h, c, ch, k, md);
         double ms = 0;
         for (int rep = 0; rep < 2 && !rc; rep++) {  // the first pass loads the kernels
@@ -1265,7 +1289,7 @@ extern "C" int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double
 extern "C" int wg_set_launch_config(wg_handle *h, int32_t cell_threads, int32_t chain_threads, int32_t snow_chunk,
                                     int32_t chain_mode) {
   if (!h || cell_threads < 64 || cell_threads > 256 || cell_threads % 64 || chain_threads < 64 ||
-      chain_threads > 256 || (snow_chunk != 5 && snow_chunk != 10) || chain_mode < 0 || chain_mode > 2)
+      chain_threads > 256 || (snow_chunk != 5 && snow_chunk != 10) || chain_mode < 0 || chain_mode > 3)
     return WG_E_ARG;
   HIP_CHECK(h, hipSetDevice(h->device));
   HIP_CHECK(h, hipStreamSynchronize(h->stream));

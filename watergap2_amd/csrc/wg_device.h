@@ -19,10 +19,13 @@ struct alignas(16) WgD2 {
 
 // Routing links of a cell (device order): chain flags in bits 0-7, reservoir type
 // in bits 8-15 and the reservoir's start month in bits 16-23; the station slot;
-// the number of upstream cells and their group-local indices in routing order
-// (at most 8, D8).
+// the number of upstream cells and the largest such number on the cell's level;
+// the cell's Kahn level counted from its group's first level; the upstream cells'
+// group-local indices in routing order (at most 8, D8).
 struct alignas(16) WgChainLink {
-  int32_t flags, station, nu, pad;
+  int32_t flags, station;
+  int16_t nu, nu_level;
+  int32_t level;
   int16_t up[8];
 };
 

@@ -40,11 +40,30 @@ constexpr double kMinStorVol = 1.e-15;          // routing.h:24
 __constant__ short kFirstDayInMonth[12] = {1, 32, 60, 91, 121, 152, 182, 213, 244, 274, 305, 335};
 __constant__ short kDaysInMonth[12] = {31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31};
 
-__device__ inline int month_of_doy(int doy) {
-  int m = 0;
+// The calendar facts the routing step reads for one day of the year, computed once
+// per launch from constant table entries (no table lookup with a run-time index
+// remains in the per-cell code).
+struct DayInfo {
+  int month;              // 0-based
+  bool first_of_month;    // the day is the month's first (routing.cpp:2956-2957)
+  double days_in_month;
+};
+
+__device__ inline DayInfo day_info(int doy) {
+  DayInfo d;
+  d.month = 0;
+  int first = kFirstDayInMonth[0], len = kDaysInMonth[0];
 #pragma unroll
-  for (int k = 1; k < 12; k++) m += (doy >= kFirstDayInMonth[k]) ? 1 : 0;
-  return m;
+  for (int k = 1; k < 12; k++) {
+    if (doy >= kFirstDayInMonth[k]) {
+      d.month = k;
+      first = kFirstDayInMonth[k];
+      len = kDaysInMonth[k];
+    }
+  }
+  d.first_of_month = (doy == first);
+  d.days_in_month = double(len);
+  return d;
 }
 
 // ---- ensemble perturbation: Philox4x32-10 + Irwin-Hall normal deviates ----
@@ -120,13 +139,6 @@ struct ChainOut {
   double glo_lake, res, k_release, glo_wetl, gw, river, tv;
 };
 
-// Suffix of the record of a cell with a global lake or a reservoir.
-__device__ inline void load_chain_suffix(const WgState &X, int64_t mp, ChainRec &r) {
-  const double *src = X.crec[mp].v;
-#pragma unroll
-  for (int f = CF_PREFIX; f < CF_COUNT; f++) r.v[f] = src[f];
-}
-
 struct RiverOut {
   double st, tv;
 };
@@ -181,14 +193,17 @@ __device__ inline double arid_groundwater(double gwr_swb, double area, double cf
 // use, no station correction).  inflow_up is G_riverInflow[n] when the cell is
 // reached.  The reservoir branch (about one cell in two hundred) reads its own
 // inputs here.
-__device__ ChainOut chain_step(const ChainRec &c, double inflow_up, int day, int month) {
+__device__ ChainOut chain_step(const ChainRec &c, double inflow_up, const DayInfo &d) {
+  const int month = d.month;
   ChainOut o;
   const double *v = c.v;
   const bool arid_swb = (c.flags & CFL_ARID) != 0;
   double inflow = v[CF_INFLOW];
   double outflow = 0.;
-  double gwr_glolak = v[CF_GL_GWR], gwr_glowet = v[CF_GW_GWR], gwr_res = 0.;
-  o.glo_lake = v[CF_GL_PREV];
+  // the record suffix (CF_GL_*, CF_OW_*, CF_RES_*) is read only for cells with a
+  // global lake or reservoir: other cells' suffix holds no data
+  double gwr_glolak = (c.flags & CFL_LAKE) ? v[CF_GL_GWR] : 0., gwr_glowet = v[CF_GW_GWR], gwr_res = 0.;
+  o.glo_lake = 0.;
   o.glo_wetl = v[CF_GW_PREV];
   o.gw = v[CF_A_PREV];
   o.res = 0.;
@@ -251,7 +266,7 @@ __device__ ChainOut chain_step(const ChainRec &c, double inflow_up, int day, int
     if (fabs(st) <= kMinStorVol) st = 0.;
     double krel = v[CF_RES_KREL];
     if (month == flag_start_month(c.flags) - 1) {
-      if (day == kFirstDayInMonth[month]) {
+      if (d.first_of_month) {
         if (st < (cap * 0.1))
           krel = 0.1;
         else
@@ -262,8 +277,8 @@ __device__ ChainOut chain_step(const ChainRec &c, double inflow_up, int day, int
     if (rtype == 1) {
       // irrigation reservoir without water use: zero demand and zero downstream use
       const double mean_demand = 0.;
-      double monthlyUse = 0.0 * kDaysInMonth[month];
-      monthlyUse = monthlyUse * 1000000000. / (kDaysInMonth[month] * 86400.);
+      double monthlyUse = 0.0 * d.days_in_month;
+      monthlyUse = monthlyUse * 1000000000. / (d.days_in_month * 86400.);
       if (mean_demand >= 0.5 * qmean)
         prov_rel = qmean / 2. * (1. + monthlyUse / mean_demand);
       else
@@ -332,6 +347,79 @@ __device__ ChainOut chain_step(const ChainRec &c, double inflow_up, int day, int
   const double st = r.st, tv = r.tv;
   o.river = st;
   o.tv = tv;
+  return o;
+}
+
+// The common case of the routing step, split so that what does not depend on the
+// upstream inflow is done beforehand: a cell without a global lake or reservoir
+// whose global wetland (if any) keeps water.  In that case the arid groundwater
+// step (routing.cpp:3305-3386) does not depend on the inflow (step_pre), and the
+// inflow-dependent rest (step_fast) is the wetland and river steps with selects
+// in place of branches.  Every operation is the one chain_step performs, in the
+// same order, so the results are chain_step's bits.  step_fast flags the cells
+// that need chain_step: a lake, a reservoir, or a wetland that runs dry (which
+// rescales the wetland's groundwater recharge, routing.cpp:3193-3200).
+struct StepPre {
+  double lgwrir, gw;
+};
+
+__device__ inline StepPre step_pre(const ChainRec &c) {
+  const double *v = c.v;
+  StepPre r{v[CF_LGWRIR], v[CF_A_PREV]};
+  if (c.flags & CFL_ARID) {
+    // chain_step's sum with no lake or reservoir recharge and the wetland's unscaled
+    const double gwr_swb = v[CF_A_GWR_LL] + 0. + v[CF_A_GWR_LW] + v[CF_GW_GWR] + 0.;
+    r.lgwrir = arid_groundwater(gwr_swb, c.area, c.cf100, v[CF_A_RECH], v[CF_A_PREV], v[CF_A_S0E], c.inv_kgw,
+                                c.ome_gw, r.gw);
+  }
+  return r;
+}
+
+__device__ inline ChainOut step_fast(const ChainRec &c, const StepPre &pre, double inflow_up, bool &slow) {
+  const double *v = c.v;
+  ChainOut o;
+  o.glo_lake = 0.;
+  o.res = 0.;
+  o.k_release = 0.;
+  o.gw = pre.gw;
+  double inflow = v[CF_INFLOW];
+  inflow += inflow_up;  // routing.cpp:2623
+  slow = (c.flags & (CFL_LAKE | CFL_RES)) != 0;
+  {
+    // global wetland, routing.cpp:3178-3297, the branch that keeps water
+    const double prev = v[CF_GW_PREV], PETgwr = v[CF_GW_PET], maxStorage = v[CF_GW_MAX];
+    const double totalInflow = inflow + v[CF_GW_WP];
+    const double PETgwrMax = totalInflow + prev;
+    double st = v[CF_GW_S0E] + c.inv_ksw * (totalInflow - PETgwr) * c.ome_sw;
+    double outflow = totalInflow + prev - st - PETgwr;
+    const bool over = st > maxStorage;
+    outflow = over ? outflow + (st - maxStorage) : outflow;
+    st = over ? maxStorage : st;
+    st = (fabs(st) <= kMinStorVol) ? 0. : st;
+    const bool gwet = (c.flags & CFL_GWET) != 0;
+    slow = slow || (gwet && PETgwr > PETgwrMax);
+    o.glo_wetl = gwet ? st : prev;
+    inflow = gwet ? outflow : inflow;
+  }
+  {
+    // river, routing.cpp:3402-3514 (river_step)
+    const double prev = v[CF_RV_PREV];
+    double rin = inflow;
+    const double rin_ldd = (rin + v[CF_LRIR]) + pre.lgwrir;
+    rin = (c.flags & CFL_LDD) ? rin_ldd : rin;
+    rin += v[CF_RV_PREC];
+    const double evapRemUse = 0. + v[CF_RV_EVAP];
+    const double evapRemUseMax = rin + v[CF_RV_TERM];
+    double tv_dry = rin + prev - evapRemUseMax;
+    tv_dry = (tv_dry < 0.) ? 0. : tv_dry;
+    double st = v[CF_RV_S0E] + v[CF_RV_INVK] * (rin - evapRemUse) * v[CF_RV_OME];
+    st = (fabs(st) <= kMinStorVol) ? 0. : st;
+    double tv = rin + prev - st - evapRemUse;
+    tv = (tv < 0.) ? 0. : tv;
+    const bool dry = evapRemUse > evapRemUseMax;
+    o.river = dry ? 0. : st;
+    o.tv = dry ? tv_dry : tv;
+  }
   return o;
 }
 
@@ -646,7 +734,6 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
 
   const int64_t fday = clock->day0 + k;
   const int doy = int((clock->doy0 - 1 + k) % 365) + 1;
-  const int month = month_of_doy(doy);
   const bool started = (clock->started != 0) || (k > 0) || (do_post != 0);
   if (fday < 0 || fday >= n_forcing_days) return;
   const float *f = forcing + fday * 4 * int64_t(n);
@@ -1187,7 +1274,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   }
   if (lk.nu == 0) {
     // no upstream cell: G_riverInflow[n] is still 0 when the cell is routed
-    const ChainOut o = chain_step(c, 0.0, doy, month);
+    const ChainOut o = chain_step(c, 0.0, day_info(doy));
     store_chain(X, mp, c.flags, o);
     if (lk.station >= 0)
       X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + lk.station] = (ldd >= 0) ? o.tv : 0.;
@@ -1213,13 +1300,14 @@ __device__ inline void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Inputs of one chain cell, prefetched a level ahead: the record prefix, the
-// outflow constants and the links (the record suffix of a cell with a global
-// lake or reservoir is read when the cell is routed).
+// Inputs of one chain cell, loaded a level (or two) before it is routed: the
+// whole routing record, the outflow constants and the links.  The record suffix
+// holds data only for cells with a global lake or reservoir; it is loaded for
+// every cell (so that no load depends on the flags) and zeroed for the others.
 struct ChainSlot {
-  double a[CF_PREFIX];
+  double a[CF_COUNT];
   double inv_ksw, ome_sw, inv_kgw, ome_gw, area, cf100;
-  int flags, nu, station;
+  int flags, nu, station, level, nu_level;
   int up[4];  // group-local upstream indices, two int16 per word
 };
 
@@ -1228,7 +1316,7 @@ __device__ inline void prefetch_slot(const WgStatic &S, const WgState &X, int64_
   const WgChainStatic cs = S.cstat[p];
   const double *src = X.crec[base + p].v;
 #pragma unroll
-  for (int f = 0; f < CF_PREFIX; f++) q.a[f] = src[f];
+  for (int f = 0; f < CF_COUNT; f++) q.a[f] = src[f];
   q.inv_ksw = cs.inv_ksw;
   q.ome_sw = cs.ome_sw;
   q.inv_kgw = cs.inv_kgw;
@@ -1238,66 +1326,109 @@ __device__ inline void prefetch_slot(const WgStatic &S, const WgState &X, int64_
   q.flags = lk.flags;
   q.station = lk.station;
   q.nu = lk.nu;
+  q.nu_level = lk.nu_level;
+  q.level = lk.level;
 #pragma unroll
   for (int k = 0; k < 4; k++)
     q.up[k] = int(uint16_t(lk.up[2 * k])) | (int(uint16_t(lk.up[2 * k + 1])) << 16);
 }
 
-// One chain cell: ordered gather of the upstream transported volumes (the serial
-// += of routing.cpp:3955-3958, in routing order), the upstream-dependent routing
-// step, LDS and global stores, the station's daily discharge.
-__device__ __forceinline__ void chain_cell(const WgStatic &S, const WgState &X, double *s_tv, const ChainSlot &q,
-                                           int64_t base, int p, int gs, int m, int doy, int month,
-                                           int daily_slot) {
-  double qq = 0.;
-  if (q.flags & CFL_CONT) {
-    double up = 0.;
+// The routing record of a slot (the suffix of a cell without a global lake or
+// reservoir holds no data; chain_step does not read it for such cells).
+__device__ inline void slot_record(const ChainSlot &q, ChainRec &c) {
 #pragma unroll
-    for (int u = 0; u < 8; u++)
-      if (u < q.nu) up += s_tv[(q.up[u >> 1] >> ((u & 1) * 16)) & 0xffff];
-    const int64_t mp = base + p;
-    ChainRec c;
-#pragma unroll
-    for (int f = 0; f < CF_PREFIX; f++) c.v[f] = q.a[f];
-#pragma unroll
-    for (int f = CF_PREFIX; f < CF_COUNT; f++) c.v[f] = 0.;
-    if (q.flags & (CFL_LAKE | CFL_RES)) load_chain_suffix(X, mp, c);
-    c.inv_ksw = q.inv_ksw;
-    c.ome_sw = q.ome_sw;
-    c.inv_kgw = q.inv_kgw;
-    c.ome_gw = q.ome_gw;
-    c.area = q.area;
-    c.cf100 = q.cf100;
-    c.flags = q.flags;
-    const ChainOut o = chain_step(c, up, doy, month);
-    store_chain(X, mp, q.flags, o);
-    s_tv[p - gs] = o.tv;
-    qq = (q.flags & CFL_LDD) ? o.tv : 0.;
-  }
-  if (q.station >= 0) X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + q.station] = qq;
+  for (int f = 0; f < CF_COUNT; f++) c.v[f] = q.a[f];
+  c.inv_ksw = q.inv_ksw;
+  c.ome_sw = q.ome_sw;
+  c.inv_kgw = q.inv_kgw;
+  c.ome_gw = q.ome_gw;
+  c.area = q.area;
+  c.cf100 = q.cf100;
+  c.flags = q.flags;
 }
 
-// One level of a group: the thread's first cell comes from `use` (loaded a level
-// earlier); the first cell of the next level is loaded into `fill` before any
-// computation, so its latency overlaps this level.  `use` and `fill` alternate
-// between two named slots, never copied, so no wait on the new loads is forced.
+// Stores of the level walk.  Every lane issues the same stores on every level,
+// whether or not it has a cell there and whatever storages the cell has: a store
+// that must not land goes through a raw buffer over the destination row at an
+// offset past the row's end, which the hardware drops.  A store behind a branch
+// would leave different counts of outstanding memory operations on the two sides
+// of the branch; the compiler's wait for the next level's inputs must then assume
+// the smaller count, and waits for the loads just issued for the level after
+// (loads and stores share one in-order counter on this architecture).
+typedef unsigned int WgU32x2 __attribute__((ext_vector_type(2)));
+// s_waitcnt immediate of the gfx9 family for vmcnt(0) (expcnt and lgkmcnt at their
+// maxima, i.e. not waited on)
+constexpr int kWaitVmcnt0 = 0x0F70;
+constexpr uint32_t kDropOffset = 0x80000000u;  // past every row (rows hold < 2^28 doubles)
+
+__device__ inline __amdgpu_buffer_rsrc_t row_buffer(double *row, int n) {
+  // raw buffer (stride 0) of n doubles; the descriptor's last word is the one the
+  // gfx9 family takes for untyped 32-bit dword access
+  return __builtin_amdgcn_make_buffer_rsrc(row, (short)0, n * 8, 0x00020000);
+}
+
+__device__ inline void store_or_drop(double *row, int n, bool keep, int i, double v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(WgU32x2, v), row_buffer(row, n),
+                                        int(keep ? uint32_t(i) * 8u : kDropOffset), 0, 0);
+}
+
+// One chain cell: ordered gather of the upstream transported volumes (the serial
+// += of routing.cpp:3955-3958, in routing order), the upstream-dependent routing
+// step, LDS and global stores, the station's daily discharge.  `act` is false for
+// a lane without a cell on this level (its slot holds some other cell of the
+// group, its results are not stored).
+__device__ __forceinline__ void chain_cell(const WgStatic &S, const WgState &X, double *s_tv, const ChainSlot &q,
+                                           bool act, int64_t base, int p, int gs, int m, const DayInfo &d,
+                                           int daily_slot) {
+  const bool run = act && (q.flags & CFL_CONT);
+  double up = 0.;
+#pragma unroll
+  for (int u = 0; u < 8; u++)
+    if (u < q.nu) up += s_tv[(q.up[u >> 1] >> ((u & 1) * 16)) & 0xffff];
+  ChainRec c;
+  slot_record(q, c);
+  const ChainOut o = chain_step(c, up, d);
+  const int n = S.n;
+  store_or_drop(X.glo_lake + base, n, run && (q.flags & CFL_LAKE), p, o.glo_lake);
+  store_or_drop(X.res + base, n, run && (q.flags & CFL_RES), p, o.res);
+  store_or_drop(X.k_release + base, n, run && (q.flags & CFL_RES), p, o.k_release);
+  store_or_drop(X.glo_wetl + base, n, run && (q.flags & CFL_GWET), p, o.glo_wetl);
+  store_or_drop(X.gw + base, n, run && (q.flags & CFL_ARID), p, o.gw);
+  store_or_drop(X.river + base, n, run, p, o.river);
+  store_or_drop(X.tv + base, n, run, p, o.tv);
+  if (run) s_tv[p - gs] = o.tv;
+  const double qq = (run && (q.flags & CFL_LDD)) ? o.tv : 0.;
+  store_or_drop(X.st_daily + (int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations, S.n_stations,
+                act && q.station >= 0, q.station, qq);
+}
+
+// The first cell of level lf at this thread's position, or the group's first cell
+// when there is none (or lf is past the group): the slot is loaded either way.
+__device__ inline int slot_cell(const int32_t *__restrict__ lo, int lf, int l1, int gs) {
+  const int lc = min(lf, l1 - 1);
+  const int p = lo[lc] + int(threadIdx.x);
+  return (lf < l1 && p < lo[lc + 1]) ? p : gs;
+}
+
+// One level of a group: the thread's first cell comes from `use` (loaded kAhead
+// levels earlier); the first cell of level l + kAhead is loaded into `fill`
+// before any computation, so its latency overlaps the levels in between.  The
+// slots rotate by name, never copied, so no wait on the new loads is forced.
 // Cells beyond the block size (levels wider than the block) load directly.
 template <int kAhead>
 __device__ __forceinline__ void chain_level(const WgStatic &S, const WgState &X, double *s_tv,
                                             const int32_t *__restrict__ lo, int l, int l1, int64_t base, int gs,
-                                            int m, int doy, int month, int daily_slot, const ChainSlot &use,
+                                            int m, const DayInfo &d, int daily_slot, const ChainSlot &use,
                                             ChainSlot &fill) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int beg = lo[l], end = lo[l + 1];
-  if (l + kAhead < l1) {
-    const int p = lo[l + kAhead] + tid;
-    if (p < lo[l + kAhead + 1]) prefetch_slot(S, X, base, p, fill);
-  }
-  if (beg + tid < end) chain_cell(S, X, s_tv, use, base, beg + tid, gs, m, doy, month, daily_slot);
+  prefetch_slot(S, X, base, slot_cell(lo, l + kAhead, l1, gs), fill);
+  const bool act = beg + tid < end;
+  chain_cell(S, X, s_tv, use, act, base, act ? beg + tid : gs, gs, m, d, daily_slot);
   for (int p = beg + tid + nt; p < end; p += nt) {
     ChainSlot q;
     prefetch_slot(S, X, base, p, q);
-    chain_cell(S, X, s_tv, q, base, p, gs, m, doy, month, daily_slot);
+    chain_cell(S, X, s_tv, q, true, base, p, gs, m, d, daily_slot);
   }
   lds_barrier();
 }
@@ -1320,19 +1451,16 @@ __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, Wg
   const int n = S.n;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int doy = int((clock->doy0 - 1 + k) % 365) + 1;
-  const int month = month_of_doy(doy);
+  const DayInfo d = day_info(doy);
   const int32_t *lo = group_level_off;
   const int l0 = group_first_level[g], l1 = group_first_level[g + 1];
   const int gs = lo[l0], ge = lo[l1];
+  if (l1 - l0 < 2 || ge <= gs) return;  // nothing routed in this kernel
   const int lvl0_end = lo[l0 + 1];
   const int64_t base = int64_t(m) * n;
   ChainSlot a, b, c;
-  for (int j = 1; j <= kAhead; j++) {
-    if (l0 + j < l1) {
-      const int p = lo[l0 + j] + tid;
-      if (p < lo[l0 + j + 1]) prefetch_slot(S, X, base, p, j == 1 ? a : b);
-    }
-  }
+  prefetch_slot(S, X, base, slot_cell(lo, l0 + 1, l1, gs), a);
+  if (kAhead == 2) prefetch_slot(S, X, base, slot_cell(lo, l0 + 2, l1, gs), b);
   // level 0 was routed by the cell kernel; cells not routed today read as 0.
   // Eight loads in flight per thread before their LDS stores.
   for (int i0 = gs + tid; i0 < ge; i0 += 8 * nt) {
@@ -1348,18 +1476,148 @@ __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, Wg
       if (i < ge) s_tv[i - gs] = v[j];
     }
   }
+  // the first slots' loads complete here on every path into the level loop, so
+  // that the loop's waits count only what the loop itself issues
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   lds_barrier();
   if (kAhead == 1) {
     for (int l = l0 + 1; l < l1; l += 2) {
-      chain_level<1>(S, X, s_tv, lo, l, l1, base, gs, m, doy, month, daily_slot, a, b);
-      if (l + 1 < l1) chain_level<1>(S, X, s_tv, lo, l + 1, l1, base, gs, m, doy, month, daily_slot, b, a);
+      chain_level<1>(S, X, s_tv, lo, l, l1, base, gs, m, d, daily_slot, a, b);
+      if (l + 1 < l1) chain_level<1>(S, X, s_tv, lo, l + 1, l1, base, gs, m, d, daily_slot, b, a);
     }
   } else {
     for (int l = l0 + 1; l < l1; l += 3) {
-      chain_level<2>(S, X, s_tv, lo, l, l1, base, gs, m, doy, month, daily_slot, a, c);
-      if (l + 1 < l1) chain_level<2>(S, X, s_tv, lo, l + 1, l1, base, gs, m, doy, month, daily_slot, b, a);
-      if (l + 2 < l1) chain_level<2>(S, X, s_tv, lo, l + 2, l1, base, gs, m, doy, month, daily_slot, c, b);
+      chain_level<2>(S, X, s_tv, lo, l, l1, base, gs, m, d, daily_slot, a, c);
+      if (l + 1 < l1) chain_level<2>(S, X, s_tv, lo, l + 1, l1, base, gs, m, d, daily_slot, b, a);
+      if (l + 2 < l1) chain_level<2>(S, X, s_tv, lo, l + 2, l1, base, gs, m, d, daily_slot, c, b);
     }
+  }
+  (void)T;
+}
+
+// Wave walk of a routing group: one wavefront routes levels 1..depth of the group
+// in batches of 64 cells taken in device order (by level), so that a batch covers
+// a stretch of one wide level or several narrow ones.  Each lane holds the inputs
+// of one cell of a batch, loaded kWaveAhead batches before the batch is routed.
+// Within a batch the levels run in order, each on the lanes whose cells lie on
+// it; transported volumes pass between lanes through LDS, which serves the LDS
+// operations of one wavefront in issue order, so no barrier is needed.  A batch's
+// results are stored when the batch is done.  The batch count is padded to a
+// multiple of the unroll (padding batches route nothing), so that every batch
+// issues the same loads and stores and no wait covers loads issued for later
+// batches.
+constexpr int kWaveAhead = 1;
+constexpr int kWaveUnroll = kWaveAhead + 1;
+
+__device__ inline void wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront", "local");
+}
+
+__device__ __forceinline__ void wave_batch(const WgStatic &S, const WgState &X, double *s_tv, int b, int p_first,
+                                           int ge, int gs, int64_t base, int m, const DayInfo &d, int daily_slot,
+                                           const ChainSlot &q, ChainSlot &fill) {
+  const int lane = threadIdx.x;
+  {
+    const int pf = p_first + (b + kWaveAhead) * 64 + lane;
+    prefetch_slot(S, X, base, pf < ge ? pf : gs, fill);
+  }
+  const int b0 = p_first + b * 64;
+  const int p = b0 + lane;
+  const bool have = p < ge;
+  ChainOut o = {0., 0., 0., 0., 0., 0., 0.};
+  bool run = false;
+  if (b0 < ge) {
+    const int last = min(63, ge - 1 - b0);
+    const int lv_a = __builtin_amdgcn_readfirstlane(q.level);
+    const int lv_b = __builtin_amdgcn_readlane(q.level, last);
+    ChainRec c;
+    slot_record(q, c);
+    const StepPre pre = step_pre(c);
+    for (int l = lv_a; l <= lv_b; l++) {
+      if (have && q.level == l) {
+        // upstream transported volumes in routing order (routing.cpp:3955-3958);
+        // entries past the cell's own count read entry 0 and add +0, which leaves
+        // the non-negative sum unchanged.  Past the level's largest count (most
+        // levels have at most two upstream cells) nothing is added.
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) t[u] = s_tv[(q.up[u >> 1] >> ((u & 1) * 16)) & 0xffff];
+        double up = 0.;
+        up += t[0];
+        up += (q.nu > 1) ? t[1] : 0.;
+        if (__builtin_amdgcn_readfirstlane(q.nu_level) > 2) {
+#pragma unroll
+          for (int u = 2; u < 8; u++) up += (u < q.nu) ? t[u] : 0.;
+        }
+        bool slow;
+        o = step_fast(c, pre, up, slow);
+        if (slow) o = chain_step(c, up, d);
+        run = (q.flags & CFL_CONT) != 0;
+        s_tv[p - gs] = run ? o.tv : 0.;
+      }
+      wave_lds_order();
+    }
+  }
+  const int n = S.n;
+  store_or_drop(X.glo_lake + base, n, run && (q.flags & CFL_LAKE), p, o.glo_lake);
+  store_or_drop(X.res + base, n, run && (q.flags & CFL_RES), p, o.res);
+  store_or_drop(X.k_release + base, n, run && (q.flags & CFL_RES), p, o.k_release);
+  store_or_drop(X.glo_wetl + base, n, run && (q.flags & CFL_GWET), p, o.glo_wetl);
+  store_or_drop(X.gw + base, n, run && (q.flags & CFL_ARID), p, o.gw);
+  store_or_drop(X.river + base, n, run, p, o.river);
+  store_or_drop(X.tv + base, n, run, p, o.tv);
+  const double qq = (run && (q.flags & CFL_LDD)) ? o.tv : 0.;
+  store_or_drop(X.st_daily + (int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations, S.n_stations,
+                have && q.station >= 0, q.station, qq);
+}
+
+__global__ __launch_bounds__(64) void wg_chain_wave_kernel(WgStatic S, WgState X, WgTables T,
+                                                           const int32_t *__restrict__ group_level_off,
+                                                           const int32_t *__restrict__ group_first_level,
+                                                           const WgClock *__restrict__ clock, int k,
+                                                           int daily_slot) {
+  extern __shared__ double s_tv[];
+  const int g = S.block_group[blockIdx.x];
+  if (g < 0) return;
+  const int m = blockIdx.y;
+  const int n = S.n;
+  const int lane = threadIdx.x;
+  const int doy = int((clock->doy0 - 1 + k) % 365) + 1;
+  const DayInfo d = day_info(doy);
+  const int32_t *lo = group_level_off;
+  const int l0 = group_first_level[g], l1 = group_first_level[g + 1];
+  const int gs = lo[l0], ge = lo[l1];
+  if (l1 - l0 < 2 || ge <= gs) return;  // nothing routed in this kernel
+  const int lvl0_end = lo[l0 + 1];
+  const int64_t base = int64_t(m) * n;
+  const int nb = (ge - lvl0_end + 64 * kWaveUnroll - 1) / (64 * kWaveUnroll) * kWaveUnroll;
+  ChainSlot s0, s1;
+  static_assert(kWaveAhead == 1, "two slots in rotation");
+  {
+    const int pa = lvl0_end + lane;
+    prefetch_slot(S, X, base, pa < ge ? pa : gs, s0);
+  }
+  // level 0 was routed by the cell kernel: its transported volumes, sixteen loads
+  // in flight per lane
+  for (int i0 = gs + lane; i0 < lvl0_end; i0 += 16 * 64) {
+    double v[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const int i = i0 + j * 64;
+      v[j] = (i < lvl0_end) ? X.tv[base + i] : 0.;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const int i = i0 + j * 64;
+      if (i < lvl0_end) s_tv[i - gs] = v[j];
+    }
+  }
+  // the first slots' loads complete here on every path into the batch loop
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+  wave_lds_order();
+  for (int b = 0; b < nb; b += kWaveUnroll) {
+    wave_batch(S, X, s_tv, b, lvl0_end, ge, gs, base, m, d, daily_slot, s0, s1);
+    wave_batch(S, X, s_tv, b + 1, lvl0_end, ge, gs, base, m, d, daily_slot, s1, s0);
   }
   (void)T;
 }
@@ -1399,7 +1657,7 @@ __global__ __launch_bounds__(256) void wg_chain_flow_kernel(WgStatic S, WgState 
   const int n = S.n;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int doy = int((clock->doy0 - 1 + k) % 365) + 1;
-  const int month = month_of_doy(doy);
+  const DayInfo d = day_info(doy);
   const int l0 = group_first_level[g], l1 = group_first_level[g + 1];
   const int depth = l1 - l0;
   const int gs = group_level_off[l0], ge = group_level_off[l1];
@@ -1451,19 +1709,8 @@ __global__ __launch_bounds__(256) void wg_chain_flow_kernel(WgStatic S, WgState 
         if (u < cur.nu) up += __longlong_as_double(vtv[(cur.up[u >> 1] >> ((u & 1) * 16)) & 0xffff]);
       const int64_t mp = base + p;
       ChainRec c;
-#pragma unroll
-      for (int f = 0; f < CF_PREFIX; f++) c.v[f] = cur.a[f];
-#pragma unroll
-      for (int f = CF_PREFIX; f < CF_COUNT; f++) c.v[f] = 0.;
-      if (cur.flags & (CFL_LAKE | CFL_RES)) load_chain_suffix(X, mp, c);
-      c.inv_ksw = cur.inv_ksw;
-      c.ome_sw = cur.ome_sw;
-      c.inv_kgw = cur.inv_kgw;
-      c.ome_gw = cur.ome_gw;
-      c.area = cur.area;
-      c.cf100 = cur.cf100;
-      c.flags = cur.flags;
-      const ChainOut o = chain_step(c, up, doy, month);
+      slot_record(cur, c);
+      const ChainOut o = chain_step(c, up, d);
       vtv[p - gs] = __double_as_longlong(o.tv);
       store_chain(X, mp, cur.flags, o);
       qq = (cur.flags & CFL_LDD) ? o.tv : 0.;
