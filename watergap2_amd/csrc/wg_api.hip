@@ -50,6 +50,11 @@ struct wg_handle {
   std::vector<int32_t> group_first_level;
   int32_t n_groups = 0, max_depth = 0;
   int32_t *d_level_off = nullptr, *d_group_first = nullptr;
+  // wave walk: the number of wide levels (more than 64 cells) leading each group
+  // above level 0, and their batches dealt to four wavefronts: the batches of
+  // wavefront w of group g are d_wave_items[d_wave_item_off[4g + w] .. [4g + w + 1])
+  int32_t *d_wave_wide = nullptr, *d_wave_item_off = nullptr;
+  WgWaveItem *d_wave_items = nullptr;
   // host copies needed at the boundary
   std::vector<double> snow_ch0;   // [M][N] channel 0 of G_SnowInElevation (never read by calcNewDay)
   WgStatic S{};
@@ -703,6 +708,36 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
       lk.level = rel_level[p];
       for (int u = 0; u < 8; u++) lk.up[u] = int16_t(u < lk.nu ? idx[off[p] + u] : 0);
     }
+    // wave walk: the leading wide levels (more than 64 cells) of each group above
+    // level 0, in level-aligned batches of 64 cells dealt round-robin to the four
+    // wavefronts of the group's workgroup
+    {
+      std::vector<int32_t> wide(size_t(H->n_groups), 0), item_off(size_t(H->n_groups) * 4 + 1, 0);
+      std::vector<WgWaveItem> items;
+      for (int32_t gg = 0; gg < H->n_groups; gg++) {
+        const int32_t l0 = H->group_first_level[gg], l1 = H->group_first_level[gg + 1];
+        int32_t la = 0;
+        while (l0 + 1 + la < l1 && H->level_off[l0 + 2 + la] - H->level_off[l0 + 1 + la] > 64) la++;
+        wide[gg] = la;
+        for (int w = 0; w < 4; w++) {
+          item_off[size_t(gg) * 4 + w] = int32_t(items.size());
+          for (int32_t L = 1; L <= la; L++) {
+            const int32_t a = H->level_off[l0 + L], e = H->level_off[l0 + L + 1];
+            int32_t j = 0;
+            for (int32_t st0 = a; st0 < e; st0 += 64, j++)
+              if (j % 4 == w) items.push_back(WgWaveItem{st0, std::min(st0 + 64, e), L, 0});
+          }
+        }
+      }
+      item_off[size_t(H->n_groups) * 4] = int32_t(items.size());
+      if (items.empty()) items.push_back(WgWaveItem{0, 0, 0, 0});
+      r_ = upload(H, &H->d_wave_wide, wide);
+      if (r_) { *out = h.release(); return r_; }
+      r_ = upload(H, &H->d_wave_item_off, item_off);
+      if (r_) { *out = h.release(); return r_; }
+      r_ = upload(H, &H->d_wave_items, items);
+      if (r_) { *out = h.release(); return r_; }
+    }
     // the largest upstream count of each level
     for (int32_t l = 0; l + 1 < int32_t(H->level_off.size()); l++) {
       int16_t mx = 0;
@@ -1042,8 +1077,9 @@ int enqueue_block(wg_handle *h, int days, hipEvent_t *evs) {
     HIP_CHECK(h, hipGetLastError());
     if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], h->stream));
     if (h->chain_mode == 3)
-      hipLaunchKernelGGL(wg_chain_wave_kernel, chgrid, dim3(64), size_t(h->max_group_cells) * sizeof(double),
-                         h->stream, h->S, h->X, h->T, h->d_level_off, h->d_group_first, h->d_clock, k, slot);
+      hipLaunchKernelGGL(wg_chain_wave_kernel, chgrid, dim3(256), size_t(h->max_group_cells) * sizeof(double),
+                         h->stream, h->S, h->X, h->T, h->d_level_off, h->d_group_first, h->d_wave_wide,
+                         h->d_wave_item_off, h->d_wave_items, h->d_clock, k, slot);
     else if (h->chain_mode == 1)
       hipLaunchKernelGGL(wg_chain_flow_kernel, chgrid, dim3(h->block_threads_chain),
                          size_t(h->max_group_cells) * sizeof(double), h->stream, h->S, h->X, h->T,

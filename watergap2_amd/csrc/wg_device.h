@@ -29,6 +29,12 @@ struct alignas(16) WgChainLink {
   int16_t up[8];
 };
 
+// A batch of the wave walk's wide levels: cells [start, end) of one level (counted
+// from the group's first level), up to 64.
+struct alignas(16) WgWaveItem {
+  int32_t start, end, level, pad;
+};
+
 // Outflow constants of the storages on the routing chain (global lake / wetland,
 // arid groundwater).
 struct alignas(16) WgChainStatic {

@@ -1495,39 +1495,40 @@ __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, Wg
   (void)T;
 }
 
-// Wave walk of a routing group: one wavefront routes levels 1..depth of the group
-// in batches of 64 cells taken in device order (by level), so that a batch covers
-// a stretch of one wide level or several narrow ones.  Each lane holds the inputs
-// of one cell of a batch, loaded kWaveAhead batches before the batch is routed.
-// Within a batch the levels run in order, each on the lanes whose cells lie on
-// it; transported volumes pass between lanes through LDS, which serves the LDS
-// operations of one wavefront in issue order, so no barrier is needed.  A batch's
-// results are stored when the batch is done.  The batch count is padded to a
-// multiple of the unroll (padding batches route nothing), so that every batch
-// issues the same loads and stores and no wait covers loads issued for later
-// batches.
-constexpr int kWaveAhead = 1;
-constexpr int kWaveUnroll = kWaveAhead + 1;
-
+// Wave walk of a routing group.  The levels above level 0 are routed in batches of
+// up to 64 cells, one cell per lane, each lane's inputs loaded one batch before
+// the batch is routed; transported volumes pass between lanes through LDS.
+//  - The leading wide levels (more than 64 cells) are cut into level-aligned
+//    batches dealt round-robin to the workgroup's four wavefronts, with an
+//    LDS-only barrier after each level.
+//  - The narrow rest is routed by wavefront 0 alone in batches of 64 cells taken
+//    in device order (by level), so that a batch covers several narrow levels;
+//    within a batch the levels run in order, each on the lanes whose cells lie
+//    on it.  LDS serves the LDS operations of one wavefront in issue order, so
+//    no barrier is needed there.
+// A batch's results are stored when the batch is done.  Every batch issues the
+// same loads and stores (padding batches route nothing), so that no wait covers
+// the loads issued for the next batch.
 __device__ inline void wave_lds_order() {
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront", "local");
 }
 
-__device__ __forceinline__ void wave_batch(const WgStatic &S, const WgState &X, double *s_tv, int b, int p_first,
-                                           int ge, int gs, int64_t base, int m, const DayInfo &d, int daily_slot,
-                                           const ChainSlot &q, ChainSlot &fill) {
-  const int lane = threadIdx.x;
+// One batch: cells [b0, bend) (b0 >= bend: a padding batch); `q` holds the
+// batch's inputs, `fill` receives those of the batch starting at pnext.
+__device__ __forceinline__ void wave_batch(const WgStatic &S, const WgState &X, double *s_tv, int b0, int bend,
+                                           int pnext, int ge, int gs, int64_t base, int m, const DayInfo &d,
+                                           int daily_slot, const ChainSlot &q, ChainSlot &fill) {
+  const int lane = threadIdx.x & 63;
   {
-    const int pf = p_first + (b + kWaveAhead) * 64 + lane;
+    const int pf = pnext + lane;
     prefetch_slot(S, X, base, pf < ge ? pf : gs, fill);
   }
-  const int b0 = p_first + b * 64;
   const int p = b0 + lane;
-  const bool have = p < ge;
+  const bool have = p < bend;
   ChainOut o = {0., 0., 0., 0., 0., 0., 0.};
   bool run = false;
-  if (b0 < ge) {
-    const int last = min(63, ge - 1 - b0);
+  if (b0 < bend) {
+    const int last = min(63, bend - 1 - b0);
     const int lv_a = __builtin_amdgcn_readfirstlane(q.level);
     const int lv_b = __builtin_amdgcn_readlane(q.level, last);
     ChainRec c;
@@ -1571,17 +1572,21 @@ __device__ __forceinline__ void wave_batch(const WgStatic &S, const WgState &X, 
                 have && q.station >= 0, q.station, qq);
 }
 
-__global__ __launch_bounds__(64) void wg_chain_wave_kernel(WgStatic S, WgState X, WgTables T,
-                                                           const int32_t *__restrict__ group_level_off,
-                                                           const int32_t *__restrict__ group_first_level,
-                                                           const WgClock *__restrict__ clock, int k,
-                                                           int daily_slot) {
+__global__ __launch_bounds__(256) void wg_chain_wave_kernel(WgStatic S, WgState X, WgTables T,
+                                                            const int32_t *__restrict__ group_level_off,
+                                                            const int32_t *__restrict__ group_first_level,
+                                                            const int32_t *__restrict__ wave_wide,
+                                                            const int32_t *__restrict__ wave_item_off,
+                                                            const WgWaveItem *__restrict__ wave_items,
+                                                            const WgClock *__restrict__ clock, int k,
+                                                            int daily_slot) {
   extern __shared__ double s_tv[];
   const int g = S.block_group[blockIdx.x];
   if (g < 0) return;
   const int m = blockIdx.y;
   const int n = S.n;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // the wavefront, uniform
   const int doy = int((clock->doy0 - 1 + k) % 365) + 1;
   const DayInfo d = day_info(doy);
   const int32_t *lo = group_level_off;
@@ -1590,34 +1595,73 @@ __global__ __launch_bounds__(64) void wg_chain_wave_kernel(WgStatic S, WgState X
   if (l1 - l0 < 2 || ge <= gs) return;  // nothing routed in this kernel
   const int lvl0_end = lo[l0 + 1];
   const int64_t base = int64_t(m) * n;
-  const int nb = (ge - lvl0_end + 64 * kWaveUnroll - 1) / (64 * kWaveUnroll) * kWaveUnroll;
-  ChainSlot s0, s1;
-  static_assert(kWaveAhead == 1, "two slots in rotation");
-  {
-    const int pa = lvl0_end + lane;
-    prefetch_slot(S, X, base, pa < ge ? pa : gs, s0);
-  }
-  // level 0 was routed by the cell kernel: its transported volumes, sixteen loads
-  // in flight per lane
-  for (int i0 = gs + lane; i0 < lvl0_end; i0 += 16 * 64) {
-    double v[16];
+  const int la = wave_wide[g];  // wide levels l0 + 1 .. l0 + la
+  const int pb = lo[l0 + 1 + la];  // first cell of the narrow rest
+  // level 0 was routed by the cell kernel: its transported volumes, eight loads
+  // in flight per thread
+  for (int i0 = gs + tid; i0 < lvl0_end; i0 += 8 * 256) {
+    double v[8];
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-      const int i = i0 + j * 64;
+    for (int j = 0; j < 8; j++) {
+      const int i = i0 + j * 256;
       v[j] = (i < lvl0_end) ? X.tv[base + i] : 0.;
     }
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-      const int i = i0 + j * 64;
+    for (int j = 0; j < 8; j++) {
+      const int i = i0 + j * 256;
       if (i < lvl0_end) s_tv[i - gs] = v[j];
     }
   }
-  // the first slots' loads complete here on every path into the batch loop
   __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-  wave_lds_order();
-  for (int b = 0; b < nb; b += kWaveUnroll) {
-    wave_batch(S, X, s_tv, b, lvl0_end, ge, gs, base, m, d, daily_slot, s0, s1);
-    wave_batch(S, X, s_tv, b + 1, lvl0_end, ge, gs, base, m, d, daily_slot, s1, s0);
+  lds_barrier();
+  ChainSlot s0, s1;
+  if (la > 0) {
+    // wide levels: this wavefront's batches in level order; a padding batch
+    // (past the list) routes nothing and passes no barrier.  Every wavefront
+    // passes exactly la barriers, one after each wide level.
+    const int it0 = wave_item_off[g * 4 + w], it1 = wave_item_off[g * 4 + w + 1];
+    auto item = [&](int i) {
+      return (i < it1) ? wave_items[i] : WgWaveItem{ge, ge, 0, 0};
+    };
+    WgWaveItem ia = item(it0);
+    {
+      const int pf = ia.start + lane;
+      prefetch_slot(S, X, base, pf < ge ? pf : gs, s0);
+    }
+    // the first slot's loads complete here on every path into the loop
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+    int lev = 1;
+    for (int i = it0; i < it1; i += 2) {
+      const WgWaveItem ib = item(i + 1), ic = item(i + 2);
+      while (lev < ia.level) {
+        lds_barrier();
+        lev++;
+      }
+      wave_batch(S, X, s_tv, ia.start, ia.end, ib.start, ge, gs, base, m, d, daily_slot, s0, s1);
+      while (lev < ib.level) {
+        lds_barrier();
+        lev++;
+      }
+      wave_batch(S, X, s_tv, ib.start, ib.end, ic.start, ge, gs, base, m, d, daily_slot, s1, s0);
+      ia = ic;
+    }
+    while (lev <= la) {
+      lds_barrier();
+      lev++;
+    }
+  }
+  if (w != 0 || pb >= ge) return;
+  // the narrow rest, wavefront 0 alone
+  const int nb = (ge - pb + 127) / 128 * 2;
+  {
+    const int pa = pb + lane;
+    prefetch_slot(S, X, base, pa < ge ? pa : gs, s0);
+  }
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+  for (int b = 0; b < nb; b += 2) {
+    const int b0 = pb + b * 64;
+    wave_batch(S, X, s_tv, b0, ge, b0 + 64, ge, gs, base, m, d, daily_slot, s0, s1);
+    wave_batch(S, X, s_tv, b0 + 64, ge, b0 + 128, ge, gs, base, m, d, daily_slot, s1, s0);
   }
   (void)T;
 }
