@@ -136,6 +136,20 @@ int upload(wg_handle *h, T **out, const std::vector<T> &host) {
   return WG_OK;
 }
 
+// Routing links in the device layout ([2][N] x 16 bytes).
+std::vector<WgI4> pack_links(const std::vector<WgChainLink> &lk) {
+  const size_t n = lk.size();
+  std::vector<WgI4> w(2 * n);
+  auto pair16 = [](int16_t lo, int16_t hi) { return int32_t(uint32_t(uint16_t(lo)) | (uint32_t(uint16_t(hi)) << 16)); };
+  for (size_t p = 0; p < n; p++) {
+    const WgChainLink &l = lk[p];
+    w[p] = WgI4{l.flags, l.station, pair16(l.nu, l.nu_level), l.level};
+    w[n + p] = WgI4{pair16(l.up[0], l.up[1]), pair16(l.up[2], l.up[3]), pair16(l.up[4], l.up[5]),
+                    pair16(l.up[6], l.up[7])};
+  }
+  return w;
+}
+
 // host array in reference order -> device order
 template <class T>
 std::vector<T> permute(const wg_handle *h, const T *ref) {
@@ -744,12 +758,19 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
       for (int32_t q = H->level_off[l]; q < H->level_off[l + 1]; q++) mx = std::max(mx, H->clink[q].nu);
       for (int32_t q = H->level_off[l]; q < H->level_off[l + 1]; q++) H->clink[q].nu_level = mx;
     }
-    WgChainLink *dlk;
-    r_ = upload(H, &dlk, H->clink);
+    WgI4 *dlk;
+    r_ = upload(H, &dlk, pack_links(H->clink));
     if (r_) { *out = h.release(); return r_; }
     S.clink = dlk;
-    WgChainStatic *dcs;
-    r_ = upload(H, &dcs, cst);
+    std::vector<WgD2> cst_p(size_t(3) * n);
+    for (int32_t p = 0; p < n; p++) {
+      const WgChainStatic &c = cst[p];
+      cst_p[p] = WgD2{c.inv_ksw, c.ome_sw};
+      cst_p[size_t(n) + p] = WgD2{c.inv_kgw, c.ome_gw};
+      cst_p[size_t(2) * n + p] = WgD2{c.area, c.cf100};
+    }
+    WgD2 *dcs;
+    r_ = upload(H, &dcs, cst_p);
     if (r_) { *out = h.release(); return r_; }
     S.cstat = dcs;
     // tile tables sized for the smallest cell block the tuner tries (64 threads)
@@ -790,7 +811,7 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     rc = dev_alloc(H, p, NM);
     if (rc) { *out = h.release(); return rc; }
   }
-  rc = dev_alloc(H, &X.crec, NM);
+  rc = dev_alloc(H, &X.crec, NM * (WG_DEV_NREC / 2));
   if (!rc) rc = dev_alloc(H, &X.lai_days, NM);
   if (!rc) rc = dev_alloc(H, &X.lai_status, NM);
   if (!rc) rc = dev_alloc(H, &X.smax, NM);
@@ -1029,8 +1050,9 @@ extern "C" int wg_set_stations(wg_handle *h, int32_t n_st, const int32_t *cells)
   HIP_CHECK(h, hipMemcpy(const_cast<int32_t *>(h->S.station), slot.data(), h->n * sizeof(int32_t),
                          hipMemcpyHostToDevice));
   for (int32_t p = 0; p < h->n; p++) h->clink[p].station = slot[p];
-  HIP_CHECK(h, hipMemcpy(const_cast<WgChainLink *>(h->S.clink), h->clink.data(),
-                         h->clink.size() * sizeof(WgChainLink), hipMemcpyHostToDevice));
+  const std::vector<WgI4> packed = pack_links(h->clink);
+  HIP_CHECK(h, hipMemcpy(const_cast<WgI4 *>(h->S.clink), packed.data(), packed.size() * sizeof(WgI4),
+                         hipMemcpyHostToDevice));
   // reallocate the accumulators
   for (auto it = h->bufs.begin(); it != h->bufs.end();) {
     if (it->p == h->X.st_sum || it->p == h->X.st_daily) {

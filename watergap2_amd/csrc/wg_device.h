@@ -44,9 +44,17 @@ struct alignas(16) WgChainStatic {
 // Routing record of a cell with upstream inflow, written by the cell kernel: a
 // prefix read by every routed cell and a suffix written and read only for cells
 // with a global lake or a reservoir (field meanings: ChainField in wg_kernels.hip).
+// Stored as pairs of fields, [M][WG_DEV_NREC / 2][N] x 16 bytes, so that the lanes
+// of a wavefront, which hold consecutive cells, read and write whole cache lines
+// with 16-byte accesses.
 #define WG_DEV_NREC 36
-struct alignas(16) WgChainRec {
-  double v[WG_DEV_NREC];
+
+// Device layout of the routing links, [2][N] x 16 bytes: (flags, station,
+// nu | nu_level << 16, level), then four words of two upstream indices (up[2k] in
+// the low half); of the chain constants, [3][N] x 16 bytes in the order of
+// WgChainStatic.
+struct alignas(16) WgI4 {
+  int32_t x, y, z, w;
 };
 
 // Land-cover tables indexed by class-1 (LCT_22.DAT, LAI_22.DAT).
@@ -80,8 +88,8 @@ struct WgStatic {
   const double *k_sw, *e_sw, *inv_ksw, *ome_sw, *k_gw, *e_gw, *inv_kgw, *ome_gw;
   const double *cf100;              // contfreq / 100
   const int8_t *cflags;             // lake / reservoir / global wetland / arid / LDD >= 0 / continental
-  const WgChainLink *clink;         // [N] routing links
-  const WgChainStatic *cstat;       // [N] outflow constants of chain storages
+  const WgI4 *clink;                // [2][N] routing links
+  const WgD2 *cstat;                // [3][N] outflow constants of chain storages
   // XCD-aware placement: the cell kernel's block b covers cells [tile_start[b],
   // tile_end[b]) (tile_start -1: idle block); the chain kernel's block i routes
   // group block_group[i] (-1: idle).  Block i runs on XCD i % 8.
@@ -105,7 +113,7 @@ struct WgState {
   float *smax;
   double *tv;                   // transported volume of the day
   // routing record written by the cell kernel for cells with upstream inflow
-  WgChainRec *crec;             // [M][N]
+  WgD2 *crec;                   // [M][WG_DEV_NREC / 2][N] routing records
   // diagnostics of the last day
   double *d_surf, *d_recharge, *d_ow_pet, *d_aet, *d_prec;
   // stations

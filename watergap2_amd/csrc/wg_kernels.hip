@@ -121,6 +121,7 @@ enum ChainField : int {
   CF_COUNT
 };
 static_assert(CF_COUNT == WG_DEV_NREC, "routing record size");
+static_assert(CF_COUNT % 2 == 0, "routing record stored in pairs");
 
 // chain flags (static per cell); bits 8-15 reservoir type, 16-23 start month
 enum {
@@ -128,6 +129,35 @@ enum {
 };
 __device__ inline int flag_res_type(int flags) { return (flags >> 8) & 0xff; }
 __device__ inline int flag_start_month(int flags) { return (flags >> 16) & 0xff; }
+
+// Device accessors of the routing links and chain constants ([k][N] x 16 bytes).
+__device__ inline WgChainLink load_link(const WgStatic &S, int p) {
+  const WgI4 a = S.clink[p], b = S.clink[int64_t(S.n) + p];
+  WgChainLink lk;
+  lk.flags = a.x;
+  lk.station = a.y;
+  lk.nu = int16_t(a.z & 0xffff);
+  lk.nu_level = int16_t(a.z >> 16);
+  lk.level = a.w;
+  const int32_t u[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    lk.up[2 * k] = int16_t(uint32_t(u[k]) & 0xffff);
+    lk.up[2 * k + 1] = int16_t(uint32_t(u[k]) >> 16);
+  }
+  return lk;
+}
+
+__device__ inline WgChainStatic load_cstat(const WgStatic &S, int p) {
+  const int64_t n = S.n;
+  const WgD2 a = S.cstat[p], b = S.cstat[n + p], c = S.cstat[2 * n + p];
+  return WgChainStatic{a.x, a.y, b.x, b.y, c.x, c.y};
+}
+
+// Pair 0 of the routing record of (member m, cell p); pair k is at [k * N].
+__device__ inline WgD2 *rec_at(const WgState &X, int n, int m, int p) {
+  return X.crec + int64_t(m) * (WG_DEV_NREC / 2) * n + p;
+}
 
 struct ChainRec {
   double v[CF_COUNT];
@@ -1172,12 +1202,12 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
 
   // record of the routing terms that do not depend on the inflow from upstream
   // (routing.cpp:2630-3514)
-  const WgChainLink lk = S.clink[p];
+  const WgChainLink lk = load_link(S, p);
   ChainRec c;
   double *v = c.v;
   const double e_sw = S.e_sw[p];
   {
-    const WgChainStatic cs = S.cstat[p];
+    const WgChainStatic cs = load_cstat(S, p);
     c.inv_ksw = cs.inv_ksw;
     c.ome_sw = cs.ome_sw;
     c.inv_kgw = cs.inv_kgw;
@@ -1280,12 +1310,15 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
       X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + lk.station] = (ldd >= 0) ? o.tv : 0.;
   } else {
     // the prefix for every routed cell, the suffix for cells with a global lake or reservoir
-    double *dst = X.crec[mp].v;
+    // the pairs holding the prefix for every routed cell (the last of them also
+    // holds the first suffix field), the rest for cells with a lake or reservoir
+    WgD2 *dst = rec_at(X, n, m, p);
+    constexpr int kPrefixPairs = (CF_PREFIX + 1) / 2;
 #pragma unroll
-    for (int f = 0; f < CF_PREFIX; f++) dst[f] = v[f];
+    for (int k = 0; k < kPrefixPairs; k++) dst[int64_t(k) * n] = WgD2{v[2 * k], v[2 * k + 1]};
     if (lk.flags & (CFL_LAKE | CFL_RES)) {
 #pragma unroll
-      for (int f = CF_PREFIX; f < CF_COUNT; f++) dst[f] = v[f];
+      for (int k = kPrefixPairs; k < CF_COUNT / 2; k++) dst[int64_t(k) * n] = WgD2{v[2 * k], v[2 * k + 1]};
     }
   }
 }
@@ -1312,25 +1345,31 @@ struct ChainSlot {
 };
 
 __device__ inline void prefetch_slot(const WgStatic &S, const WgState &X, int64_t base, int p, ChainSlot &q) {
-  const WgChainLink lk = S.clink[p];
-  const WgChainStatic cs = S.cstat[p];
-  const double *src = X.crec[base + p].v;
+  const int64_t n = S.n;
+  const WgChainStatic cs = load_cstat(S, p);
+  const WgD2 *src = X.crec + base * (WG_DEV_NREC / 2) + p;  // rec_at(X, n, m, p) for base = m * n
 #pragma unroll
-  for (int f = 0; f < CF_COUNT; f++) q.a[f] = src[f];
+  for (int k = 0; k < CF_COUNT / 2; k++) {
+    const WgD2 v = src[k * n];
+    q.a[2 * k] = v.x;
+    q.a[2 * k + 1] = v.y;
+  }
   q.inv_ksw = cs.inv_ksw;
   q.ome_sw = cs.ome_sw;
   q.inv_kgw = cs.inv_kgw;
   q.ome_gw = cs.ome_gw;
   q.area = cs.area;
   q.cf100 = cs.cf100;
-  q.flags = lk.flags;
-  q.station = lk.station;
-  q.nu = lk.nu;
-  q.nu_level = lk.nu_level;
-  q.level = lk.level;
-#pragma unroll
-  for (int k = 0; k < 4; k++)
-    q.up[k] = int(uint16_t(lk.up[2 * k])) | (int(uint16_t(lk.up[2 * k + 1])) << 16);
+  const WgI4 a = S.clink[p], b = S.clink[n + p];
+  q.flags = a.x;
+  q.station = a.y;
+  q.nu = a.z & 0xffff;
+  q.nu_level = a.z >> 16;
+  q.level = a.w;
+  q.up[0] = b.x;
+  q.up[1] = b.y;
+  q.up[2] = b.z;
+  q.up[3] = b.w;
 }
 
 // The routing record of a slot (the suffix of a cell without a global lake or
@@ -1714,7 +1753,7 @@ __global__ __launch_bounds__(256) void wg_chain_flow_kernel(WgStatic S, WgState 
     if (i < lvl0_end)
       v = X.tv[base + i];  // level 0 was routed by the cell kernel
     else
-      v = (S.clink[i].flags & CFL_CONT) ? __longlong_as_double(kTvPending) : 0.;
+      v = (S.clink[i].x & CFL_CONT) ? __longlong_as_double(kTvPending) : 0.;
     s_tv[i - gs] = v;
   }
   __syncthreads();
