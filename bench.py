@@ -361,7 +361,7 @@ def main():
 
     # launch configuration chosen by the library's autotuner (state restored after)
     if args.no_autotune:
-        tuned, tuned_ms = (256, 64, 5, 3), 0.0  # the library default
+        tuned, tuned_ms = (256, 512, 5, 3), 0.0  # the library default at one member
     else:
         tuned, tuned_ms = model.autotune(n_days=5)
     print(f"# autotune: cell tile {tuned[0]}, chain workgroup {tuned[1]}, snow chunk {tuned[2]}, "

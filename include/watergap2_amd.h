@@ -258,8 +258,9 @@ int wg_step_days_profiled(wg_handle *h, int32_t forcing_day0, int32_t first_doy,
 int wg_last_timing(wg_handle *h, double *ms, int64_t *n_launches);
 
 /* Autotuner, no counterpart in the reference: times the launch configurations
- * (cell-kernel tile 128 or 256 cells, chain workgroup 64, 128 or 256 threads,
- * snow chunk 5 or 10 sub-cell pairs, chain kernel mode 0, 1, 2 or 3)
+ * (cell-kernel tile 128 or 256 cells, chain kernel mode 0, 1 or 2 with a
+ * workgroup of 64, 128 or 256 threads or mode 3 with 256 or 512, snow chunk 5 or
+ * 10 sub-cell pairs)
  * on n_days days of the loaded forcing, each from the same saved state, keeps the
  * fastest and restores the state.  Needs device memory for one copy of the
  * handle's buffers.  config receives [cell threads, chain threads, snow chunk,
@@ -267,10 +268,10 @@ int wg_last_timing(wg_handle *h, double *ms, int64_t *n_launches);
  * configuration. */
 int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double *ms_per_day);
 /* Sets the launch configuration directly (cell threads 64..256 in steps of 64,
- * chain threads 64..256, snow chunk 5 or 10, chain mode 0 level walk loading
- * one level ahead, 1 dataflow walk without per-level barriers, 2 level walk
- * loading two levels ahead, 3 wave walk: one wavefront per routing group in
- * batches of 64 cells, the default; chain threads do not apply to mode 3). */
+ * chain threads 64..256 (mode 3: 512 runs eight wavefronts, less four), snow
+ * chunk 5 or 10, chain mode 0 level walk loading one level ahead, 1 dataflow
+ * walk without per-level barriers, 2 level walk loading two levels ahead, 3 wave
+ * walk in batches of 64 cells, the default). */
 int wg_set_launch_config(wg_handle *h, int32_t cell_threads, int32_t chain_threads, int32_t snow_chunk,
                          int32_t chain_mode);
 

@@ -204,7 +204,7 @@ def test_launch_configs_and_autotune_are_bit_identical(wg, regional_world):
     for cell, chain, chunk, mode in ((256, 256, 5, 0), (128, 64, 10, 0), (192, 128, 5, 0), (64, 256, 10, 0),
                                      (256, 256, 5, 1), (128, 64, 10, 1), (256, 128, 5, 1),
                                      (256, 256, 5, 2), (128, 64, 10, 2), (192, 128, 5, 2),
-                                     (256, 64, 5, 3), (128, 64, 10, 3)):
+                                     (256, 256, 5, 3), (128, 512, 10, 3)):
         m = wg.WaterBalanceModel(w)
         m.load_forcing(f)
         m.set_launch_config(cell, chain, chunk, mode)
@@ -221,7 +221,7 @@ def test_launch_configs_and_autotune_are_bit_identical(wg, regional_world):
     m.step_days(0, 1, 10)
     before = m.get_state()
     cfg, ms = m.autotune(n_days=5)
-    assert cfg[0] in (128, 256) and cfg[1] in (64, 128, 256) and cfg[2] in (5, 10) and cfg[3] in (0, 1, 2, 3) and ms > 0
+    assert cfg[0] in (128, 256) and cfg[1] in (64, 128, 256, 512) and cfg[2] in (5, 10) and cfg[3] in (0, 1, 2, 3) and ms > 0
     after = m.get_state()
     for k in before:
         assert np.array_equal(np.asarray(before[k]), np.asarray(after[k])), k

@@ -257,7 +257,7 @@ class WaterBalanceModel:
         self._c(self._lib.wg_autotune(self._h, n_days, cfg, C.byref(ms)))
         return tuple(cfg), ms.value
 
-    def set_launch_config(self, cell_threads=256, chain_threads=64, snow_chunk=5, chain_mode=3):
+    def set_launch_config(self, cell_threads=256, chain_threads=512, snow_chunk=5, chain_mode=3):
         self._c(self._lib.wg_set_launch_config(self._h, cell_threads, chain_threads, snow_chunk, chain_mode))
 
     def timing(self):

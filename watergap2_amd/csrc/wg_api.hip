@@ -51,10 +51,11 @@ struct wg_handle {
   int32_t n_groups = 0, max_depth = 0;
   int32_t *d_level_off = nullptr, *d_group_first = nullptr;
   // wave walk: the number of wide levels (more than 64 cells) leading each group
-  // above level 0, and their batches dealt to four wavefronts: the batches of
-  // wavefront w of group g are d_wave_items[d_wave_item_off[4g + w] .. [4g + w + 1])
-  int32_t *d_wave_wide = nullptr, *d_wave_item_off = nullptr;
-  WgWaveItem *d_wave_items = nullptr;
+  // above level 0, and their batches dealt to W = 4 or 8 wavefronts (tables [0]
+  // and [1]): the batches of wavefront w of group g are
+  // d_wave_items[i][d_wave_item_off[i][W g + w] .. [W g + w + 1])
+  int32_t *d_wave_wide = nullptr, *d_wave_item_off[2] = {nullptr, nullptr};
+  WgWaveItem *d_wave_items[2] = {nullptr, nullptr};
   // host copies needed at the boundary
   std::vector<double> snow_ch0;   // [M][N] channel 0 of G_SnowInElevation (never read by calcNewDay)
   WgStatic S{};
@@ -71,7 +72,7 @@ struct wg_handle {
   int32_t last_daily_days = 0;
   // graphs keyed by block length
   std::map<int, hipGraphExec_t> graphs;
-  int block_threads_cell = 256, block_threads_chain = 64;
+  int block_threads_cell = 256, block_threads_chain = 256;
   int snow_chunk = 5;             // sub-cell pairs per double-buffered chunk of the cell kernel
   int chain_mode = 3;             // chain kernel: 0 level walk, loads one level ahead; 1 dataflow walk;
                                   // 2 level walk, loads two levels ahead; 3 wave walk in batches
@@ -441,9 +442,13 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
   if (e != hipSuccess) return WG_E_HIP;
   e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e != hipSuccess) return WG_E_HIP;
-  if (const char *v = getenv("WG_CHAIN_THREADS")) h->block_threads_chain = std::max(64, std::min(256, atoi(v)));
+  // the wave walk on eight wavefronts for few members (its wide levels finish
+  // sooner), on four for many (more workgroups fit per CU)
+  h->block_threads_chain = n_members <= 4 ? 512 : 256;
+  if (const char *v = getenv("WG_CHAIN_THREADS")) h->block_threads_chain = std::max(64, std::min(512, atoi(v)));
   if (const char *v = getenv("WG_XCDS")) h->n_xcd = std::max(1, std::min(64, atoi(v)));
   if (const char *v = getenv("WG_CHAIN_MODE")) h->chain_mode = std::max(0, std::min(3, atoi(v)));
+  if (h->chain_mode != 3) h->block_threads_chain = std::min(h->block_threads_chain, 256);
   int group_cells = 1024;
   if (const char *v = getenv("WG_GROUP_CELLS")) group_cells = std::max(1, atoi(v));
   int rc = build_schedule(h.get(), st, group_cells);
@@ -723,34 +728,41 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
       for (int u = 0; u < 8; u++) lk.up[u] = int16_t(u < lk.nu ? idx[off[p] + u] : 0);
     }
     // wave walk: the leading wide levels (more than 64 cells) of each group above
-    // level 0, in level-aligned batches of 64 cells dealt round-robin to the four
+    // level 0, in level-aligned batches of 64 cells dealt round-robin to the
     // wavefronts of the group's workgroup
     {
-      std::vector<int32_t> wide(size_t(H->n_groups), 0), item_off(size_t(H->n_groups) * 4 + 1, 0);
-      std::vector<WgWaveItem> items;
+      std::vector<int32_t> wide(size_t(H->n_groups), 0);
       for (int32_t gg = 0; gg < H->n_groups; gg++) {
         const int32_t l0 = H->group_first_level[gg], l1 = H->group_first_level[gg + 1];
         int32_t la = 0;
         while (l0 + 1 + la < l1 && H->level_off[l0 + 2 + la] - H->level_off[l0 + 1 + la] > 64) la++;
         wide[gg] = la;
-        for (int w = 0; w < 4; w++) {
-          item_off[size_t(gg) * 4 + w] = int32_t(items.size());
-          for (int32_t L = 1; L <= la; L++) {
-            const int32_t a = H->level_off[l0 + L], e = H->level_off[l0 + L + 1];
-            int32_t j = 0;
-            for (int32_t st0 = a; st0 < e; st0 += 64, j++)
-              if (j % 4 == w) items.push_back(WgWaveItem{st0, std::min(st0 + 64, e), L, 0});
-          }
-        }
       }
-      item_off[size_t(H->n_groups) * 4] = int32_t(items.size());
-      if (items.empty()) items.push_back(WgWaveItem{0, 0, 0, 0});
       r_ = upload(H, &H->d_wave_wide, wide);
       if (r_) { *out = h.release(); return r_; }
-      r_ = upload(H, &H->d_wave_item_off, item_off);
-      if (r_) { *out = h.release(); return r_; }
-      r_ = upload(H, &H->d_wave_items, items);
-      if (r_) { *out = h.release(); return r_; }
+      for (int t = 0; t < 2; t++) {
+        const int nw = t == 0 ? 4 : 8;
+        std::vector<int32_t> item_off(size_t(H->n_groups) * nw + 1, 0);
+        std::vector<WgWaveItem> items;
+        for (int32_t gg = 0; gg < H->n_groups; gg++) {
+          const int32_t l0 = H->group_first_level[gg];
+          for (int w = 0; w < nw; w++) {
+            item_off[size_t(gg) * nw + w] = int32_t(items.size());
+            for (int32_t L = 1; L <= wide[gg]; L++) {
+              const int32_t a = H->level_off[l0 + L], e = H->level_off[l0 + L + 1];
+              int32_t j = 0;
+              for (int32_t st0 = a; st0 < e; st0 += 64, j++)
+                if (j % nw == w) items.push_back(WgWaveItem{st0, std::min(st0 + 64, e), L, 0});
+            }
+          }
+        }
+        item_off[size_t(H->n_groups) * nw] = int32_t(items.size());
+        if (items.empty()) items.push_back(WgWaveItem{0, 0, 0, 0});
+        r_ = upload(H, &H->d_wave_item_off[t], item_off);
+        if (r_) { *out = h.release(); return r_; }
+        r_ = upload(H, &H->d_wave_items[t], items);
+        if (r_) { *out = h.release(); return r_; }
+      }
     }
     // the largest upstream count of each level
     for (int32_t l = 0; l + 1 < int32_t(H->level_off.size()); l++) {
@@ -1098,10 +1110,17 @@ int enqueue_block(wg_handle *h, int days, hipEvent_t *evs) {
     launch_cell(h, cgrid, k, k > 0 ? 1 : 0, 1, k == days - 1 ? 1 : 0, slot);
     HIP_CHECK(h, hipGetLastError());
     if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], h->stream));
-    if (h->chain_mode == 3)
-      hipLaunchKernelGGL(wg_chain_wave_kernel, chgrid, dim3(256), size_t(h->max_group_cells) * sizeof(double),
-                         h->stream, h->S, h->X, h->T, h->d_level_off, h->d_group_first, h->d_wave_wide,
-                         h->d_wave_item_off, h->d_wave_items, h->d_clock, k, slot);
+    if (h->chain_mode == 3) {
+      // the wave walk runs 8 wavefronts with a 512-thread configuration, else 4
+      if (h->block_threads_chain >= 512)
+        hipLaunchKernelGGL(wg_chain_wave_kernel<8>, chgrid, dim3(512), size_t(h->max_group_cells) * sizeof(double),
+                           h->stream, h->S, h->X, h->T, h->d_level_off, h->d_group_first, h->d_wave_wide,
+                           h->d_wave_item_off[1], h->d_wave_items[1], h->d_clock, k, slot);
+      else
+        hipLaunchKernelGGL(wg_chain_wave_kernel<4>, chgrid, dim3(256), size_t(h->max_group_cells) * sizeof(double),
+                           h->stream, h->S, h->X, h->T, h->d_level_off, h->d_group_first, h->d_wave_wide,
+                           h->d_wave_item_off[0], h->d_wave_items[0], h->d_clock, k, slot);
+    }
     else if (h->chain_mode == 1)
       hipLaunchKernelGGL(wg_chain_flow_kernel, chgrid, dim3(h->block_threads_chain),
                          size_t(h->max_group_cells) * sizeof(double), h->stream, h->S, h->X, h->T,
@@ -1271,8 +1290,9 @@ int apply_config(wg_handle *h, int cell_threads, int chain_threads, int chunk, i
 
 }  // namespace
 
-// Tries cell-kernel tiles of 128 and 256 cells, chain workgroups of 64, 128 and 256
-// threads and snow chunks of 5 and 10 sub-cell pairs on n_days days of the loaded
+// Tries cell-kernel tiles of 128 and 256 cells, the chain modes with their
+// workgroups (level walks 64, 128 or 256 threads, wave walk 4 or 8 wavefronts)
+// and snow chunks of 5 and 10 sub-cell pairs on n_days days of the loaded
 // forcing, each from the same saved state, and keeps the fastest.  Results do not
 // depend on the choice (every configuration is bit-identical).
 extern "C" int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double *ms_per_day) {
@@ -1298,7 +1318,7 @@ extern "C" int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double
   const int32_t daily = h->last_daily_days;
   int rc = WG_OK;
   if (copy_all(true) != hipSuccess) rc = fail(h, WG_E_HIP, "autotune snapshot copy");
-  const int cells[] = {128, 256}, chains[] = {64, 128, 256}, chunks[] = {5, 10}, modes[] = {0, 1, 2, 3};
+  const int cells[] = {128, 256}, chains[] = {64, 128, 256, 512}, chunks[] = {5, 10}, modes[] = {0, 1, 2, 3};
   double best = 1e300;
   int bc = h->block_threads_cell, bch = h->block_threads_chain, bk = h->snow_chunk, bm = h->chain_mode;
   for (int c : cells)
@@ -1307,7 +1327,8 @@ extern "C" int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double
         for (int md : modes) {
         if (rc) break;
         if (md == 1 && h->max_depth > WG_MAX_GROUP_DEPTH) continue;
-        if (md == 3 && ch != 64) continue;  // the wave walk runs one wavefront per group
+        // the wave walk runs 4 or 8 wavefronts (256 or 512 threads); the level walks up to 256 threads
+        if (md == 3 ? (ch != 256 && ch != 512) : ch > 256) continue;
         rc = apply_config(h, c, ch, k, md);
         double ms = 0;
         for (int rep = 0; rep < 2 && !rc; rep++) {  // the first pass loads the kernels
@@ -1343,11 +1364,12 @@ extern "C" int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double
 }
 
 // Sets the launch configuration directly (cell tile 64-256 in steps of 64, chain
-// workgroup 64-256, snow chunk 5 or 10).
+// workgroup 64-256, or up to 512 for the wave walk, snow chunk 5 or 10).
 extern "C" int wg_set_launch_config(wg_handle *h, int32_t cell_threads, int32_t chain_threads, int32_t snow_chunk,
                                     int32_t chain_mode) {
   if (!h || cell_threads < 64 || cell_threads > 256 || cell_threads % 64 || chain_threads < 64 ||
-      chain_threads > 256 || (snow_chunk != 5 && snow_chunk != 10) || chain_mode < 0 || chain_mode > 3)
+      chain_threads > (chain_mode == 3 ? 512 : 256) || (snow_chunk != 5 && snow_chunk != 10) || chain_mode < 0 ||
+      chain_mode > 3)
     return WG_E_ARG;
   HIP_CHECK(h, hipSetDevice(h->device));
   HIP_CHECK(h, hipStreamSynchronize(h->stream));

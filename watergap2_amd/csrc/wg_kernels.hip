@@ -1538,8 +1538,8 @@ __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, Wg
 // up to 64 cells, one cell per lane, each lane's inputs loaded one batch before
 // the batch is routed; transported volumes pass between lanes through LDS.
 //  - The leading wide levels (more than 64 cells) are cut into level-aligned
-//    batches dealt round-robin to the workgroup's four wavefronts, with an
-//    LDS-only barrier after each level.
+//    batches dealt round-robin to the workgroup's kWaves wavefronts (4 or 8),
+//    with an LDS-only barrier after each level.
 //  - The narrow rest is routed by wavefront 0 alone in batches of 64 cells taken
 //    in device order (by level), so that a batch covers several narrow levels;
 //    within a batch the levels run in order, each on the lanes whose cells lie
@@ -1611,7 +1611,8 @@ __device__ __forceinline__ void wave_batch(const WgStatic &S, const WgState &X, 
                 have && q.station >= 0, q.station, qq);
 }
 
-__global__ __launch_bounds__(256) void wg_chain_wave_kernel(WgStatic S, WgState X, WgTables T,
+template <int kWaves>
+__global__ __launch_bounds__(64 * kWaves) void wg_chain_wave_kernel(WgStatic S, WgState X, WgTables T,
                                                             const int32_t *__restrict__ group_level_off,
                                                             const int32_t *__restrict__ group_first_level,
                                                             const int32_t *__restrict__ wave_wide,
@@ -1636,18 +1637,19 @@ __global__ __launch_bounds__(256) void wg_chain_wave_kernel(WgStatic S, WgState 
   const int64_t base = int64_t(m) * n;
   const int la = wave_wide[g];  // wide levels l0 + 1 .. l0 + la
   const int pb = lo[l0 + 1 + la];  // first cell of the narrow rest
-  // level 0 was routed by the cell kernel: its transported volumes, eight loads
+  // level 0 was routed by the cell kernel: its transported volumes, four loads
   // in flight per thread
-  for (int i0 = gs + tid; i0 < lvl0_end; i0 += 8 * 256) {
-    double v[8];
+  constexpr int kThreads = 64 * kWaves;
+  for (int i0 = gs + tid; i0 < lvl0_end; i0 += 4 * kThreads) {
+    double v[4];
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const int i = i0 + j * 256;
+    for (int j = 0; j < 4; j++) {
+      const int i = i0 + j * kThreads;
       v[j] = (i < lvl0_end) ? X.tv[base + i] : 0.;
     }
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const int i = i0 + j * 256;
+    for (int j = 0; j < 4; j++) {
+      const int i = i0 + j * kThreads;
       if (i < lvl0_end) s_tv[i - gs] = v[j];
     }
   }
@@ -1658,7 +1660,7 @@ __global__ __launch_bounds__(256) void wg_chain_wave_kernel(WgStatic S, WgState 
     // wide levels: this wavefront's batches in level order; a padding batch
     // (past the list) routes nothing and passes no barrier.  Every wavefront
     // passes exactly la barriers, one after each wide level.
-    const int it0 = wave_item_off[g * 4 + w], it1 = wave_item_off[g * 4 + w + 1];
+    const int it0 = wave_item_off[g * kWaves + w], it1 = wave_item_off[g * kWaves + w + 1];
     auto item = [&](int i) {
       return (i < it1) ? wave_items[i] : WgWaveItem{ge, ge, 0, 0};
     };
