@@ -381,14 +381,15 @@ __device__ ChainOut chain_step(const ChainRec &c, double inflow_up, const DayInf
 }
 
 // The common case of the routing step, split so that what does not depend on the
-// upstream inflow is done beforehand: a cell without a global lake or reservoir
-// whose global wetland (if any) keeps water.  In that case the arid groundwater
-// step (routing.cpp:3305-3386) does not depend on the inflow (step_pre), and the
-// inflow-dependent rest (step_fast) is the wetland and river steps with selects
-// in place of branches.  Every operation is the one chain_step performs, in the
-// same order, so the results are chain_step's bits.  step_fast flags the cells
-// that need chain_step: a lake, a reservoir, or a wetland that runs dry (which
-// rescales the wetland's groundwater recharge, routing.cpp:3193-3200).
+// upstream inflow is done beforehand: a cell without a reservoir whose global lake
+// and global wetland (if any) keep water.  In that case the arid groundwater step
+// (routing.cpp:3305-3386) does not depend on the inflow (step_pre), and the
+// inflow-dependent rest (step_fast) is the lake, wetland and river steps, the
+// latter two with selects in place of branches.  Every operation is the one
+// chain_step performs, in the same order, so the results are chain_step's bits.
+// step_fast flags the cells that need chain_step: a reservoir, or a lake or
+// wetland that runs dry (which rescales its groundwater recharge,
+// routing.cpp:2708 and 3235).
 struct StepPre {
   double lgwrir, gw;
 };
@@ -397,8 +398,9 @@ __device__ inline StepPre step_pre(const ChainRec &c) {
   const double *v = c.v;
   StepPre r{v[CF_LGWRIR], v[CF_A_PREV]};
   if (c.flags & CFL_ARID) {
-    // chain_step's sum with no lake or reservoir recharge and the wetland's unscaled
-    const double gwr_swb = v[CF_A_GWR_LL] + 0. + v[CF_A_GWR_LW] + v[CF_GW_GWR] + 0.;
+    // chain_step's sum with the lake's and wetland's recharge unscaled and no reservoir
+    const double gwr_glolak = (c.flags & CFL_LAKE) ? v[CF_GL_GWR] : 0.;
+    const double gwr_swb = v[CF_A_GWR_LL] + gwr_glolak + v[CF_A_GWR_LW] + v[CF_GW_GWR] + 0.;
     r.lgwrir = arid_groundwater(gwr_swb, c.area, c.cf100, v[CF_A_RECH], v[CF_A_PREV], v[CF_A_S0E], c.inv_kgw,
                                 c.ome_gw, r.gw);
   }
@@ -414,7 +416,25 @@ __device__ inline ChainOut step_fast(const ChainRec &c, const StepPre &pre, doub
   o.gw = pre.gw;
   double inflow = v[CF_INFLOW];
   inflow += inflow_up;  // routing.cpp:2623
-  slow = (c.flags & (CFL_LAKE | CFL_RES)) != 0;
+  slow = (c.flags & CFL_RES) != 0;
+  if (c.flags & CFL_LAKE) {
+    // global lake, routing.cpp:2630-2804, the branch that keeps water
+    const double prev = v[CF_GL_PREV], use = v[CF_GL_USE], maxStorage = v[CF_GL_MAX];
+    const double totalInflow = inflow + v[CF_GL_LP];
+    const double useMax = totalInflow + maxStorage + prev;
+    slow = slow || (use > useMax);
+    double st = v[CF_GL_S0E] + c.inv_ksw * (totalInflow - use) * c.ome_sw;
+    double outflow = totalInflow + prev - st - use;
+    const bool over = st > maxStorage;
+    outflow = over ? outflow + (st - maxStorage) : outflow;
+    st = over ? maxStorage : st;
+    const bool neg = outflow < 0.;
+    st = neg ? prev + totalInflow - use : st;
+    outflow = neg ? 0. : outflow;
+    st = (fabs(st) <= kMinStorVol) ? 0. : st;
+    o.glo_lake = st;
+    inflow = outflow;
+  }
   {
     // global wetland, routing.cpp:3178-3297, the branch that keeps water
     const double prev = v[CF_GW_PREV], PETgwr = v[CF_GW_PET], maxStorage = v[CF_GW_MAX];
@@ -674,7 +694,9 @@ __device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p)
 
 struct SnowDay {
   double tempC, gradnt, scale_num, laf, prec_to_soil, soil_pet, snowfz, snowmt, melt_factor;
+  double eff_bare;  // the effective precipitation of a sub-cell without snow above freezing
   int elev0;
+  bool melt_ok;     // melt_factor >= 0 (a NaN or negative factor takes the full step)
 };
 
 struct SnowAcc {
@@ -686,6 +708,14 @@ struct SnowAcc {
 // sub-cell order; melt_factor = M_DEGDAY_F * ddf_lct (daily.cpp:1013).
 __device__ inline double snow_subcell(double s, int elev_e, int first, const SnowDay &d, SnowAcc &a) {
   double temp_elev = d.tempC - ((elev_e - d.elev0) * d.gradnt);
+  if (s == 0. && temp_elev > d.snowfz && d.melt_ok) {
+    // A sub-cell without snow above freezing: the storage stays 0 and the sums other
+    // than the effective precipitation gain exactly +0 (they never hold -0), so only
+    // that one is updated; eff_bare = prec_to_soil + 0 as the full step adds it.
+    if (first) a.temp_max = temp_elev;
+    a.eff += d.eff_bare;
+    return 0.;
+  }
   double snowmelt_elev = 0., eff_before = 0.;
   s = s * d.scale_num / d.laf;
   if (fabs(s) <= kMinStorVol) s = 0.;
@@ -914,6 +944,8 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     sd.snowfz = T_SNOWFZ;
     sd.snowmt = T_SNOWMT;
     sd.melt_factor = DEGDAY_F * ddf;
+    sd.eff_bare = daily_prec_to_soil + 0.;
+    sd.melt_ok = sd.melt_factor >= 0.;
     bool any_after = false;
     WgD2 sa[kSnowChunk], sb[kSnowChunk];
     int32_t ea[kSnowChunk], eb[kSnowChunk];
