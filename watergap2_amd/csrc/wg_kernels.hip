@@ -695,6 +695,7 @@ __device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p)
 struct SnowDay {
   double tempC, gradnt, scale_num, laf, prec_to_soil, soil_pet, snowfz, snowmt, melt_factor;
   double eff_bare;  // the effective precipitation of a sub-cell without snow above freezing
+  double inv_laf;   // 1 / laf, correctly rounded
   int elev0;
   bool melt_ok;     // melt_factor >= 0 (a NaN or negative factor takes the full step)
 };
@@ -717,7 +718,18 @@ __device__ inline double snow_subcell(double s, int elev_e, int first, const Sno
     return 0.;
   }
   double snowmelt_elev = 0., eff_before = 0.;
-  s = s * d.scale_num / d.laf;
+  {
+    // s * scale_num / laf with the reused reciprocal: q0 = x (1/laf), one exact
+    // residual and one correction give the correctly rounded quotient (Markstein's
+    // theorem for a correctly rounded reciprocal; tests/test_division.py compares
+    // it with the division on random operands).  An infinite or NaN x is its own quotient by the
+    // finite positive laf.  (The quotient of a finite x overflows only for
+    // x > 1e292 mm, since a positive land area fraction is at least about 1e-16.)
+    const double x = s * d.scale_num;
+    const double q0 = x * d.inv_laf;
+    const double q = fma(fma(-q0, d.laf, x), d.inv_laf, q0);
+    s = (fabs(x) <= 1.7976931348623157e308) ? q : x;
+  }
   if (fabs(s) <= kMinStorVol) s = 0.;
   const double init_s = s;
   if (s > 1000.) {
@@ -945,6 +957,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     sd.snowmt = T_SNOWMT;
     sd.melt_factor = DEGDAY_F * ddf;
     sd.eff_bare = daily_prec_to_soil + 0.;
+    sd.inv_laf = 1. / landAreaFrac;
     sd.melt_ok = sd.melt_factor >= 0.;
     bool any_after = false;
     WgD2 sa[kSnowChunk], sb[kSnowChunk];
