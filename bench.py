@@ -246,11 +246,11 @@ def surrogate_run(args):
     sur_s = time.perf_counter() - t1
     sur_ok = int(sres["within_1pc"].sum())
     runner.close()
-    by_round = np.bincount(sres["round"], minlength=5)[1:].tolist()
+    by_round = np.bincount(sres["round"], minlength=7)[1:].tolist()
     first_hits = by_round[0]
-    fb = by_round[3]
-    log(f"surrogate: basins per round {by_round[:3]}, {fb} to the plain fallback "
-        f"({sres['fallback_runs']} runs), {sur_s:.1f}s")
+    fb = by_round[5]
+    log(f"surrogate: basins per round {by_round[:5]} (three surrogate, two bracketed), {fb} to the plain "
+        f"fallback ({sres['fallback_runs']} runs), {sur_s:.1f}s")
     plain_rate = plain_ok / plain_s * 3600.0
     sur_rate = sur_ok / sur_s * 3600.0
     out.update({
@@ -263,7 +263,7 @@ def surrogate_run(args):
         "plain": {"seconds": round(plain_s, 2), "model_runs": res["model_runs"], "within_1pc": plain_ok,
                   "runs_per_basin_mean": round(float(res["runs"].mean()), 2)},
         "surrogate": {"seconds": round(sur_s, 2), "first_round_hits": first_hits,
-                      "hits_per_round": by_round[:3], "fallback_basins": fb,
+                      "hits_per_round": by_round[:5], "fallback_basins": fb,
                       "fallback_runs": sres["fallback_runs"], "within_1pc": sur_ok,
                       "runs_per_basin_mean": round(float(sres["runs"].mean()), 2),
                       "median_abs_gamma_error": round(float(np.median(np.abs(sres["gamma"] - hidden))), 4)},

@@ -184,3 +184,22 @@ def test_runoff_target_units(wg):
     q = np.ones((2, 5))
     assert np.allclose(np.exp(sg.runoff_target(q, bs)), 1e6 / area)
     assert np.allclose(sg.runoff_target(q[1:], bs, idx=[1]), np.log(1e6 / area[1]))
+
+
+def test_bracket_proposals():
+    """Interpolation between the verified points that straddle the target, and
+    outward steps within the search range when all points lie on one side."""
+    from watergap2_amd import surrogate as sg
+    # log runoff falls linearly in log gamma: y = 2 - log(g); target 2 - log(1.5)
+    pts = [(1.0, 2.0), (2.0, 2.0 - np.log(2.0)), (0.5, 2.0 + np.log(2.0))]
+    p = sg.bracket_proposals(pts, 2.0 - np.log(1.5))
+    assert p.shape == (3,) and abs(p[0] - 1.5) < 1e-12
+    assert 1.0 < p.min() and p.max() < 2.0
+    # every point has too much runoff: step to larger gamma, capped at 5
+    p = sg.bracket_proposals([(2.0, 3.0), (4.0, 2.5)], 1.0)
+    assert np.allclose(p, [5.0, 5.0, 5.0])
+    p = sg.bracket_proposals([(1.0, 3.0)], 1.0)
+    assert np.allclose(p, [1.5, 2.0, 3.0])
+    # too little runoff everywhere: smaller gamma, floored at 0.1
+    p = sg.bracket_proposals([(0.3, 0.5), (1.0, 0.2)], 1.0)
+    assert np.allclose(p, [0.2, 0.15, 0.1])

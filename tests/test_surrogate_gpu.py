@@ -26,7 +26,7 @@ def test_surrogate_needs_fewer_runs_than_the_plain_search(wg):
     plain = plain_search(r, measured, n_years=30)
     sur = sg.calibrate_with_surrogate(model, r, feats, measured)
     r.close()
-    print(f"rounds {np.bincount(sur['round'], minlength=5)[1:].tolist()}; model runs per basin: surrogate "
+    print(f"rounds {np.bincount(sur['round'], minlength=7)[1:].tolist()}; model runs per basin: surrogate "
           f"{sur['runs'].mean():.2f}, plain {plain['runs'].mean():.2f}", flush=True)
     assert sur["within_1pc"][plain["status"] == 1].all()
     assert sur["runs"].mean() < plain["runs"].mean()

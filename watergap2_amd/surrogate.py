@@ -184,7 +184,38 @@ def long_term_mean(annual):
     return thirty_year_mean(a) if a.shape[-1] == 9 else a.mean(-1)
 
 
-def calibrate_with_surrogate(model, runner, feats, measured, spread=0.006, rounds=3, log=None):
+GAMMA_LIMITS = (0.1, 5.0)  # the search range of calibGammaClass (calibration.cpp:47-48)
+
+
+def bracket_proposals(points, target, n=3):
+    """Three gamma values aimed at `target` (log runoff) from verified (gamma, log
+    runoff) points: interpolated in log gamma between the closest points on either
+    side of the target (runoff falls as gamma rises), at fractions w and w +- a
+    quarter of the bracket; with points on one side only, steps beyond the
+    outermost point by factors of 1.5, 2 and 3, within the search range."""
+    g = np.array([p_[0] for p_ in points], dtype=np.float64)
+    y = np.array([p_[1] for p_ in points], dtype=np.float64)
+    lo_g, hi_g = np.log(GAMMA_LIMITS[0]), np.log(GAMMA_LIMITS[1])
+    above = y > target           # too much runoff: gamma too small
+    if above.any() and (~above).any():
+        ia = np.flatnonzero(above)[np.argmax(g[above])]
+        ib = np.flatnonzero(~above)[np.argmin(g[~above])]
+        la, lb = np.log(g[ia]), np.log(g[ib])
+        if la < lb:
+            w = (y[ia] - target) / max(y[ia] - y[ib], 1e-12)
+            ws = np.clip(np.array([w, w - 0.25, w + 0.25]), 0.02, 0.98)
+            return np.exp(la + ws * (lb - la))[:n]
+    if above.all():              # every point has too much runoff: larger gamma
+        base = np.log(g.max())
+        steps = np.log([1.5, 2.0, 3.0])
+    else:                        # too little runoff everywhere: smaller gamma
+        base = np.log(g.min())
+        steps = -np.log([1.5, 2.0, 3.0])
+    return np.exp(np.clip(base + steps, lo_g, hi_g))[:n]
+
+
+def calibrate_with_surrogate(model, runner, feats, measured, spread=0.006, rounds=3, bracket_rounds=2,
+                             log=None):
     """Surrogate-guided calibration of every basin of the runner's set.
 
     Round 1: the network's proposal for each basin, verified by one model run of
@@ -193,12 +224,14 @@ def calibrate_with_surrogate(model, runner, feats, measured, spread=0.006, round
     miss after one point; shifted and scaled through the two closest points after
     two or more), aimed at the target and at the target +- `spread` (log runoff),
     three proposals per basin verified as three members of one run of just those
-    basins.  Then the plain search (the reference's findNewGamma) for what still
-    misses.
+    basins.  Then up to `bracket_rounds` rounds that interpolate between the
+    verified points on either side of the target (runoff falls as gamma rises),
+    three points per basin.  Then the plain search (the reference's findNewGamma)
+    for what still misses.
 
-    Returns per-basin round (1..rounds, or rounds + 1 = plain fallback), gamma,
-    model runs per basin, whether it meets the 1 % criterion, and the fallback's
-    lock-step runs."""
+    Returns per-basin round (1..rounds + bracket_rounds, or one more = plain
+    fallback), gamma, model runs per basin, whether it meets the 1 % criterion,
+    and the fallback's lock-step runs."""
     from .calibration import BasinRunner, BasinSet, plain_search, within_1pc
     bs = runner.basins
     nb = bs.n_basins
@@ -263,6 +296,26 @@ def calibrate_with_surrogate(model, runner, feats, measured, spread=0.006, round
                     n_new += 1
         if log:
             log(f"surrogate round {r}: {n_new} more within 1 %, {int((~hit).sum())} left")
+    for r in range(rounds + 1, rounds + 1 + bracket_rounds):
+        miss = np.flatnonzero(~hit)
+        if not len(miss):
+            break
+        props = np.stack([bracket_proposals(points[b], target[b]) for b in miss], axis=1)
+        sr = sub_runner(miss, props.shape[0])
+        q2 = sr.run(props, n_years=n_years)
+        sr.close()
+        runs[miss] += 1
+        n_new = 0
+        for j, b in enumerate(miss):
+            for k in range(props.shape[0]):
+                points[b].append((float(props[k, j]), float(log_runoff(q2[k, j:j + 1], [b])[0])))
+                if not hit[b] and within_1pc(q2[k, j], measured[b]):
+                    hit[b] = True
+                    rnd[b] = r
+                    gam[b] = props[k, j]
+                    n_new += 1
+        if log:
+            log(f"bracket round {r}: {n_new} more within 1 %, {int((~hit).sum())} left")
     miss = np.flatnonzero(~hit)
     fallback_runs = 0
     if len(miss):
@@ -271,7 +324,7 @@ def calibrate_with_surrogate(model, runner, feats, measured, spread=0.006, round
         sr.close()
         fallback_runs = res["model_runs"]
         runs[miss] += res["runs"]
-        rnd[miss] = rounds + 1
+        rnd[miss] = rounds + bracket_rounds + 1
         gam[miss] = res["gamma"]
         hit[miss] = res["status"] == 1
     return {"round": rnd, "gamma": gam, "runs": runs, "within_1pc": hit, "fallback_runs": fallback_runs}
