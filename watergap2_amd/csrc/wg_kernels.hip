@@ -571,43 +571,53 @@ __device__ inline double lai_day(int &days, int init_days, int &status, int lct,
 // 3286-3295, 3546-3586), then the surface-water and land area fractions
 // (routing.cpp:5046-5174) and updateLandAreaFrac (routing.cpp:5343-5352).
 __device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p) {
+  // Every input is loaded before the first store, so that the loads are in flight
+  // together (the arrays cover all cells, so the loads of unused fields are in bounds).
   const double evarexp = S.evarexp[p];
   const double cellArea = S.cell_area[p];
+  const double lake_area = S.lake_area[p], lake_depth = S.lake_depth[p], glo_lake = X.glo_lake[mp];
+  const double reservoir_area = S.reservoir_area[p], stor_cap = S.stor_cap[p], res = X.res[mp];
+  const double glo_wetland = S.glo_wetland[p], wetl_depth = S.wetl_depth[p], glo_wetl = X.glo_wetl[mp];
+  const bool contcell = S.contcell[p] != 0;
+  const double L = S.river_length[p], B = S.bottom_width[p], width_bf = S.width_bf[p];
+  const double st = X.river[mp], smax = S.river_stor_max[p];
+  const double loc_lake = S.loc_lake[p], loc_wetland = S.loc_wetland[p];
+  const double f_glo_lake = S.f_glo_lake[p], contfreq = S.contfreq[p];
   double red_ll = X.red_ll[mp], red_lw = X.red_lw[mp], red_gw = X.red_gw[mp];
-  if (S.lake_area[p] > 0.) {
-    const double maxStorage = S.lake_area[p] * S.lake_depth[p];
-    double r = 1. - wg_pow(fabs(X.glo_lake[mp] - maxStorage) / (2. * maxStorage), (evarexp * kEvapoRedExp));
+  double river_frac_next = X.river_frac_next[mp];
+  double river_frac_change = X.river_frac_change[mp];
+  double red_river = X.red_river[mp];
+  const double fswb_old = X.fswb_next[mp];
+  const double laf = X.laf[mp];
+
+  double red_gl = 0., red_res = 0.;
+  if (lake_area > 0.) {
+    const double maxStorage = lake_area * lake_depth;
+    double r = 1. - wg_pow(fabs(glo_lake - maxStorage) / (2. * maxStorage), (evarexp * kEvapoRedExp));
     if (r < 0.) r = 0.;
     if (r > 1.) r = 1.;
-    X.red_gl[mp] = r;
+    red_gl = r;
   }
-  if (S.reservoir_area[p] > 0.) {
-    const double maxStorage = S.stor_cap[p];
-    double r = 1. - wg_pow(fabs(X.res[mp] - maxStorage) / maxStorage, kEvapoRedExpRes);
+  if (reservoir_area > 0.) {
+    const double maxStorage = stor_cap;
+    double r = 1. - wg_pow(fabs(res - maxStorage) / maxStorage, kEvapoRedExpRes);
     if (r < 0.) r = 0.;
     if (r > 1.) r = 1.;
-    X.red_res[mp] = r;
+    red_res = r;
   }
-  const double glo_wetland = S.glo_wetland[p];
   if (glo_wetland > 0) {
-    const double maxStorage = ((glo_wetland) / 100.) * cellArea * S.wetl_depth[p];
-    double r = 1. - wg_pow(fabs(X.glo_wetl[mp] - maxStorage) / maxStorage, (evarexp * kEvapoRedExp));
+    const double maxStorage = ((glo_wetland) / 100.) * cellArea * wetl_depth;
+    double r = 1. - wg_pow(fabs(glo_wetl - maxStorage) / maxStorage, (evarexp * kEvapoRedExp));
     if (r < 0.) r = 0.;
     if (r > 1.) r = 1.;
     red_gw = r;
   }
-  double river_frac_next = X.river_frac_next[mp];
-  double river_frac_change = X.river_frac_change[mp];
-  double red_river = X.red_river[mp];
-  if (S.contcell[p]) {
-    const double L = S.river_length[p], B = S.bottom_width[p];
-    const double st = X.river[mp];
+  if (contcell) {
     const double riverAreaFrac = river_frac_next;
     const double csa = st / L;
     const double depth = -B / (4. * 1000.) + sqrt(B / 1000. * B / (16. * 1000.) + 0.5 * csa);
     double width = B / 1000. + 4. * depth;
-    if (width > S.width_bf[p] / 1000.) width = S.width_bf[p] / 1000.;
-    const double smax = S.river_stor_max[p];
+    if (width > width_bf / 1000.) width = width_bf / 1000.;
     red_river = 1. - wg_pow(fabs(st - smax) / smax, (evarexp * kEvapoRedExp));
     if (red_river < 0.) red_river = 0.;
     if (red_river > 1.) red_river = 1.;
@@ -615,7 +625,6 @@ __device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p)
     river_frac_change = river_frac_next - riverAreaFrac;
   }
   // fractions for the next day
-  const double loc_lake = S.loc_lake[p], loc_wetland = S.loc_wetland[p];
   double fLocLake, fLocWet, fGloWet;
   if ((loc_lake > 0.) && (red_ll > 0.))
     fLocLake = (red_ll * loc_lake / 100.);
@@ -629,11 +638,9 @@ __device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p)
     fGloWet = (red_gw * glo_wetland / 100.);
   else
     fGloWet = 0.;
-  const double fswb_old = X.fswb_next[mp];
   double fswb_next = fLocLake + fLocWet + fGloWet;
-  const double f_glo_lake = S.f_glo_lake[p];
-  const double max_raf = S.contfreq[p] / 100. - f_glo_lake;
-  if (((S.lake_area[p] > 0.) || (S.reservoir_area[p] > 0.)) && (f_glo_lake == 1.)) {
+  const double max_raf = contfreq / 100. - f_glo_lake;
+  if (((lake_area > 0.) || (reservoir_area > 0.)) && (f_glo_lake == 1.)) {
     river_frac_next = 0.;
     river_frac_change = 0.;
     red_river = 0.;
@@ -675,9 +682,10 @@ __device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p)
   }
   fswb_next = fLocLake + fLocWet + fGloWet;
   const double change_pct = fswb_next * 100. - fswb_old * 100.;
-  const double laf = X.laf[mp];
   double laf_next = laf - (change_pct + (river_frac_change * 100.));
   if (laf_next < 0.) laf_next = 0.;
+  if (lake_area > 0.) X.red_gl[mp] = red_gl;
+  if (reservoir_area > 0.) X.red_res[mp] = red_res;
   X.laf_prev[mp] = laf;
   X.laf[mp] = laf_next;
   X.fswb_next[mp] = fswb_next;
@@ -788,7 +796,9 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
                                                       int daily_slot) {
   const int t0 = S.tile_start[blockIdx.x];
   if (t0 < 0) return;
-  const int p = t0 + int(threadIdx.x);
+  const int lane = int(threadIdx.x) & 63;
+  if (lane >= S.wave_cells) return;
+  const int p = t0 + (int(threadIdx.x) >> 6) * S.wave_cells + lane;
   if (p >= S.tile_end[blockIdx.x]) return;
   const int m = blockIdx.y;
   const int n = S.n;
