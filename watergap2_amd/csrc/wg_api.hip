@@ -22,9 +22,8 @@
 
 #include "../../include/watergap2_amd.h"
 #include "wg_device.h"
-#include "wg_libm.h"
 // The kernels are compiled in this translation unit so that their launches need
-// no relocatable device code.
+// no relocatable device code.  (wg_kernels.hip includes wg_libm.h.)
 #include "wg_kernels.hip"
 
 namespace {

@@ -23,7 +23,24 @@
 #include <hip/hip_runtime.h>
 
 #include "wg_device.h"
+// The exp/log/pow table lives in LDS, one copy per wavefront (up to 4 per
+// workgroup): a lane-varying index into a global table costs a vector-memory
+// round trip per lookup, two per pow.
+__shared__ double wg_exp2_lds[4][128];
+#define WG_EXP2_TAB_LDS (wg_exp2_lds[threadIdx.x >> 6])
 #include "wg_libm.h"
+
+namespace {
+// Copies the table into the wavefront's LDS copy with one direct global-to-LDS load
+// (16 bytes per lane, lane i to bytes 16i..16i+15).  The compiler waits for it
+// before the first LDS read; workgroups of at most 256 threads.
+__device__ inline void load_exp2_table() {
+  const int lane = threadIdx.x & 63;
+  __builtin_amdgcn_global_load_lds((const void *)(wg_exp2_tab_d + 2 * lane),
+                                   (__attribute__((address_space(3))) void *)&wg_exp2_lds[threadIdx.x >> 6][0], 16,
+                                   0, 0);
+}
+}  // namespace
 
 namespace {
 
@@ -794,6 +811,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
                                                       const WgClock *__restrict__ clock, int k,
                                                       int do_post, int do_vertical, int write_diag,
                                                       int daily_slot) {
+  load_exp2_table();
   const int t0 = S.tile_start[blockIdx.x];
   if (t0 < 0) return;
   const int lane = int(threadIdx.x) & 63;
@@ -1904,6 +1922,7 @@ __global__ void wg_clock_set(WgClock *clock, int64_t day0, int32_t doy0, int32_t
 // the device returns the host's bits.
 __global__ void wg_libm_kernel(int fn, int n, const double *__restrict__ x, const double *__restrict__ y,
                                double *__restrict__ out) {
+  load_exp2_table();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double r;

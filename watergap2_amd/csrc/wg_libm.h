@@ -50,7 +50,11 @@ static const double wg_exp2_tab_h[128] = {WG_EXP2_TAB_VALUES};
 #if defined(__HIPCC__) || defined(__HIP__)
 __constant__ static const double wg_exp2_tab_d[128] = {WG_EXP2_TAB_VALUES};
 #endif
-#if defined(__HIP_DEVICE_COMPILE__)
+#if defined(__HIP_DEVICE_COMPILE__) && defined(WG_EXP2_TAB_LDS)
+/* the including file declares the LDS copy, which every kernel that calls these
+ * functions fills from wg_exp2_tab_d before its first call */
+#define WG_EXP2_TAB WG_EXP2_TAB_LDS
+#elif defined(__HIP_DEVICE_COMPILE__)
 #define WG_EXP2_TAB wg_exp2_tab_d
 #else
 #define WG_EXP2_TAB wg_exp2_tab_h
