@@ -491,7 +491,8 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
   // derived statics in reference order
   std::vector<double> alpha(n), river_length(n), river_slope(n), roughness(n), width_bf(n), bottom(n),
       stor_max(n), lake_depth(n), wetl_depth(n), f_glo_lake(n), fswb_init(n), k_sw(n), e_sw(n), inv_ksw(n),
-      ome_sw(n), k_gw(n), e_gw(n), inv_kgw(n), ome_gw(n);
+      ome_sw(n), k_gw(n), e_gw(n), inv_kgw(n), ome_gw(n), vel_c1(n), vel_c2(n), rw_c0(n), rw_c1(n), rw_c2(n),
+      rw_c3(n);
   std::vector<float> lai_max(n);
   std::vector<int8_t> contcell(n);
   for (int32_t i = 0; i < n; i++) {
@@ -521,6 +522,14 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     e_gw[i] = wg_exp(-1. * k_gw[i]);
     inv_kgw[i] = 1. / k_gw[i];
     ome_gw[i] = 1. - wg_exp(-1. * k_gw[i]);
+    // constants of getNewRiverVelocity (routing.cpp:7309-7341) and of the river width
+    // (routing.cpp:3546-3560), in the reference's operation order
+    vel_c1[i] = 1. / (P(WG_M_RIVRGH_C, i) * roughness[i]);
+    vel_c2[i] = wg_pow(river_slope[i], 0.5);
+    rw_c0[i] = -bottom[i] / (4. * 1000.);
+    rw_c1[i] = bottom[i] / 1000. * bottom[i] / (16. * 1000.);
+    rw_c2[i] = bottom[i] / 1000.;
+    rw_c3[i] = width_bf[i] / 1000.;
   }
 
   // initial state (one member, reference order)
@@ -587,6 +596,12 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     UPR(d, st->mean_outflow); S.mean_outflow = d;
     UP(d, river_length); S.river_length = d;
     UP(d, river_slope); S.river_slope = d;
+    UP(d, vel_c1); S.vel_c1 = d;
+    UP(d, vel_c2); S.vel_c2 = d;
+    UP(d, rw_c0); S.rw_c0 = d;
+    UP(d, rw_c1); S.rw_c1 = d;
+    UP(d, rw_c2); S.rw_c2 = d;
+    UP(d, rw_c3); S.rw_c3 = d;
     UP(d, roughness); S.roughness = d;
     UP(d, width_bf); S.width_bf = d;
     UP(d, bottom); S.bottom_width = d;

@@ -86,6 +86,9 @@ struct WgStatic {
   const int8_t *res_type, *start_month;
   // exp(-k) terms of the surface-water and groundwater outflow coefficients
   const double *k_sw, *e_sw, *inv_ksw, *ome_sw, *k_gw, *e_gw, *inv_kgw, *ome_gw;
+  // per-cell constants of the river velocity and width (routing.cpp:3546-3560, 7309-7341):
+  // 1/(M_RIVRGH_C roughness), slope^0.5, -B/4000, (B/1000) B/16000, B/1000, width_bf/1000
+  const double *vel_c1, *vel_c2, *rw_c0, *rw_c1, *rw_c2, *rw_c3;
   const double *cf100;              // contfreq / 100
   const int8_t *cflags;             // lake / reservoir / global wetland / arid / LDD >= 0 / continental
   const WgI4 *clink;                // [2][N] routing links

@@ -504,13 +504,14 @@ __device__ inline void store_chain(const WgState &X, int64_t mp, int flags, cons
 }
 
 // routing.cpp:7309-7341 getNewRiverVelocity
-__device__ inline double new_river_velocity(double slope, double bottom, double rough, double stor,
-                                            double length, double mult) {
+// c1 = 1. / (mult * rough) and c2 = pow(slope, 0.5) are per-cell constants computed
+// on the host with the same operations, so the product below is the reference's.
+__device__ inline double new_river_velocity(double c1, double c2, double bottom, double stor, double length) {
   const double csa = (stor / length) * 1000000.;
   const double depth = -bottom / 4.0 + sqrt(bottom * bottom / 16.0 + 0.5 * csa);
   const double wp = bottom + 2.0 * depth * sqrt(5.0);
   const double hr = csa / wp;
-  double v = 1. / (mult * rough) * wg_pow(hr, (2. / 3.)) * wg_pow(slope, 0.5);
+  double v = c1 * wg_pow(hr, (2. / 3.)) * c2;
   v = v * 86.4;
   return (v < 0.00001) ? 0.00001 : v;
 }
@@ -596,7 +597,8 @@ __device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p)
   const double reservoir_area = S.reservoir_area[p], stor_cap = S.stor_cap[p], res = X.res[mp];
   const double glo_wetland = S.glo_wetland[p], wetl_depth = S.wetl_depth[p], glo_wetl = X.glo_wetl[mp];
   const bool contcell = S.contcell[p] != 0;
-  const double L = S.river_length[p], B = S.bottom_width[p], width_bf = S.width_bf[p];
+  const double L = S.river_length[p];
+  const double rw_c0 = S.rw_c0[p], rw_c1 = S.rw_c1[p], rw_c2 = S.rw_c2[p], rw_c3 = S.rw_c3[p];
   const double st = X.river[mp], smax = S.river_stor_max[p];
   const double loc_lake = S.loc_lake[p], loc_wetland = S.loc_wetland[p];
   const double f_glo_lake = S.f_glo_lake[p], contfreq = S.contfreq[p];
@@ -632,9 +634,11 @@ __device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p)
   if (contcell) {
     const double riverAreaFrac = river_frac_next;
     const double csa = st / L;
-    const double depth = -B / (4. * 1000.) + sqrt(B / 1000. * B / (16. * 1000.) + 0.5 * csa);
-    double width = B / 1000. + 4. * depth;
-    if (width > width_bf / 1000.) width = width_bf / 1000.;
+    // -B / (4. * 1000.), B / 1000. * B / (16. * 1000.), B / 1000. and width_bf / 1000.
+    // are per-cell constants computed on the host with these operations
+    const double depth = rw_c0 + sqrt(rw_c1 + 0.5 * csa);
+    double width = rw_c2 + 4. * depth;
+    if (width > rw_c3) width = rw_c3;
     red_river = 1. - wg_pow(fabs(st - smax) / smax, (evarexp * kEvapoRedExp));
     if (red_river < 0.) red_river = 0.;
     if (red_river > 1.) red_river = 1.;
@@ -730,19 +734,15 @@ struct SnowAcc {
   int thresh;
 };
 
-// Updates one sub-cell storage s (elevation elev_e) and the cell's running sums in
-// sub-cell order; melt_factor = M_DEGDAY_F * ddf_lct (daily.cpp:1013).
-__device__ inline double snow_subcell(double s, int elev_e, int first, const SnowDay &d, SnowAcc &a) {
-  double temp_elev = d.tempC - ((elev_e - d.elev0) * d.gradnt);
-  if (s == 0. && temp_elev > d.snowfz && d.melt_ok) {
-    // A sub-cell without snow above freezing: the storage stays 0 and the sums other
-    // than the effective precipitation gain exactly +0 (they never hold -0), so only
-    // that one is updated; eff_bare = prec_to_soil + 0 as the full step adds it.
-    if (first) a.temp_max = temp_elev;
-    a.eff += d.eff_bare;
-    return 0.;
-  }
-  double snowmelt_elev = 0., eff_before = 0.;
+// Updates one sub-cell storage s and the cell's running sums in sub-cell order
+// (daily.cpp:920-1044); temp_elev is the sub-cell temperature T - (z_e - z0) gradnt
+// and melt_factor = M_DEGDAY_F * ddf_lct (daily.cpp:1013).  The reference's branches
+// are selects here: every operation is the reference's, on the same operands, and
+// a branch not taken adds +0 to a sum, which leaves it unchanged (the sums start at
+// +0 and never hold -0).  Without branches the two sub-cells of a pair are
+// independent chains that the compiler interleaves.
+__device__ inline double snow_subcell(double s, double temp_elev, int elev_e, bool first, const SnowDay &d,
+                                      SnowAcc &a) {
   {
     // s * scale_num / laf with the reused reciprocal: q0 = x (1/laf), one exact
     // residual and one correction give the correctly rounded quotient (Markstein's
@@ -755,38 +755,29 @@ __device__ inline double snow_subcell(double s, int elev_e, int first, const Sno
     const double q = fma(fma(-q0, d.laf, x), d.inv_laf, q0);
     s = (fabs(x) <= 1.7976931348623157e308) ? q : x;
   }
-  if (fabs(s) <= kMinStorVol) s = 0.;
+  s = (fabs(s) <= kMinStorVol) ? 0. : s;
   const double init_s = s;
-  if (s > 1000.) {
-    if (a.thresh == 0.) {
-      a.thresh = elev_e;
-    } else if (a.thresh > 0.) {
-      temp_elev = d.tempC - ((a.thresh - d.elev0) * d.gradnt);
-    }
-  }
-  if (temp_elev <= d.snowfz) {
-    s += d.prec_to_soil;
-    if (s > d.soil_pet) {
-      s -= d.soil_pet;
-      a.evapo += d.soil_pet;
-    } else {
-      a.evapo += s;
-      s = 0.;
-    }
-  } else {
-    eff_before = d.prec_to_soil;
-  }
-  if (temp_elev > d.snowmt) {
-    if (!(s < 0.)) {
-      snowmelt_elev = d.melt_factor * (temp_elev - d.snowmt);
-      if (snowmelt_elev > s) {
-        snowmelt_elev = s;
-        s = 0.;
-      } else {
-        s -= snowmelt_elev;
-      }
-    }
-  }
+  // threshold elevation above 1000 mm of storage (daily.cpp:958-976)
+  const bool big = s > 1000.;
+  const bool use_thresh = big && a.thresh > 0;
+  const double temp_thresh = d.tempC - ((a.thresh - d.elev0) * d.gradnt);
+  a.thresh = (big && a.thresh == 0) ? elev_e : a.thresh;
+  temp_elev = use_thresh ? temp_thresh : temp_elev;
+  // snowfall and sublimation below the freezing temperature, rain above it
+  const bool freeze = temp_elev <= d.snowfz;
+  const double s_fall = s + d.prec_to_soil;
+  const bool keeps = s_fall > d.soil_pet;
+  const double s_subl = s_fall - d.soil_pet;
+  a.evapo += freeze ? (keeps ? d.soil_pet : s_fall) : 0.;
+  s = freeze ? (keeps ? s_subl : 0.) : s;
+  const double eff_before = freeze ? 0. : d.prec_to_soil;
+  // degree-day melt
+  const bool melts = (temp_elev > d.snowmt) && !(s < 0.);
+  const double melt_pot = d.melt_factor * (temp_elev - d.snowmt);
+  const bool all = melt_pot > s;
+  const double s_left = s - melt_pot;
+  const double snowmelt_elev = melts ? (all ? s : melt_pot) : 0.;
+  s = melts ? (all ? 0. : s_left) : s;
   const double eff_elev = eff_before + snowmelt_elev;
   a.change += s - init_s;
   if (first) a.temp_max = temp_elev;
@@ -1009,8 +1000,23 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
       for (int j = 0; j < kSnowChunk; j++) {
         const int32_t e2 = ea[j];
         WgD2 v = sa[j];
-        v.x = snow_subcell(v.x, int(int16_t(e2 & 0xffff)), c == 0 && j == 0, sd, acc);
-        v.y = snow_subcell(v.y, int(int16_t(e2 >> 16)), 0, sd, acc);
+        const int ex = int(int16_t(e2 & 0xffff)), ey = int(int16_t(e2 >> 16));
+        const double tx = sd.tempC - ((ex - sd.elev0) * sd.gradnt);
+        const double ty = sd.tempC - ((ey - sd.elev0) * sd.gradnt);
+        const bool first = c == 0 && j == 0;
+        // Two sub-cells without snow above freezing keep a zero storage and add
+        // eff_bare = prec_to_soil + 0 to the effective precipitation, the full step's
+        // result for them; the wavefront takes this path when all its lanes may.
+        const bool bare = v.x == 0. && v.y == 0. && tx > sd.snowfz && ty > sd.snowfz && sd.melt_ok;
+        if (__builtin_amdgcn_ballot_w64(!bare) == 0) {
+          if (first) acc.temp_max = tx;
+          acc.eff += sd.eff_bare;
+          acc.eff += sd.eff_bare;
+          v = WgD2{0., 0.};
+        } else {
+          v.x = snow_subcell(v.x, tx, ex, first, sd, acc);
+          v.y = snow_subcell(v.y, ty, ey, false, sd, acc);
+        }
         const bool nz = v.x != 0. || v.y != 0.;
         any_after |= nz;
         if (snow_any || nz) sp[int64_t(c * kSnowChunk + j) * n] = v;
@@ -1293,8 +1299,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     // river velocity and storage terms, routing.cpp:3424-3478
     const double L = S.river_length[p];
     const double prev = X.river[mp];
-    const double velocity = new_river_velocity(S.river_slope[p], S.bottom_width[p], S.roughness[p], prev, L,
-                                               S.m_rivrgh[p]);
+    const double velocity = new_river_velocity(S.vel_c1[p], S.vel_c2[p], S.bottom_width[p], prev, L);
     const double K = velocity / L;
     const double eK = wg_exp(-1. * K);
     const double riverAreaFrac = X.river_frac_next[mp];
