@@ -72,7 +72,6 @@ struct wg_handle {
   // graphs keyed by block length
   std::map<int, hipGraphExec_t> graphs;
   int block_threads_cell = 256, block_threads_chain = 256;
-  int cell_wave_cells = 64;       // cells per cell-kernel wavefront (16..64)
   int snow_chunk = 5;             // sub-cell pairs per double-buffered chunk of the cell kernel
   int chain_mode = 3;             // chain kernel: 0 level walk, loads one level ahead; 1 dataflow walk;
                                   // 2 level walk, loads two levels ahead; 3 wave walk in batches
@@ -236,11 +235,9 @@ extern "C" const char *wg_last_error(const wg_handle *h) { return h ? h->err.c_s
 
 namespace {
 
-// Cell-kernel tiles of block_threads_cell / 64 wavefronts of wave_cells cells each:
-// block 8j + x covers tile j of XCD bin x.
+// Cell-kernel tiles of block_threads_cell cells: block 8j + x covers tile j of XCD bin x.
 void build_tiles(wg_handle *h) {
-  const int nx = h->n_xcd, T = h->block_threads_cell / 64 * h->cell_wave_cells;
-  h->S.wave_cells = h->cell_wave_cells;
+  const int nx = h->n_xcd, T = h->block_threads_cell;
   int32_t max_tiles = 1;
   for (int x = 0; x < nx; x++) max_tiles = std::max(max_tiles, (h->bin_end[x] - h->bin_start[x] + T - 1) / T);
   h->tile_start.assign(size_t(nx) * max_tiles, -1);
@@ -450,7 +447,6 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
   if (const char *v = getenv("WG_CHAIN_THREADS")) h->block_threads_chain = std::max(64, std::min(512, atoi(v)));
   if (const char *v = getenv("WG_XCDS")) h->n_xcd = std::max(1, std::min(64, atoi(v)));
   if (const char *v = getenv("WG_CHAIN_MODE")) h->chain_mode = std::max(0, std::min(3, atoi(v)));
-  if (const char *v = getenv("WG_CELL_WAVE_CELLS")) h->cell_wave_cells = std::max(16, std::min(64, atoi(v)));
   if (h->chain_mode != 3) h->block_threads_chain = std::min(h->block_threads_chain, 256);
   int group_cells = 1024;
   if (const char *v = getenv("WG_GROUP_CELLS")) group_cells = std::max(1, atoi(v));
@@ -803,11 +799,11 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     r_ = upload(H, &dcs, cst_p);
     if (r_) { *out = h.release(); return r_; }
     S.cstat = dcs;
-    // tile tables sized for the smallest cell tile (one wavefront of 16 cells)
+    // tile tables sized for the smallest cell block the tuner tries (64 threads)
     int32_t *dt;
     {
       int32_t cap = 1;
-      for (int x = 0; x < H->n_xcd; x++) cap = std::max(cap, (H->bin_end[x] - H->bin_start[x] + 15) / 16);
+      for (int x = 0; x < H->n_xcd; x++) cap = std::max(cap, (H->bin_end[x] - H->bin_start[x] + 63) / 64);
       H->tile_capacity = cap * H->n_xcd;
     }
     r_ = dev_alloc(H, &dt, size_t(H->tile_capacity));

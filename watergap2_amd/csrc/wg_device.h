@@ -97,9 +97,6 @@ struct WgStatic {
   // tile_end[b]) (tile_start -1: idle block); the chain kernel's block i routes
   // group block_group[i] (-1: idle).  Block i runs on XCD i % 8.
   const int32_t *tile_start, *tile_end, *block_group;
-  // cells per wavefront in a cell-kernel tile (64: full wavefronts); wavefront w of
-  // block b covers cells tile_start[b] + w * wave_cells + [0, wave_cells)
-  int32_t wave_cells;
   // schedule
   const int32_t *up_off, *up_idx;   // CSR of upstream cells sorted by routing order
   const int32_t *station;           // station slot of each cell, -1 = none

@@ -805,9 +805,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   load_exp2_table();
   const int t0 = S.tile_start[blockIdx.x];
   if (t0 < 0) return;
-  const int lane = int(threadIdx.x) & 63;
-  if (lane >= S.wave_cells) return;
-  const int p = t0 + (int(threadIdx.x) >> 6) * S.wave_cells + lane;
+  const int p = t0 + int(threadIdx.x);
   if (p >= S.tile_end[blockIdx.x]) return;
   const int m = blockIdx.y;
   const int n = S.n;
