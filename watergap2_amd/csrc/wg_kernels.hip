@@ -605,27 +605,42 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
           eb[j] = ep[int64_t((c + 1) * kSnowChunk + j) * n];
         }
       }
+      // Sub-cells without snow above freezing keep a zero storage and add
+      // eff_bare = prec_to_soil + 0 to the effective precipitation, the full step's
+      // result for them.  The wavefront takes that path for the whole chunk when all
+      // its lanes may; otherwise every lane runs the full step, one straight-line
+      // block for the chunk, whose sub-cells the compiler interleaves.
+      double tx[kSnowChunk], ty[kSnowChunk];
+      int ex[kSnowChunk], ey[kSnowChunk];
+      bool bare = true;
 #pragma unroll
       for (int j = 0; j < kSnowChunk; j++) {
         const int32_t e2 = ea[j];
-        WgD2 v = sa[j];
-        const int ex = int(int16_t(e2 & 0xffff)), ey = int(int16_t(e2 >> 16));
-        const double tx = sd.tempC - ((ex - sd.elev0) * sd.gradnt);
-        const double ty = sd.tempC - ((ey - sd.elev0) * sd.gradnt);
-        const bool first = c == 0 && j == 0;
-        // Two sub-cells without snow above freezing keep a zero storage and add
-        // eff_bare = prec_to_soil + 0 to the effective precipitation, the full step's
-        // result for them; the wavefront takes this path when all its lanes may.
-        const bool bare = v.x == 0. && v.y == 0. && tx > sd.snowfz && ty > sd.snowfz && sd.melt_ok;
-        if (__builtin_amdgcn_ballot_w64(!bare) == 0) {
-          if (first) acc.temp_max = tx;
+        ex[j] = int(int16_t(e2 & 0xffff));
+        ey[j] = int(int16_t(e2 >> 16));
+        tx[j] = sd.tempC - ((ex[j] - sd.elev0) * sd.gradnt);
+        ty[j] = sd.tempC - ((ey[j] - sd.elev0) * sd.gradnt);
+        bare = bare && sa[j].x == 0. && sa[j].y == 0. && tx[j] > sd.snowfz && ty[j] > sd.snowfz;
+      }
+      bare = bare && sd.melt_ok;
+      if (__builtin_amdgcn_ballot_w64(!bare) == 0) {
+        if (c == 0) acc.temp_max = tx[0];
+#pragma unroll
+        for (int j = 0; j < kSnowChunk; j++) {
           acc.eff += sd.eff_bare;
           acc.eff += sd.eff_bare;
-          v = WgD2{0., 0.};
-        } else {
-          v.x = snow_subcell(v.x, tx, ex, first, sd, acc);
-          v.y = snow_subcell(v.y, ty, ey, false, sd, acc);
+          sa[j] = WgD2{0., 0.};
         }
+      } else {
+#pragma unroll
+        for (int j = 0; j < kSnowChunk; j++) {
+          sa[j].x = snow_subcell(sa[j].x, tx[j], ex[j], c == 0 && j == 0, sd, acc);
+          sa[j].y = snow_subcell(sa[j].y, ty[j], ey[j], false, sd, acc);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kSnowChunk; j++) {
+        const WgD2 v = sa[j];
         const bool nz = v.x != 0. || v.y != 0.;
         any_after |= nz;
         if (snow_any || nz) sp[int64_t(c * kSnowChunk + j) * n] = v;
