@@ -199,7 +199,11 @@ __device__ inline double lai_day(int &days, int init_days, int &status, int lct,
 // storages, river width and area fraction (routing.cpp:2792-2802, 3071-3078,
 // 3286-3295, 3546-3586), then the surface-water and land area fractions
 // (routing.cpp:5046-5174) and updateLandAreaFrac (routing.cpp:5343-5352).
-__device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p) {
+struct PostDay {
+  double laf, laf_prev, red_ll, red_lw, red_gw, red_gl, red_res, river_frac_next;
+};
+
+__device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p, PostDay &out) {
   // Every input is loaded before the first store, so that the loads are in flight
   // together (the arrays cover all cells, so the loads of unused fields are in bounds).
   const double evarexp = S.evarexp[p];
@@ -327,6 +331,14 @@ __device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p)
   X.red_river[mp] = red_river;
   X.river_frac_next[mp] = river_frac_next;
   X.river_frac_change[mp] = river_frac_change;
+  out.laf = laf_next;
+  out.laf_prev = laf;
+  out.red_ll = red_ll;
+  out.red_lw = red_lw;
+  out.red_gw = red_gw;
+  out.red_gl = red_gl;
+  out.red_res = red_res;
+  out.river_frac_next = river_frac_next;
 }
 
 
@@ -399,6 +411,158 @@ __device__ inline double snow_subcell(double s, double temp_elev, int elev_e, bo
 
 }  // namespace
 
+// Per-cell inputs of the cell kernel: statics (s_), state (x_) and land-cover
+// table entries (t_).  The kernel loads them at two points, each before the first
+// store of the part that reads them, so that a part's loads are in flight together
+// (after a store the compiler must keep later loads behind it): load_cell_in before
+// post_day for PET, LAI, canopy and snow, load_cell_in_b after the snow loop for
+// soil and routing.  Every array covers all cells, so the loads of fields a cell
+// does not use are in bounds.
+struct CellIn {
+  int16_t s_aindex;
+  double s_alpha;
+  double s_bottom_width;
+  float s_built_up;
+  double s_cell_area;
+  double s_contfreq;
+  double s_degday;
+  double s_e_gw;
+  double s_e_sw;
+  int16_t s_elev0;
+  double s_evarexp;
+  double s_fswb_init;
+  double s_glo_wetland;
+  float s_gw_factor;
+  double s_inv_kgw;
+  double s_k_sw;
+  float s_lai_max;
+  double s_lake_area;
+  double s_lake_depth;
+  int8_t s_land_cover;
+  int8_t s_ldd;
+  double s_loc_lake;
+  double s_loc_wetland;
+  double s_m_netrad;
+  double s_m_prec;
+  double s_mcwh;
+  double s_mean_outflow;
+  double s_ome_gw;
+  double s_pcritgwa;
+  double s_pet_mxdy;
+  double s_reservoir_area;
+  int16_t s_rgmax;
+  double s_river_length;
+  double s_vel_c1;
+  double s_vel_c2;
+  double s_stor_cap;
+  double s_t_gradnt;
+  double s_t_snowfz;
+  double s_t_snowmt;
+  int8_t s_texture;
+  double s_wetl_depth;
+  double x_canopy;
+  double x_cfa;
+  double x_gamma;
+  double x_glo_lake;
+  double x_glo_wetl;
+  double x_gw;
+  double x_k_release;
+  int32_t x_lai_days;
+  double x_lai_precsum;
+  int32_t x_lai_status;
+  double x_loc_lake;
+  double x_loc_wetl;
+  double x_res;
+  double x_river;
+  float x_smax;
+  double x_snow;
+  uint8_t x_snow_any;
+  double x_soil;
+  double x_transfer;
+  double t_albedo, t_albedo_snow, t_ddf, t_emissivity;
+  float t_lai_a, t_lai_b;
+  int32_t t_initial_days;
+  WgChainLink link;
+  WgChainStatic cstat;
+};
+
+__device__ inline void load_cell_in(const WgStatic &S, const WgState &X, const WgTables &T, int64_t mp, int p,
+                                    CellIn &ci) {
+  ci.s_aindex = S.aindex[p];
+  ci.s_alpha = S.alpha[p];
+  ci.s_degday = S.degday[p];
+  ci.s_elev0 = S.elev0[p];
+  ci.s_lai_max = S.lai_max[p];
+  ci.s_land_cover = S.land_cover[p];
+  ci.s_m_netrad = S.m_netrad[p];
+  ci.s_m_prec = S.m_prec[p];
+  ci.s_mcwh = S.mcwh[p];
+  ci.s_pet_mxdy = S.pet_mxdy[p];
+  ci.s_t_gradnt = S.t_gradnt[p];
+  ci.s_t_snowfz = S.t_snowfz[p];
+  ci.s_t_snowmt = S.t_snowmt[p];
+  ci.x_canopy = X.canopy[mp];
+  ci.x_lai_days = X.lai_days[mp];
+  ci.x_lai_precsum = X.lai_precsum[mp];
+  ci.x_lai_status = X.lai_status[mp];
+  ci.x_snow = X.snow[mp];
+  ci.x_snow_any = X.snow_any[mp];
+  ci.x_transfer = X.transfer[mp];
+  const int l = ci.s_land_cover - 1;
+  ci.t_albedo = T.albedo[l];
+  ci.t_albedo_snow = T.albedo_snow[l];
+  ci.t_ddf = T.ddf[l];
+  ci.t_emissivity = T.emissivity[l];
+  ci.t_initial_days = T.initial_days[l];
+  ci.t_lai_a = T.lai_a[l];
+  ci.t_lai_b = T.lai_b[l];
+}
+
+__device__ inline void load_cell_in_b(const WgStatic &S, const WgState &X, int64_t mp, int p, CellIn &ci) {
+  ci.s_bottom_width = S.bottom_width[p];
+  ci.s_built_up = S.built_up[p];
+  ci.s_cell_area = S.cell_area[p];
+  ci.s_contfreq = S.contfreq[p];
+  ci.s_e_gw = S.e_gw[p];
+  ci.s_e_sw = S.e_sw[p];
+  ci.s_evarexp = S.evarexp[p];
+  ci.s_fswb_init = S.fswb_init[p];
+  ci.s_glo_wetland = S.glo_wetland[p];
+  ci.s_gw_factor = S.gw_factor[p];
+  ci.s_inv_kgw = S.inv_kgw[p];
+  ci.s_k_sw = S.k_sw[p];
+  ci.s_lake_area = S.lake_area[p];
+  ci.s_lake_depth = S.lake_depth[p];
+  ci.s_ldd = S.ldd[p];
+  ci.s_loc_lake = S.loc_lake[p];
+  ci.s_loc_wetland = S.loc_wetland[p];
+  ci.s_mean_outflow = S.mean_outflow[p];
+  ci.s_ome_gw = S.ome_gw[p];
+  ci.s_pcritgwa = S.pcritgwa[p];
+  ci.s_reservoir_area = S.reservoir_area[p];
+  ci.s_rgmax = S.rgmax[p];
+  ci.s_river_length = S.river_length[p];
+  ci.s_vel_c1 = S.vel_c1[p];
+  ci.s_vel_c2 = S.vel_c2[p];
+  ci.s_stor_cap = S.stor_cap[p];
+  ci.s_texture = S.texture[p];
+  ci.s_wetl_depth = S.wetl_depth[p];
+  ci.x_cfa = X.cfa[mp];
+  ci.x_gamma = X.gamma[mp];
+  ci.x_glo_lake = X.glo_lake[mp];
+  ci.x_glo_wetl = X.glo_wetl[mp];
+  ci.x_gw = X.gw[mp];
+  ci.x_k_release = X.k_release[mp];
+  ci.x_loc_lake = X.loc_lake[mp];
+  ci.x_loc_wetl = X.loc_wetl[mp];
+  ci.x_res = X.res[mp];
+  ci.x_river = X.river[mp];
+  ci.x_smax = X.smax[mp];
+  ci.x_soil = X.soil[mp];
+  ci.link = load_link(S, p);
+  ci.cstat = load_cstat(S, p);
+}
+
 struct WgClock {
   int64_t day0;   // forcing day (index into the loaded forcing) of launch offset 0
   int32_t doy0;   // julian day of launch offset 0
@@ -407,8 +571,11 @@ struct WgClock {
 
 // One lane per (member, cell).  Grid: x over cell tiles, y over members.
 // kSnowChunk: sub-cell pairs per double-buffered chunk of the snow loop (5 or 10).
+// With chunks of 5 pairs the kernel is held to 168 registers, three wavefronts per
+// SIMD, for many members, where it is bandwidth-bound; chunks of 10 leave the
+// compiler free, for one member, where it is latency-bound.
 template <int kSnowChunk>
-__global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgTables T,
+__global__ __launch_bounds__(256, kSnowChunk == 5 ? 3 : 1) void wg_cell_kernel(WgStatic S, WgState X, WgTables T,
                                                       const float *__restrict__ forcing, int64_t n_forcing_days,
                                                       const WgClock *__restrict__ clock, int k,
                                                       int do_post, int do_vertical, int write_diag,
@@ -422,7 +589,22 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   const int n = S.n;
   const int64_t mp = int64_t(m) * n + p;
 
-  if (do_post) post_day(S, X, mp, p);
+  CellIn ci;
+  if (do_vertical) load_cell_in(S, X, T, mp, p, ci);
+  // the fractions post_day writes, which the rest of the kernel reads
+  PostDay pd;
+  if (do_post) {
+    post_day(S, X, mp, p, pd);
+  } else if (do_vertical) {
+    pd.laf = X.laf[mp];
+    pd.laf_prev = X.laf_prev[mp];
+    pd.red_ll = X.red_ll[mp];
+    pd.red_lw = X.red_lw[mp];
+    pd.red_gw = X.red_gw[mp];
+    pd.red_gl = X.red_gl[mp];
+    pd.red_res = X.red_res[mp];
+    pd.river_frac_next = X.river_frac_next[mp];
+  }
   // calcNewDay and routing run for continental cells only (integrateWGHM.cpp:770,
   // routing.cpp:1876); other cells keep a zero transported volume.
   if (!do_vertical) return;
@@ -451,43 +633,43 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   // ------------------------------------------------------------------
   // calcNewDay, daily.cpp:94-1279
   // ------------------------------------------------------------------
-  const int lct = S.land_cover[p];
-  const bool arid_gw = (S.aindex[p] == 1);
-  const double landAreaFrac = X.laf[mp];
-  const double lafPrev = started ? X.laf_prev[mp] : landAreaFrac;  // daily.cpp:159-169
+  const int lct = ci.s_land_cover;
+  const bool arid_gw = (ci.s_aindex == 1);
+  const double landAreaFrac = pd.laf;
+  const double lafPrev = started ? pd.laf_prev : landAreaFrac;  // daily.cpp:159-169
   double dailyTempC = tf, dailyPrec = pf;
   const double dailyShortWave = swf, dailyLongWave = lwf;
-  dailyPrec = S.m_prec[p] * dailyPrec;  // daily.cpp:248
+  dailyPrec = ci.s_m_prec * dailyPrec;  // daily.cpp:248
   const double temp2 = dailyTempC + 237.3;
   const double e_s = 0.6108 * wg_exp(17.27 * dailyTempC / temp2);
-  const double alpha = S.alpha[p];
-  const double maxDailyPET = S.pet_mxdy[p];
+  const double alpha = ci.s_alpha;
+  const double maxDailyPET = ci.s_pet_mxdy;
 
   // lai.cpp:152-177
   double dailyLai;
   {
-    const float lmax_f = S.lai_max[p];
+    const float lmax_f = ci.s_lai_max;
     const double lmax = double(lmax_f);
-    const double lmin = T.lai_a[lct - 1] + T.lai_b[lct - 1] * lmax_f;
-    int days = X.lai_days[mp], status = X.lai_status[mp];
-    double precsum = X.lai_precsum[mp];
-    dailyLai = lai_day(days, T.initial_days[lct - 1], status, lct, arid_gw, lmin, lmax, precsum, dailyPrec,
+    const double lmin = ci.t_lai_a + ci.t_lai_b * lmax_f;
+    int days = ci.x_lai_days, status = ci.x_lai_status;
+    double precsum = ci.x_lai_precsum;
+    dailyLai = lai_day(days, ci.t_initial_days, status, lct, arid_gw, lmin, lmax, precsum, dailyPrec,
                        dailyTempC > 8.);
     X.lai_days[mp] = days;
     X.lai_status[mp] = status;
     X.lai_precsum[mp] = precsum;
   }
   // daily.cpp:366-462 albedo, latent heat, radiation (cloud = 1)
-  const double albedo = (X.snow[mp] > 3.) ? T.albedo_snow[lct - 1] : T.albedo[lct - 1];
+  const double albedo = (ci.x_snow > 3.) ? ci.t_albedo_snow : ci.t_albedo;
   const double lat_heat = (dailyTempC > 0) ? (2.501 - 0.002361 * dailyTempC) : 2.835;
   const double conv_Wm2_to_mmd = 0.0864 / lat_heat;
   const double solar_rad = conv_Wm2_to_mmd * dailyShortWave;
   const double long_wave_rad_in = conv_Wm2_to_mmd * dailyLongWave;
   const double temp_K = dailyTempC + 273.2;
-  const double long_wave_rad_out = T.emissivity[lct - 1] * 0.000000004903 * wg_pow(temp_K, 4.) / lat_heat;
+  const double long_wave_rad_out = ci.t_emissivity * 0.000000004903 * wg_pow(temp_K, 4.) / lat_heat;
   const double net_long_wave_rad = long_wave_rad_in - long_wave_rad_out;
   const double net_short_wave_rad = solar_rad * (1. - albedo);
-  const double net_rad = S.m_netrad[p] * (net_short_wave_rad + net_long_wave_rad);
+  const double net_rad = ci.s_m_netrad * (net_short_wave_rad + net_long_wave_rad);
   const double openWaterNetRad = solar_rad * (1. - kOpenWaterAlbedo) + net_long_wave_rad;
   // daily.cpp:476-522 Priestley-Taylor
   const double inc_svp = 4098. * e_s / (temp2 * temp2);
@@ -502,10 +684,10 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   double daily_runoff = 0., total_daily_runoff = 0., daily_gw_recharge = 0., pot_gw_recharge = 0.;
   double soil_water_overflow = 0., neg_land_aet = 0.;
   double dailyCellLandAET_uncorr = 0.;
-  double transfer = X.transfer[mp];
+  double transfer = ci.x_transfer;
 
   // interception, daily.cpp:825-895
-  double canopy = X.canopy[mp];
+  double canopy = ci.x_canopy;
   if (landAreaFrac <= 0.) {
     transfer = canopy;
     canopy = 0.;
@@ -514,7 +696,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     if (fabs(canopy) <= kMinStorVol) canopy = 0.;
     initialStorage = canopy;
     if (dailyLai > 0.00001) {
-      const double max_canopy_storage = S.mcwh[p] * dailyLai;
+      const double max_canopy_storage = ci.s_mcwh * dailyLai;
       const double canopy_deficiency = max_canopy_storage - canopy;
       if (dailyPrec < canopy_deficiency) {
         canopy += dailyPrec;
@@ -547,9 +729,9 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   // Sub-cells are stored in pairs ([member][pair][cell] x 2 doubles, elevations as
   // packed int16 pairs) and streamed in double-buffered chunks so that a
   // wavefront keeps many 16-byte loads in flight.
-  const double T_SNOWFZ = S.t_snowfz[p], T_SNOWMT = S.t_snowmt[p];
-  const double T_GRADNT = S.t_gradnt[p], DEGDAY_F = S.degday[p];
-  const double ddf = T.ddf[lct - 1];
+  const double T_SNOWFZ = ci.s_t_snowfz, T_SNOWMT = ci.s_t_snowmt;
+  const double T_GRADNT = ci.s_t_gradnt, DEGDAY_F = ci.s_degday;
+  const double ddf = ci.t_ddf;
   SnowAcc acc;
   acc.snow_sum = 0.;
   acc.eff = 0.;
@@ -557,13 +739,13 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   acc.change = 0.;
   acc.temp_max = 0.;
   acc.thresh = 0;
-  const int elev0 = S.elev0[p];
+  const int elev0 = ci.s_elev0;
   WgD2 *sp = X.snow2 + int64_t(m) * WG_DEV_NPAIR * n + p;
   const int32_t *ep = S.elev_pairs + p;
   // A cell whose sub-cell storages are all exactly zero (snow_any == 0) starts the
   // loop from zeros without reading them, and stores only storages that become
   // non-zero: on warm days such a cell moves no snow bytes at all.
-  const bool snow_any = X.snow_any[mp] != 0;
+  const bool snow_any = ci.x_snow_any != 0;
   if (landAreaFrac <= 0.) {
     for (int k = 0; k < WG_DEV_NPAIR; k++) {
       const WgD2 v = snow_any ? sp[int64_t(k) * n] : WgD2{0., 0.};
@@ -661,22 +843,23 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     acc.change /= 100.;
     landStorageChangeSum += acc.change;
   }
+  load_cell_in_b(S, X, mp, p, ci);
   const double snow_sum = acc.snow_sum;
   const double TempElevMax = acc.temp_max;
   dailyEffPrec = acc.eff;
   X.snow[mp] = snow_sum;
 
   // immediate runoff, daily.cpp:1066-1071
-  const float built_up = S.built_up[p];
+  const float built_up = ci.s_built_up;
   if (built_up > 0.) {
     immediate_runoff = kRunoffFracBuiltUp * dailyEffPrec * built_up;
     dailyEffPrec -= immediate_runoff;
   }
 
   // soil, runoff, AET, recharge, daily.cpp:1080-1239
-  const double cfa = X.cfa[mp];
-  const float smax = X.smax[mp];
-  double soil = X.soil[mp];
+  const double cfa = ci.x_cfa;
+  const float smax = ci.x_smax;
+  double soil = ci.x_soil;
   if (landAreaFrac <= 0.) {
     transfer += soil;
     transfer *= cfa;
@@ -694,7 +877,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     if (TempElevMax > T_SNOWFZ) {
       if (smax > 0.) {
         const double soil_saturation = soil / smax;
-        daily_runoff = dailyEffPrec * wg_pow(soil_saturation, X.gamma[mp]);
+        daily_runoff = dailyEffPrec * wg_pow(soil_saturation, ci.x_gamma);
         if (dailySoilPET > (maxDailyPET - dailyCanopyEvapo) * soil_saturation)
           dailyAET = (maxDailyPET - dailyCanopyEvapo) * soil_saturation;
         else
@@ -708,15 +891,15 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
         }
         daily_runoff *= cfa;
         immediate_runoff *= cfa;
-        const double rgmax = S.rgmax[p] / 100.;
-        const float fg = S.gw_factor[p];
+        const double rgmax = ci.s_rgmax / 100.;
+        const float fg = ci.s_gw_factor;
         if (rgmax < (fg * daily_runoff))
           daily_gw_recharge = rgmax;
         else
           daily_gw_recharge = fg * daily_runoff;
         pot_gw_recharge = 0.;
-        if (((arid_gw) && (S.texture[p] < 21)) && (S.ldd[p] >= 0)) {
-          if (dailyPrec <= S.pcritgwa[p]) {
+        if (((arid_gw) && (ci.s_texture < 21)) && (ci.s_ldd >= 0)) {
+          if (dailyPrec <= ci.s_pcritgwa) {
             pot_gw_recharge = daily_gw_recharge;
             daily_gw_recharge = 0.;
           }
@@ -771,11 +954,11 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   // ------------------------------------------------------------------
   // routing without upstream water, routing.cpp:1877-2617
   // ------------------------------------------------------------------
-  const double cellArea = S.cell_area[p];
-  const double cf = S.contfreq[p];
-  const int ldd = S.ldd[p];
+  const double cellArea = ci.s_cell_area;
+  const double cf = ci.s_contfreq;
+  const int ldd = ci.s_ldd;
   const double laf = landAreaFrac;      // G_landAreaFrac of the day
-  const double laf_prev = X.laf_prev[mp];
+  const double laf_prev = pd.laf_prev;
   const double ow_prec = dailyPrec, ow_pet = dailyOpenWaterPET;
   double dailyLocalSurfaceRunoff, fswb_catchment = 0.;
   double localRunoff = 0., lrir = 0., lgwrir = 0., recharge_km3 = 0.;
@@ -784,19 +967,19 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   else
     dailyLocalSurfaceRunoff = surf_runoff * cellArea / 1000000. * laf / 100.;
   if (ldd >= 0) {
-    fswb_catchment = S.fswb_init[p] * 20.;
+    fswb_catchment = ci.s_fswb_init * 20.;
     if (fswb_catchment > 1.) fswb_catchment = 1.;
     lrir = (1. - fswb_catchment) * dailyLocalSurfaceRunoff;
   }
-  const bool arid = (S.aindex[p] == 1);
+  const bool arid = (ci.s_aindex == 1);
   if (arid && (ldd >= 0)) {
     localRunoff = fswb_catchment * dailyLocalSurfaceRunoff;
     recharge_km3 = recharge_handoff * cellArea * (laf / 100.) / 1000000.;
   } else {
     // humid cells (routing.cpp:1968-2039) and inland sinks (2112-2159)
-    const double gw_prev = X.gw[mp];
+    const double gw_prev = ci.x_gw;
     const double netGWin = recharge_handoff * cellArea * (laf / 100.) / 1000000.;
-    double gw = gw_prev * S.e_gw[p] + S.inv_kgw[p] * netGWin * S.ome_gw[p];
+    double gw = gw_prev * ci.s_e_gw + ci.s_inv_kgw * netGWin * ci.s_ome_gw;
     if (fabs(gw) <= kMinStorVol) gw = 0.;
     double gw_runoff = gw_prev - gw + netGWin;
     if (gw_runoff <= 0.) {
@@ -819,15 +1002,15 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   }
 
   double inflow = localRunoff, outflow = 0.;
-  const double k_sw = S.k_sw[p];
-  const double evarexp = S.evarexp[p];
+  const double k_sw = ci.s_k_sw;
+  const double evarexp = ci.s_evarexp;
   const bool arid_swb = arid && (ldd >= 0);
   double gwr_loclak = 0., gwr_locwet = 0.;
   // local lake, routing.cpp:2316-2490
-  const double loc_lake = S.loc_lake[p];
+  const double loc_lake = ci.s_loc_lake;
   if (loc_lake > 0.) {
-    const double prev = X.loc_lake[mp], rf = X.red_ll[mp];
-    const double maxStorage = ((loc_lake) / 100.) * cellArea * S.lake_depth[p];
+    const double prev = ci.x_loc_lake, rf = pd.red_ll;
+    const double maxStorage = ((loc_lake) / 100.) * cellArea * ci.s_lake_depth;
     double evap = ((1.0 - cfa) * ow_prec * rf) + (cfa * (ow_pet * rf));
     if (evap < 0.) evap = 0.;
     const double totalInflow = inflow + (ow_prec * rf) * (cellArea / 1000000.) * (loc_lake / 100.);
@@ -866,10 +1049,10 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     X.red_ll[mp] = r;
   }
   // local wetland, routing.cpp:2495-2617
-  const double loc_wetland = S.loc_wetland[p];
+  const double loc_wetland = ci.s_loc_wetland;
   if (loc_wetland > 0.) {
-    const double prev = X.loc_wetl[mp], rf = X.red_lw[mp];
-    const double maxStorage = ((loc_wetland) / 100.) * cellArea * S.wetl_depth[p];
+    const double prev = ci.x_loc_wetl, rf = pd.red_lw;
+    const double maxStorage = ((loc_wetland) / 100.) * cellArea * ci.s_wetl_depth;
     double evap = ((1.0 - cfa) * ow_prec * rf) + (cfa * (ow_pet * rf));
     if (evap < 0.) evap = 0.;
     const double totalInflow = inflow + (ow_prec * rf * (cellArea / 1000000.) * (loc_wetland / 100.));
@@ -905,12 +1088,12 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
 
   // record of the routing terms that do not depend on the inflow from upstream
   // (routing.cpp:2630-3514)
-  const WgChainLink lk = load_link(S, p);
+  const WgChainLink lk = ci.link;
   ChainRec c;
   double *v = c.v;
-  const double e_sw = S.e_sw[p];
+  const double e_sw = ci.s_e_sw;
   {
-    const WgChainStatic cs = load_cstat(S, p);
+    const WgChainStatic &cs = ci.cstat;
     c.inv_ksw = cs.inv_ksw;
     c.ome_sw = cs.ome_sw;
     c.inv_kgw = cs.inv_kgw;
@@ -921,12 +1104,12 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   }
   {
     // river velocity and storage terms, routing.cpp:3424-3478
-    const double L = S.river_length[p];
-    const double prev = X.river[mp];
-    const double velocity = new_river_velocity(S.vel_c1[p], S.vel_c2[p], S.bottom_width[p], prev, L);
+    const double L = ci.s_river_length;
+    const double prev = ci.x_river;
+    const double velocity = new_river_velocity(ci.s_vel_c1, ci.s_vel_c2, ci.s_bottom_width, prev, L);
     const double K = velocity / L;
     const double eK = wg_exp(-1. * K);
-    const double riverAreaFrac = X.river_frac_next[mp];
+    const double riverAreaFrac = pd.river_frac_next;
     v[CF_INFLOW] = inflow;
     v[CF_LRIR] = lrir;
     v[CF_LGWRIR] = lgwrir;
@@ -942,7 +1125,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   v[CF_OW_PET] = ow_pet;
   {
     // global lake, routing.cpp:2633-2676
-    const double lake_area = S.lake_area[p];
+    const double lake_area = ci.s_lake_area;
     v[CF_GL_LP] = 0.;
     v[CF_GL_USE] = 0.;
     v[CF_GL_MAX] = 0.;
@@ -950,11 +1133,11 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     v[CF_GL_S0E] = 0.;
     v[CF_GL_GWR] = 0.;
     if (lake_area > 0.) {
-      const double prev = X.glo_lake[mp], rf = X.red_gl[mp];
+      const double prev = ci.x_glo_lake, rf = pd.red_gl;
       double evap = ((1.0 - cfa) * ow_prec) + (cfa * ow_pet * rf);
       if (evap < 0.) evap = 0.;
       const double gwr = arid_swb ? kKswbgw * rf * (lake_area / (cellArea * (cf / 100.))) : 0.;
-      v[CF_GL_MAX] = lake_area * S.lake_depth[p];
+      v[CF_GL_MAX] = lake_area * ci.s_lake_depth;
       v[CF_GL_LP] = (ow_prec * (lake_area / 1000000.));
       v[CF_GL_GWR] = gwr;
       v[CF_GL_USE] = evap * (lake_area / 1000000.) + gwr * cellArea * (cf / 100.) / 1000000. + 0.;
@@ -964,7 +1147,7 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
   }
   {
     // global wetland, routing.cpp:3181-3222
-    const double glo_wetland = S.glo_wetland[p];
+    const double glo_wetland = ci.s_glo_wetland;
     v[CF_GW_WP] = 0.;
     v[CF_GW_PET] = 0.;
     v[CF_GW_MAX] = 0.;
@@ -972,11 +1155,11 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     v[CF_GW_S0E] = 0.;
     v[CF_GW_GWR] = 0.;
     if (glo_wetland > 0) {
-      const double prev = X.glo_wetl[mp], rf = X.red_gw[mp];
+      const double prev = ci.x_glo_wetl, rf = pd.red_gw;
       double evap = ((1.0 - cfa) * (ow_prec * rf)) + (cfa * (ow_pet * rf));
       if (evap < 0.) evap = 0.;
       const double gwr = arid_swb ? kKswbgw * rf * glo_wetland / 100. / (cf / 100.) : 0.;
-      v[CF_GW_MAX] = ((glo_wetland) / 100.) * cellArea * S.wetl_depth[p];
+      v[CF_GW_MAX] = ((glo_wetland) / 100.) * cellArea * ci.s_wetl_depth;
       v[CF_GW_WP] = (ow_prec * rf * (cellArea / 1000000.) * (glo_wetland / 100.));
       v[CF_GW_GWR] = gwr;
       v[CF_GW_PET] = evap * (cellArea / 1000000.) * ((glo_wetland) / 100.) + gwr * cellArea * (cf / 100.) / 1000000.;
@@ -989,20 +1172,20 @@ __global__ __launch_bounds__(256) void wg_cell_kernel(WgStatic S, WgState X, WgT
     v[CF_A_GWR_LL] = gwr_loclak;
     v[CF_A_GWR_LW] = gwr_locwet;
     v[CF_A_RECH] = recharge_km3;
-    const double gprev = arid_swb ? X.gw[mp] : 0.;
+    const double gprev = arid_swb ? ci.x_gw : 0.;
     v[CF_A_PREV] = gprev;
-    v[CF_A_S0E] = gprev * S.e_gw[p];
+    v[CF_A_S0E] = gprev * ci.s_e_gw;
   }
   {
     // reservoir, routing.cpp:2807-2860 inputs
     const bool res = (lk.flags & CFL_RES) != 0;
-    v[CF_RES_PREV] = res ? X.res[mp] : 0.;
-    v[CF_RES_RED] = res ? X.red_res[mp] : 0.;
+    v[CF_RES_PREV] = res ? ci.x_res : 0.;
+    v[CF_RES_RED] = res ? pd.red_res : 0.;
     v[CF_RES_CFA] = cfa;
-    v[CF_RES_KREL] = res ? X.k_release[mp] : 0.;
-    v[CF_RES_CAP] = res ? S.stor_cap[p] : 0.;
-    v[CF_RES_QMEAN] = res ? S.mean_outflow[p] : 0.;
-    v[CF_RES_AREA] = res ? S.reservoir_area[p] : 0.;
+    v[CF_RES_KREL] = res ? ci.x_k_release : 0.;
+    v[CF_RES_CAP] = res ? ci.s_stor_cap : 0.;
+    v[CF_RES_QMEAN] = res ? ci.s_mean_outflow : 0.;
+    v[CF_RES_AREA] = res ? ci.s_reservoir_area : 0.;
   }
   if (lk.nu == 0) {
     // no upstream cell: G_riverInflow[n] is still 0 when the cell is routed

@@ -27,7 +27,7 @@
 #include "wg_libm_tables.h"
 
 #if defined(__HIPCC__) || defined(__HIP__)
-#define WG_FN __host__ __device__ static inline
+#define WG_FN __host__ __device__ static inline __attribute__((always_inline))
 #else
 #define WG_FN static inline
 #endif
