@@ -269,7 +269,9 @@ int wg_last_timing(wg_handle *h, double *ms, int64_t *n_launches);
 int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double *ms_per_day);
 /* Sets the launch configuration directly (cell threads 64..256 in steps of 64,
  * chain threads 64..256 (mode 3: 512 runs eight wavefronts, less four), snow
- * chunk 5 or 10, chain mode 0 level walk loading one level ahead, 1 dataflow
+ * chunk 5 or 10 (the chunk-5 cell kernel is held to three wavefronts per SIMD,
+ * which suits many members; the chunk-10 one is not), chain mode 0 level walk
+ * loading one level ahead, 1 dataflow
  * walk without per-level barriers, 2 level walk loading two levels ahead, 3 wave
  * walk in batches of 64 cells, the default). */
 int wg_set_launch_config(wg_handle *h, int32_t cell_threads, int32_t chain_threads, int32_t snow_chunk,
