@@ -168,13 +168,7 @@ def test_members_params_and_perturbation(wg, oracle, regional_world):
     smax = (np.tile(w.smax, (M, 1)) * rng.uniform(0.8, 1.2, (M, n))).astype(np.float32)
     seeds = np.array([11, 12, 13], dtype=np.uint64)
     f = w.forcing(seed=0, day0=0, n_days=n_days)
-    m = wg.WaterBalanceModel(w, n_members=M)
-    m.set_member_params(gamma, cfa, smax)
-    m.set_forcing_perturbation(seeds, 0.2, 1.0)
-    m.set_stations(w.basin_outlet)
-    m.load_forcing(f)
-    m.step_days(0, 1, n_days)
-    tot, _ = m.station_discharge()
+    refs = []
     for k in range(M):
         o = oracle.Oracle(w)
         o.set_params(gamma[k], cfa[k], smax[k])
@@ -182,8 +176,21 @@ def test_members_params_and_perturbation(wg, oracle, regional_world):
         for d in range(n_days):
             t, p = oracle.perturb_forcing(int(seeds[k]), d, f[d, 0], f[d, 1], 0.2, 1.0)
             o.step_day(t, p, f[d, 2], f[d, 3], d + 1)
-        assert_close(m.get_state(k), o.get_state(), label=f"m{k}:")
-        assert_close({"discharge": tot[k]}, {"discharge": o.station_discharge()}, label=f"m{k}:")
+        refs.append((o.get_state(), o.station_discharge()))
+    # both cell-kernel builds: snow chunks of 5 pairs (held to three wavefronts per
+    # SIMD) and of 10
+    for chunk in (5, 10):
+        m = wg.WaterBalanceModel(w, n_members=M)
+        m.set_launch_config(256, 256, chunk, 3)
+        m.set_member_params(gamma, cfa, smax)
+        m.set_forcing_perturbation(seeds, 0.2, 1.0)
+        m.set_stations(w.basin_outlet)
+        m.load_forcing(f)
+        m.step_days(0, 1, n_days)
+        tot, _ = m.station_discharge()
+        for k in range(M):
+            assert_close(m.get_state(k), refs[k][0], label=f"chunk {chunk} m{k}:")
+            assert_close({"discharge": tot[k]}, {"discharge": refs[k][1]}, label=f"chunk {chunk} m{k}:")
 
 
 def test_global_grid_20_days(wg, oracle, global_world):
