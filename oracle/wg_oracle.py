@@ -26,10 +26,11 @@ _LIBS = {}
 
 def load(variant="wg"):
     """The oracle library: variant "wg" uses the deterministic exp/pow shared with
-    the kernels, "glibc" the host C library's."""
+    the kernels, "glibc" the host C library's; any other value is the path of a
+    library built from a modified copy of wg_oracle.c (the mutation tests)."""
     if variant in _LIBS:
         return _LIBS[variant]
-    path = LIB_PATH if variant == "wg" else GLIBC_LIB_PATH
+    path = {"wg": LIB_PATH, "glibc": GLIBC_LIB_PATH}.get(variant, variant)
     if not os.path.exists(path):
         build()
     lib = C.CDLL(path)
@@ -70,6 +71,26 @@ def load(variant="wg"):
     lib.wgo_pow.restype = C.c_double
     _LIBS[variant] = lib
     return lib
+
+
+def build_variant(replacements, out_path, glibc=False):
+    """Builds the oracle from a copy of wg_oracle.c with each (old, new) text
+    replacement applied (every `old` must occur exactly once) into out_path: the
+    mutants that show a pinning check fails when the arithmetic it pins is wrong."""
+    src = open(os.path.join(HERE, "wg_oracle.c")).read()
+    for old, new in replacements:
+        if src.count(old) != 1:
+            raise ValueError(f"replacement target occurs {src.count(old)} times: {old!r}")
+        src = src.replace(old, new)
+    c_path = out_path[:-3] + ".c"
+    with open(c_path, "w") as f:
+        f.write(src)
+    cmd = ["gcc", "-O2", "-std=c99", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-w",
+           "-I", HERE, "-shared", "-o", out_path, c_path, "-lm"]
+    if glibc:
+        cmd.insert(1, "-DWG_ORACLE_GLIBC")
+    subprocess.run(cmd, check=True)
+    return out_path
 
 
 def rout_order(inflow9, outflow):
