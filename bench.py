@@ -115,6 +115,35 @@ def ensemble_run(wg, world, forcing, members, serial_members):
             "speedup_vs_serial": round(serial_per_member * members / batched_s, 2)}
 
 
+def ensemble_counter_run(args):
+    """The batched ensemble (members of the global grid with their own forcing
+    perturbation, gamma and Smax) launched eagerly for a few days, with per-kernel
+    HIP-event durations: the workload of the ensemble's counter profiles."""
+    import numpy as np
+    import watergap2_amd as wg
+    world = wg.World(wg.global_spec(seed=1))
+    n, members, days = world.n, args.ensemble_members, args.ensemble_counters
+    rng = np.random.default_rng(2024)
+    gamma = np.tile(world.params[wg.PARAM_INDEX["P_GAMRUN_C"]], (members, 1)) * rng.uniform(0.8, 1.2, (members, n))
+    smax = (np.tile(world.smax, (members, 1)) * rng.uniform(0.8, 1.2, (members, n))).astype(np.float32)
+    m = wg.WaterBalanceModel(world, n_members=members)
+    m.set_member_params(gamma=gamma, smax=smax)
+    m.set_forcing_perturbation(np.arange(1000, 1000 + members, dtype=np.uint64), 0.2, 1.0)
+    m.load_forcing(world.forcing(seed=1, day0=0, n_days=days))
+    info = m.schedule_info()
+    m.step_days(0, 1, days, profiled=True)
+    ms, launches = m.timing()
+    cell_ms, chain_ms = ms[0] / launches[0], ms[1] / launches[1]
+    l0 = info["level0_cells"]
+    out = {"members": members, "days": days}
+    for name, kms, kb in (("wg_cell_kernel", cell_ms, cell_kernel_bytes(n, members, l0)),
+                          ("wg_chain_kernel", chain_ms, chain_kernel_bytes(n, members, l0))):
+        out[name] = {"avg_launch_us": round(kms * 1e3, 2), "algorithmic_bytes_per_launch": kb,
+                     "achieved_GBs": round(kb / (kms * 1e-3) / 1e9, 1)}
+    m.close()
+    return out
+
+
 def calibration_run(args):
     """Metric 2: basins calibrated per hour at the 1 % criterion, plain search.
 
@@ -317,6 +346,9 @@ def main():
     ap.add_argument("--sur-candidates", type=int, default=40)
     ap.add_argument("--sur-epochs", type=int, default=30)
     ap.add_argument("--sur-eval-basins", type=int, default=500)
+    ap.add_argument("--ensemble-counters", type=int, default=0,
+                    help="only run this many eagerly launched days of the batched ensemble (counter runs) "
+                         "and exit")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this size (counter calibration) and exit")
     args = ap.parse_args()
@@ -331,6 +363,9 @@ def main():
         print(json.dumps({"copy_probe_bytes": a.numel() * 4, "launches": 3}))
         return
 
+    if args.ensemble_counters:
+        print(json.dumps(ensemble_counter_run(args)))
+        return
     if args.calibration:
         print(json.dumps(calibration_run(args)))
         return

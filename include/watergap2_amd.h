@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define WG_ABI_VERSION 1
+#define WG_ABI_VERSION 2
 
 /* error codes */
 #define WG_OK 0
@@ -118,6 +118,11 @@ typedef struct wg_static_fields {
   const int32_t *downstream;   /* G_OUTFLC: 1-based downstream cell, 0 = none */
   const int32_t *rout_order;   /* G_ROUT_ORDER: 1-based position of each cell in the routing loop */
   const wg_lct_tables *lct;
+  /* mean demand of irrigation reservoirs (res_type 1), routing.cpp:344-403; both may be NULL */
+  const double *mean_nus;      /* G_NUs_<first>_<last>.UNF0: mean annual net surface-water use, m3/yr
+                                  (NULL: no demand, so type-1 reservoirs release like type 2) */
+  const double *alloc_coeff;   /* [N][5] G_ALLOC_COEFF.5.UNF0 (rout_prepare.cpp:895-944, wg_alloc_coeff);
+                                  NULL: 1 for every cell */
 } wg_static_fields;
 
 /* Option profile.  Only the values below are implemented; wg_create rejects others. */
@@ -197,11 +202,100 @@ int wg_inflow_outflow(int32_t n_cells, const int8_t *ldd, const int16_t *row, co
 int wg_rout_order(int32_t n_cells, const int32_t *inflow9, const int32_t *outflow,
                   int32_t *rout_order, int32_t *level);
 
+/* ---- routing preprocessing (host): prepare_routing_files, rout_prepare.cpp:61-336 ----
+ * Cell arrays in the reference's cell order; cell numbers in inflow9 / outflow are
+ * 1-based with 0 = none, as in G_INFLC.9.UNF4 / G_OUTFLC.UNF4. */
+
+/* Arc/Info flow codes (G_FLOWDIR.UNF2: 1 E, 2 SE, 4 S, 8 SW, 16 W, 32 NW, 64 N,
+ * 128 NE, 0 sink, -1 inland sink) to WaterGAP LDD codes; any other value is a sink.
+ * Replaces rout_prepare.cpp:68-112. */
+int wg_ldd_from_flowdir(int32_t n_cells, const int16_t *flowdir, int8_t *ldd);
+/* Cells whose flow direction points at a position without land become sinks
+ * (LDD 5).  Replaces rout_prepare.cpp:287-336 (replace_flow_to_ocean, which the
+ * reference keeps but prepare_routing_files does not call). */
+int wg_replace_flow_to_ocean(int32_t n_cells, const int16_t *row, const int16_t *col, const int32_t *cell_at,
+                             int8_t *ldd);
+/* Number of cells draining through each cell, itself included (G_FLOW_ACC.UNF2,
+ * 16-bit); 0 for cells on or below a loop.  Replaces rout_prepare.cpp:441-494. */
+int wg_flow_accumulation(int32_t n_cells, const int32_t *inflow9, int16_t *flow_acc);
+/* Basin number of every cell (endpoints LDD 5 / -1 numbered 1.. in cell order,
+ * G_BASINS.UNF2) and the number of cells to the outlet (G_CELLS_TO_OUTLET.UNF2,
+ * may be NULL); LDD codes 0 and -99 become 5, as in the reference.  Replaces
+ * rout_prepare.cpp:497-571. */
+int wg_derive_basins(int32_t n_cells, int8_t *ldd, const int32_t *inflow9, uint16_t *basin, uint16_t *dist,
+                     int32_t *n_basins);
+/* Renumbers the basins of more than one cell 1..n_multi and sets the one-cell
+ * basins to 0 (G_BASINS_2.UNF2).  Replaces rout_prepare.cpp:573-604. */
+int wg_reindex_basins(int32_t n_cells, int32_t n_basins, uint16_t *basin, int32_t *n_multi);
+/* The 8 neighbour cells of each cell (E, NE, N, NW, W, SW, S, SE; 0 = none,
+ * G_NEIGHBOUR_CELLS.8.UNF4).  Replaces rout_prepare.cpp:672-692. */
+int wg_neighbour_cells(int32_t n_cells, const int16_t *row, const int16_t *col, const int32_t *cell_at,
+                       int32_t *neighbour8);
+/* Distances (km) from a cell of row r (1-based) to its neighbour in LDD direction
+ * k (1..9): dist[(k-1)*360 + (r-1)] (GCELLDIST.9.UNF0).  Replaces
+ * rout_prepare.cpp:694-752. */
+int wg_cell_distances(float *dist);
+/* River slope (m/m, floor 1e-5) along the flow direction from GALTMOD.UNF0 and the
+ * meandering ratio (G_RIVERSLOPE.UNF0).  Replaces rout_prepare.cpp:754-788. */
+int wg_river_slope(int32_t n_cells, const float *meander, const float *dist, const float *altitude,
+                   const int32_t *outflow, const int8_t *ldd, const int16_t *row, float *slope);
+/* River length (km) with the meandering ratio (G_RIVER_LENGTH.UNF0).  Replaces
+ * rout_prepare.cpp:790-832. */
+int wg_river_length(int32_t n_cells, const float *meander, const int16_t *row, const int16_t *col, const float *dist,
+                    const int32_t *outflow, float *length);
+/* Allocation coefficients [N][5] of the up to 5 cells downstream of each
+ * reservoir (G_ALLOC_COEFF.5.UNF0; the reference stores them as float); 1 where no
+ * reservoir allocates.  mean_outflow as read from G_MEAN_OUTFLOW.UNF0.  Replaces
+ * rout_prepare.cpp:895-944. */
+int wg_alloc_coeff(int32_t n_cells, const int32_t *outflow, const double *reservoir_area, const double *mean_outflow,
+                   double *alloc_coeff);
+/* Start month (1..12) of the reservoirs' operational year from the monthly and
+ * annual mean outflow (G_START_MONTH.UNF1).  Replaces rout_prepare.cpp:950-1019. */
+int wg_reservoir_start_month(int32_t n_cells, const double *mean_outflow_monthly, const double *mean_outflow,
+                             int8_t *start_month);
+
+/* Inputs and outputs of the whole preprocessing step (file names of the reference). */
+typedef struct wg_routing_inputs {
+  int32_t n_cells;
+  const int16_t *flowdir;       /* G_FLOWDIR.UNF2 (Arc/Info codes), or NULL to use ldd */
+  const int8_t *ldd;            /* G_LDD.UNF1 when flowdir is NULL */
+  const int16_t *row, *col;     /* GR.UNF2, GC.UNF2 */
+  const int32_t *cell_at;       /* GCRC.UNF4 as [row-1][col-1] of 360 x 720, 1-based cell, 0 = ocean */
+  const float *altitude;        /* GALTMOD.UNF0 */
+  const float *meander;         /* G_MEANDERING_RATIO.UNF0 */
+  const double *reservoir_area; /* G_RESAREA.UNF0 (NULL: no reservoir outputs) */
+  const double *mean_outflow;   /* G_MEAN_OUTFLOW.UNF0 */
+  const double *mean_outflow_monthly; /* G_MEAN_OUTFLOW.12.UNF0 [N][12] */
+} wg_routing_inputs;
+
+typedef struct wg_routing_files {
+  int8_t *ldd;                  /* flow directions after the loop and no-data corrections */
+  int32_t *inflow9;             /* G_INFLC.9.UNF4 [N][9] */
+  int32_t *outflow;             /* G_OUTFLC.UNF4 */
+  int16_t *flow_acc;            /* G_FLOW_ACC.UNF2 */
+  uint16_t *basins;             /* G_BASINS_2.UNF2 */
+  uint16_t *cells_to_outlet;    /* G_CELLS_TO_OUTLET.UNF2 (may be NULL) */
+  float *cell_dist;             /* GCELLDIST.9.UNF0 [9][360] (may be NULL) */
+  float *river_slope;           /* G_RIVERSLOPE.UNF0 (may be NULL) */
+  float *river_length;          /* G_RIVER_LENGTH.UNF0 (may be NULL) */
+  int32_t *rout_order;          /* G_ROUT_ORDER.UNF4 */
+  double *alloc_coeff;          /* G_ALLOC_COEFF.5.UNF0 [N][5] (may be NULL) */
+  int8_t *start_month;          /* G_START_MONTH.UNF1 (may be NULL) */
+  int32_t n_basins;             /* basins of more than one cell */
+} wg_routing_files;
+
+/* prepare_routing_files (rout_prepare.cpp:61-336): flow directions, inflow cells,
+ * flow accumulation with the loop correction, basins, outflow cells, distances,
+ * slope, length, routing order and, with reservoir inputs, the allocation
+ * coefficients and start months. */
+int wg_prepare_routing(const wg_routing_inputs *in, wg_routing_files *out);
+
 /* ---- model ---- */
 
 /* Creates a model of n_members members sharing the static fields.  Reservoirs
  * with reservoir_area > 0 are in operation from the first day (resYearOpt = 1,
- * routing.cpp:5647-5720).  1 <= n_cells < 2^28, 1 <= n_members <= 65535. */
+ * routing.cpp:5647-5720).  1 <= n_cells < 2^27; the cells that drain to one outlet, packed
+ * into routing groups, must fit the LDS limit of 18,204 cells per group, 1 <= n_members <= 65535. */
 int wg_create(int32_t n_cells, const wg_static_fields *st, const wg_options_profile *opt,
               int32_t n_members, wg_handle **out);
 int wg_destroy(wg_handle *h);

@@ -58,6 +58,7 @@ struct wgo_model {
   double *river_length, *river_slope, *roughness, *width_bf, *bottom_width, *river_stor_max;
   double *lake_depth, *wetl_depth;   /* routing.cpp:5621-5644 */
   double *f_glo_lake, *fswb_init;    /* routing.cpp:745-765 */
+  double *mean_demand;               /* routing.cpp:360-403, m3/s */
   int32_t *routing_cell;              /* routing.cpp:530-538 */
   /* vertical state (daily.h) */
   double *canopy, *soil, *snow, *snow_elev, *transfer;
@@ -188,7 +189,7 @@ wgo_model *wgo_create(int32_t n, const wg_static_fields *st) {
   m->contcell = (short *)calloc((size_t)n, sizeof(short));
   ALLOC_D(m->river_length); ALLOC_D(m->river_slope); ALLOC_D(m->roughness); ALLOC_D(m->width_bf);
   ALLOC_D(m->bottom_width); ALLOC_D(m->river_stor_max); ALLOC_D(m->lake_depth); ALLOC_D(m->wetl_depth);
-  ALLOC_D(m->f_glo_lake); ALLOC_D(m->fswb_init);
+  ALLOC_D(m->f_glo_lake); ALLOC_D(m->fswb_init); ALLOC_D(m->mean_demand);
   m->routing_cell = (int32_t *)calloc((size_t)n, sizeof(int32_t));
   ALLOC_D(m->canopy); ALLOC_D(m->soil); ALLOC_D(m->snow); ALLOC_D(m->transfer);
   m->snow_elev = (double *)calloc((size_t)n * WG_NELEV, sizeof(double));
@@ -269,6 +270,28 @@ wgo_model *wgo_create(int32_t n, const wg_static_fields *st) {
     if (m->laf[i] < 0.) m->laf[i] = 0.;
     m->laf_prev[i] = 0.;
   }
+  /* routing.cpp:360-403: mean demand of the cell and of up to reservoir_dsc = 5
+   * downstream cells (m3/yr -> m3/s).  G_downstreamCell holds 1-based cell numbers
+   * and the loop tests G_reservoir_area_full[downstreamCell] with that number,
+   * i.e. the cell after the downstream cell; an index past the last cell is taken
+   * as a cell without a reservoir.  Allocation coefficients default to 1
+   * (rout_prepare.cpp:906-909). */
+  for (int32_t i = 0; i < n; i++) {
+    m->mean_demand[i] = 0;
+    if (st->mean_nus) {
+      short k = 0;
+      int32_t downstreamCell = st->downstream[i];
+      m->mean_demand[i] += st->mean_nus[i];
+      while (k < 5 && downstreamCell > 0 &&
+             (downstreamCell < n ? st->reservoir_area[downstreamCell] : 0.) <= 0) {
+        const double coeff = st->alloc_coeff ? st->alloc_coeff[(size_t)i * 5 + k] : 1.0;
+        k++;
+        m->mean_demand[i] += st->mean_nus[downstreamCell - 1] * coeff;
+        downstreamCell = st->downstream[downstreamCell - 1];
+      }
+    }
+    m->mean_demand[i] = m->mean_demand[i] / 31536000.;
+  }
   /* routing.cpp:530-538 */
   for (int32_t i = 0; i < n; i++) m->routing_cell[st->rout_order[i] - 1] = i + 1;
   m->started = 0;
@@ -279,7 +302,7 @@ void wgo_destroy(wgo_model *m) {
   if (!m) return;
   double **d[] = {&m->gamma, &m->cfa, &m->river_length, &m->river_slope, &m->roughness, &m->width_bf,
                   &m->bottom_width, &m->river_stor_max, &m->lake_depth, &m->wetl_depth, &m->f_glo_lake,
-                  &m->fswb_init, &m->canopy, &m->soil, &m->snow, &m->snow_elev, &m->transfer,
+                  &m->fswb_init, &m->mean_demand, &m->canopy, &m->soil, &m->snow, &m->snow_elev, &m->transfer,
                   &m->lai_precsum, &m->surf_runoff, &m->gw_recharge, &m->ow_prec, &m->ow_pet,
                   &m->aet_uncorr, &m->prec_out, &m->gw, &m->loc_lake, &m->loc_wetl, &m->glo_lake,
                   &m->glo_wetl, &m->res, &m->river, &m->red_ll, &m->red_lw, &m->red_gl, &m->red_gw,
@@ -960,8 +983,9 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
       }
     }
     if ((st->res_type[i] + 0) == 1) {
-      /* irrigation reservoir without water use: demand and downstream use are zero */
-      const double mean_demand = 0.;
+      /* irrigation reservoir; without water use the daily net use of the cell and
+       * of its downstream cells is 0 (routing.cpp:2972-2983) */
+      const double mean_demand = m->mean_demand[i];
       double monthlyUse = 0.0 * days_in_month[month];
       monthlyUse = monthlyUse * 1000000000. / (days_in_month[month] * 86400.);
       if (mean_demand >= 0.5 * qmean)

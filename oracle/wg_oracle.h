@@ -32,6 +32,20 @@ int wgo_inflow_outflow(int32_t n, const int8_t *ldd, const int16_t *row, const i
 int wgo_rout_order(int32_t n, const int32_t *inflow9, const int32_t *outflow, int32_t *order,
                    int32_t *level);
 
+/* Routing preprocessing (wg_prepare_oracle.c), rout_prepare.cpp:441-604, 694-832, 895-1019 */
+int wgo_flow_accumulation(int32_t n, const int32_t *inflow9, int16_t *acc);
+int wgo_derive_basins(int32_t n, int8_t *ldd, const int32_t *inflow9, uint16_t *ws, uint16_t *dist,
+                      int32_t *n_basins);
+int wgo_reindex_basins(int32_t n, int32_t nb, uint16_t *ws, int32_t *n_multi);
+void wgo_cell_distances(float *dist);
+void wgo_river_slope(int32_t n, const float *meander, const float *dist, const float *alt, const int32_t *outflow,
+                     const int8_t *ldd, const int16_t *row, float *slope);
+void wgo_river_length(int32_t n, const float *meander, const int16_t *row, const int16_t *col, const float *dist,
+                      const int32_t *outflow, float *len);
+void wgo_alloc_coeff(int32_t n, const int32_t *outflow, const double *res_area, const double *mean_outflow,
+                     double *alloc);
+void wgo_reservoir_start_month(int32_t n, const double *monthly, const double *annual, int8_t *start_month);
+
 /* Initialisation as routingClass::init + initFractionStatus + setStoragesToZero +
  * setLakeWetlToMaximum + annualInit (first year) + laiClass::init. */
 wgo_model *wgo_create(int32_t n, const wg_static_fields *st);

@@ -69,6 +69,20 @@ def load(variant="wg"):
         getattr(lib, fn).restype = C.c_double
     lib.wgo_pow.argtypes = [C.c_double, C.c_double]
     lib.wgo_pow.restype = C.c_double
+    P_u16 = C.POINTER(C.c_uint16)
+    lib.wgo_flow_accumulation.argtypes = [i32, P_i32, P_i16]
+    lib.wgo_derive_basins.argtypes = [i32, P_i8, P_i32, P_u16, P_u16, P_i32]
+    lib.wgo_reindex_basins.argtypes = [i32, i32, P_u16, P_i32]
+    lib.wgo_cell_distances.argtypes = [P_f]
+    lib.wgo_cell_distances.restype = None
+    lib.wgo_river_slope.argtypes = [i32, P_f, P_f, P_f, P_i32, P_i8, P_i16, P_f]
+    lib.wgo_river_slope.restype = None
+    lib.wgo_river_length.argtypes = [i32, P_f, P_i16, P_i16, P_f, P_i32, P_f]
+    lib.wgo_river_length.restype = None
+    lib.wgo_alloc_coeff.argtypes = [i32, P_i32, P_d, P_d, P_d]
+    lib.wgo_alloc_coeff.restype = None
+    lib.wgo_reservoir_start_month.argtypes = [i32, P_d, P_d, P_i8]
+    lib.wgo_reservoir_start_month.restype = None
     _LIBS[variant] = lib
     return lib
 
@@ -86,11 +100,84 @@ def build_variant(replacements, out_path, glibc=False):
     with open(c_path, "w") as f:
         f.write(src)
     cmd = ["gcc", "-O2", "-std=c99", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-w",
-           "-I", HERE, "-shared", "-o", out_path, c_path, "-lm"]
+           "-I", HERE, "-shared", "-o", out_path, c_path, os.path.join(HERE, "wg_prepare_oracle.c"), "-lm"]
     if glibc:
         cmd.insert(1, "-DWG_ORACLE_GLIBC")
     subprocess.run(cmd, check=True)
     return out_path
+
+
+def _p(arr, ct):
+    return _abi.ptr(arr, ct)
+
+
+def flow_accumulation(inflow9):
+    """rout_prepare.cpp:441-494 with the reference's sweeps."""
+    inflow9 = np.ascontiguousarray(inflow9, dtype=np.int32)
+    acc = np.zeros(len(inflow9), np.int16)
+    load().wgo_flow_accumulation(len(inflow9), _p(inflow9, _abi.P_i32), _p(acc, _abi.P_i16))
+    return acc
+
+
+def derive_basins(ldd, inflow9):
+    """rout_prepare.cpp:497-604: (ldd after the no-data fix, basins, cells to the
+    outlet, number of endpoints, basins renumbered, number of multi-cell basins)."""
+    P_u16 = C.POINTER(C.c_uint16)
+    ldd = np.array(ldd, dtype=np.int8)
+    inflow9 = np.ascontiguousarray(inflow9, dtype=np.int32)
+    n = len(ldd)
+    ws = np.zeros(n, np.uint16)
+    dist = np.zeros(n, np.uint16)
+    nb = C.c_int32()
+    lib = load()
+    lib.wgo_derive_basins(n, _p(ldd, _abi.P_i8), _p(inflow9, _abi.P_i32), _p(ws, P_u16), _p(dist, P_u16), C.byref(nb))
+    ws2 = ws.copy()
+    nm = C.c_int32()
+    lib.wgo_reindex_basins(n, nb.value, _p(ws2, P_u16), C.byref(nm))
+    return ldd, ws, dist, nb.value, ws2, nm.value
+
+
+def cell_distances():
+    d = np.zeros(9 * 360, np.float32)
+    load().wgo_cell_distances(_p(d, _abi.P_f))
+    return d
+
+
+def river_slope(meander, dist, altitude, outflow, ldd, row):
+    args = [np.ascontiguousarray(a, dtype=t) for a, t in ((meander, np.float32), (dist, np.float32),
+                                                           (altitude, np.float32), (outflow, np.int32),
+                                                           (ldd, np.int8), (row, np.int16))]
+    out = np.zeros(len(args[0]), np.float32)
+    cts = [_abi.P_f, _abi.P_f, _abi.P_f, _abi.P_i32, _abi.P_i8, _abi.P_i16]
+    load().wgo_river_slope(len(out), *[_p(a, c) for a, c in zip(args, cts)], _p(out, _abi.P_f))
+    return out
+
+
+def river_length(meander, row, col, dist, outflow):
+    args = [np.ascontiguousarray(a, dtype=t) for a, t in ((meander, np.float32), (row, np.int16), (col, np.int16),
+                                                           (dist, np.float32), (outflow, np.int32))]
+    out = np.zeros(len(args[0]), np.float32)
+    cts = [_abi.P_f, _abi.P_i16, _abi.P_i16, _abi.P_f, _abi.P_i32]
+    load().wgo_river_length(len(out), *[_p(a, c) for a, c in zip(args, cts)], _p(out, _abi.P_f))
+    return out
+
+
+def alloc_coeff(outflow, reservoir_area, mean_outflow):
+    outflow = np.ascontiguousarray(outflow, dtype=np.int32)
+    ra = np.ascontiguousarray(reservoir_area, dtype=np.float64)
+    mo = np.ascontiguousarray(mean_outflow, dtype=np.float64)
+    out = np.zeros((len(outflow), 5))
+    load().wgo_alloc_coeff(len(outflow), _p(outflow, _abi.P_i32), _p(ra, _abi.P_d), _p(mo, _abi.P_d),
+                           _p(out, _abi.P_d))
+    return out
+
+
+def reservoir_start_month(monthly, annual):
+    monthly = np.ascontiguousarray(monthly, dtype=np.float64)
+    annual = np.ascontiguousarray(annual, dtype=np.float64)
+    out = np.zeros(len(annual), np.int8)
+    load().wgo_reservoir_start_month(len(annual), _p(monthly, _abi.P_d), _p(annual, _abi.P_d), _p(out, _abi.P_i8))
+    return out
 
 
 def rout_order(inflow9, outflow):

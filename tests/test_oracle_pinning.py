@@ -378,8 +378,49 @@ def check_reservoir(wg, variant="wg"):
     close(after["reservoir"][c], 0.6 * cap - 4.0 * cr * cr * 0.3 * q * 86400.0 / 1e9)
 
 
+def check_irrigation_reservoir(wg, variant="wg"):
+    """Irrigation reservoir (type 1): the mean demand D is the cell's mean net
+    surface-water use plus that of up to 5 downstream cells (routing.cpp:360-403,
+    m3/yr / 31536000 s); with no daily use the provisional release is Q_mean / 2
+    when D >= Q_mean / 2, else Q_mean - D (routing.cpp:2990-2996).  The reference's
+    walk tests the reservoir area one cell past the downstream cell
+    (G_reservoir_area_full[downstreamCell] with a 1-based number, routing.cpp:372):
+    a reservoir two cells down stops the walk after the first."""
+    # strip: cell 0 -> cell 1 -> cell 2 (outlet); use on cells 0, 1, 2
+    cases = []
+    for share, stop_early in ((0.75, False), (0.2, False), (0.75, True)):
+        w, c = reservoir_world(wg, c_ratio=0.8, res_type=1)
+        q = w.mean_outflow[c]
+        nus = np.zeros(w.n)
+        nus[[0, 1, 2]] = np.array([0.2, 0.3, 0.5]) * share * q * 31536000.0
+        w.mean_nus = nus
+        if stop_early:
+            w.reservoir_area[2] = 0.01 * w.cell_area[2]
+            w.glo_res[2] = 1.0
+            w.stor_cap[2] = 0.01
+            w.mean_outflow[2] = 1.0
+            w.res_type[2] = 2
+            w.start_month[2] = 1
+        demand = (nus[0] if stop_early else nus[0] + nus[1] + nus[2]) / 31536000.0
+        prov = q / 2.0 if demand >= 0.5 * q else q - demand
+        o = wg_oracle.Oracle(w, variant)
+        st = dry_state(o)
+        s0 = 0.6 * w.stor_cap[c]
+        st["reservoir"][c] = s0
+        st["k_release"][c] = 0.3
+        after, _ = run_one_day(wg, w, st, day(w.n), 181, variant)
+        close(after["reservoir"][c], s0 - 0.3 * prov * 86400.0 / 1e9)
+        cases.append(prov / q)
+    # D = 0.75 Q, 0.2 Q, and 0.15 Q when the walk stops after the reservoir cell
+    np.testing.assert_allclose(cases, [0.5, 0.8, 0.85], rtol=1e-12)
+
+
 def test_reservoir_release_rules(wg):
     check_reservoir(wg)
+
+
+def test_irrigation_reservoir_demand(wg):
+    check_irrigation_reservoir(wg)
 
 
 # ---------------------------------------------------------------------------
@@ -705,6 +746,15 @@ MUTANTS = [
                                     "((2. * c_ratio * c_ratio) * m->k_release[i] * prov_rel)")], check_reservoir),
     ("reservoir reduction exponent", [("#define EVAPO_RED_EXP_RES 2.81383", "#define EVAPO_RED_EXP_RES 3.32193")],
      check_reservoir),
+    ("irrigation demand without the downstream cells",
+     [("        m->mean_demand[i] += st->mean_nus[downstreamCell - 1] * coeff;\n", "\n")],
+     check_irrigation_reservoir),
+    ("irrigation demand walk testing the downstream cell itself",
+     [("(downstreamCell < n ? st->reservoir_area[downstreamCell] : 0.) <= 0)",
+       "st->reservoir_area[downstreamCell - 1] <= 0)")], check_irrigation_reservoir),
+    ("irrigation release Q - D always",
+     [("      if (mean_demand >= 0.5 * qmean)\n        prov_rel", "      if (0)\n        prov_rel")],
+     check_irrigation_reservoir),
     ("local lake outflow not subtracted", [("    m->loc_lake[i] -= outflow;\n", "\n")], check_conservation),
     ("global wetland spill lost", [("      outflow += (m->glo_wetl[i] - maxStorage);\n", "\n")], check_conservation),
     ("reservoir negative storage not charged to outflow",

@@ -76,6 +76,24 @@ def test_edge_cases(wg, oracle, regional_edge_world):
     assert (regional_edge_world.smax < 0).any()
 
 
+def test_irrigation_reservoir_demand(wg, oracle):
+    """Irrigation reservoirs (type 1) release against the mean demand of their cell
+    and of up to five downstream cells (routing.cpp:360-403, 2972-2996), with the
+    allocation coefficients of rout_prepare.cpp:895-944 stored as float like the
+    reference's G_ALLOC_COEFF.5.UNF0."""
+    from watergap2_amd import prepare
+    print("case: edge-case grid with irrigation demand, 150 days", flush=True)
+    w = wg.World(wg.regional_spec(seed=5, nrows=30, ncols=40, edge_cases=True, glolake_every=20, res_every=25))
+    res = w.reservoir_area > 0
+    assert (w.res_type[res] == 1).sum() >= 5
+    q_typ = float(np.median(w.mean_outflow[res]))
+    rng = np.random.default_rng(3)
+    w.mean_nus = rng.uniform(0.0, 1.0, w.n) * q_typ * 31536000.0 / 2.0
+    w.alloc_coeff = prepare.alloc_coeff(w.downstream, w.reservoir_area, w.mean_outflow).astype(np.float32)
+    m, o = run_pair(wg, oracle, w, 150, seed=6, first_doy=1)
+    check_pair(m, o, 150)
+
+
 def test_random_initial_state(wg, oracle, regional_edge_world):
     """Random storages drive rarely taken branches: deep snow (threshold elevation),
     empty and negative lakes, saturated soils, land area fraction zero."""

@@ -96,6 +96,11 @@ class World:
         self.upstream_area = np.ctypeslib.as_array(w.upstream_area, shape=(n,)).copy()
         self.lct = _abi.LctTables()
         C.memmove(C.byref(self.lct), C.byref(w.lct), C.sizeof(_abi.LctTables))
+        # mean net surface-water use (m3/yr) and the allocation coefficients of the
+        # five downstream cells ([N][5]) behind the irrigation reservoirs' demand;
+        # None = no demand
+        self.mean_nus = None
+        self.alloc_coeff = None
 
     def __del__(self):
         try:
@@ -126,7 +131,11 @@ class World:
             if name == "lct":
                 s.lct = C.pointer(self.lct)
             else:
-                setattr(s, name, _abi.ptr(getattr(self, name), ct))
+                arr = getattr(self, name)
+                if arr is not None and name in ("mean_nus", "alloc_coeff"):
+                    arr = np.ascontiguousarray(arr, dtype=np.float64)
+                    setattr(self, name, arr)
+                setattr(s, name, _abi.ptr(arr, ct))
         return s
 
     def basin_cells(self, b):

@@ -57,7 +57,7 @@ STATIC_FIELDS = [
     ("lake_area", P_d), ("reservoir_area", P_d), ("glo_res", P_d), ("stor_cap", P_d),
     ("mean_outflow", P_d), ("res_type", P_i8), ("start_month", P_i8), ("river_length", P_f),
     ("river_slope", P_f), ("roughness", P_f), ("bankfull_flow", P_f), ("downstream", P_i32),
-    ("rout_order", P_i32), ("lct", C.POINTER(LctTables)),
+    ("rout_order", P_i32), ("lct", C.POINTER(LctTables)), ("mean_nus", P_d), ("alloc_coeff", P_d),
 ]
 
 
@@ -186,8 +186,36 @@ class GammaSearch(C.Structure):
                 ("measured_avg_adapt", C.c_float)]
 
 
+class RoutingInputs(C.Structure):
+    """wg_routing_inputs (include/watergap2_amd.h)."""
+    _fields_ = [("n_cells", C.c_int32), ("flowdir", P_i16), ("ldd", P_i8), ("row", P_i16), ("col", P_i16),
+                ("cell_at", P_i32), ("altitude", P_f), ("meander", P_f), ("reservoir_area", P_d),
+                ("mean_outflow", P_d), ("mean_outflow_monthly", P_d)]
+
+
+class RoutingFiles(C.Structure):
+    """wg_routing_files (include/watergap2_amd.h)."""
+    _fields_ = [("ldd", P_i8), ("inflow9", P_i32), ("outflow", P_i32), ("flow_acc", P_i16),
+                ("basins", C.POINTER(C.c_uint16)), ("cells_to_outlet", C.POINTER(C.c_uint16)),
+                ("cell_dist", P_f), ("river_slope", P_f), ("river_length", P_f), ("rout_order", P_i32),
+                ("alloc_coeff", P_d), ("start_month", P_i8), ("n_basins", C.c_int32)]
+
+
 def _declare(lib):
     i32 = C.c_int32
+    P_u16 = C.POINTER(C.c_uint16)
+    lib.wg_ldd_from_flowdir.argtypes = [i32, P_i16, P_i8]
+    lib.wg_replace_flow_to_ocean.argtypes = [i32, P_i16, P_i16, P_i32, P_i8]
+    lib.wg_flow_accumulation.argtypes = [i32, P_i32, P_i16]
+    lib.wg_derive_basins.argtypes = [i32, P_i8, P_i32, P_u16, P_u16, P_i32]
+    lib.wg_reindex_basins.argtypes = [i32, i32, P_u16, P_i32]
+    lib.wg_neighbour_cells.argtypes = [i32, P_i16, P_i16, P_i32, P_i32]
+    lib.wg_cell_distances.argtypes = [P_f]
+    lib.wg_river_slope.argtypes = [i32, P_f, P_f, P_f, P_i32, P_i8, P_i16, P_f]
+    lib.wg_river_length.argtypes = [i32, P_f, P_i16, P_i16, P_f, P_i32, P_f]
+    lib.wg_alloc_coeff.argtypes = [i32, P_i32, P_d, P_d, P_d]
+    lib.wg_reservoir_start_month.argtypes = [i32, P_d, P_d, P_i8]
+    lib.wg_prepare_routing.argtypes = [C.POINTER(RoutingInputs), C.POINTER(RoutingFiles)]
     lib.wg_last_error.restype = C.c_char_p
     lib.wg_last_error.argtypes = [C.c_void_p]
     lib.wg_default_options.argtypes = [C.POINTER(OptionsProfile)]
@@ -245,7 +273,10 @@ def _declare(lib):
 
 # Every symbol declared in include/watergap2_amd.h (checked by the CPU tests).
 ABI_SYMBOLS = [
-    "wg_default_options", "wg_inflow_outflow", "wg_rout_order", "wg_create", "wg_destroy",
+    "wg_default_options", "wg_inflow_outflow", "wg_rout_order",
+    "wg_ldd_from_flowdir", "wg_replace_flow_to_ocean", "wg_flow_accumulation", "wg_derive_basins", "wg_reindex_basins", "wg_neighbour_cells",
+    "wg_cell_distances", "wg_river_slope", "wg_river_length", "wg_alloc_coeff", "wg_reservoir_start_month",
+    "wg_prepare_routing", "wg_create", "wg_destroy",
     "wg_last_error", "wg_get_state", "wg_set_state", "wg_set_member_params",
     "wg_set_forcing_perturbation", "wg_load_forcing", "wg_set_stations", "wg_step_days",
     "wg_get_station_discharge", "wg_reset_station_discharge", "wg_get_day_outputs",

@@ -65,15 +65,11 @@ class BasinSet:
         self.basin_outlet = np.array(outlets, dtype=np.int32)
         self.basin = np.repeat(np.arange(self.n_basins, dtype=np.int32), sizes)
         self.max_level = max(w.max_level for w in self.worlds)
+        self.mean_nus = None
+        self.alloc_coeff = None
 
     def static_fields(self):
-        s = _abi.StaticFields()
-        for name, ct in _abi.STATIC_FIELDS:
-            if name == "lct":
-                s.lct = C.pointer(self.lct)
-            else:
-                setattr(s, name, _abi.ptr(getattr(self, name), ct))
-        return s
+        return World.static_fields(self)
 
     def forcing(self, seed, day0, n_days):
         """[n_days, 4, N] forcing: each basin's own synthetic series."""

@@ -235,6 +235,58 @@ extern "C" const char *wg_last_error(const wg_handle *h) { return h ? h->err.c_s
 
 namespace {
 
+// LDS of the wave walk: the transported volume (8 B) and the chain flags (1 B) of
+// every cell of the largest routing group.
+size_t wave_lds_bytes(const wg_handle *h) { return (size_t(h->max_group_cells) * 9 + 15) / 16 * 16; }
+
+// Provisional release (m3/s) of each reservoir cell for the release rule of
+// routing.cpp:2972-2999 without water use (the daily net use of the cell and of
+// its downstream cells is 0, so the month's use term is +0 and the value is the
+// same every day):
+//  - irrigation reservoirs (type 1): from the mean demand of the cell and of up to
+//    five downstream cells, routing.cpp:360-403: the cell's mean net surface-water
+//    use plus each downstream cell's weighted by its allocation coefficient,
+//    walking down until a cell with a reservoir, in m3/yr / 31536000 = m3/s.  The
+//    reference tests the reservoir area at G_reservoir_area_full[downstreamCell]
+//    with the 1-based downstream cell number, i.e. the area of the cell after the
+//    downstream cell (routing.cpp:372); the restatement keeps that and treats an
+//    index past the last cell as a cell without a reservoir.  Without net-use
+//    input the demand is 0.  Allocation coefficients default to 1, the value
+//    rout_prepare.cpp:906-909 gives cells that no reservoir allocates to.
+//  - other reservoirs (type 2): the mean outflow.
+std::vector<double> reservoir_prov_rel(const wg_static_fields *st, int32_t n) {
+  std::vector<double> out(n, 0.);
+  for (int32_t i = 0; i < n; i++) {
+    if (!(st->reservoir_area[i] > 0.)) continue;
+    const double qmean = st->mean_outflow[i];
+    if (st->res_type[i] == 2) {
+      out[i] = qmean;
+      continue;
+    }
+    double mean_demand = 0.;
+    if (st->mean_nus) {
+      mean_demand += st->mean_nus[i];
+      int k = 0;
+      int32_t dc = st->downstream[i];
+      while (k < 5 && dc > 0 && (dc < n ? st->reservoir_area[dc] : 0.) <= 0) {
+        mean_demand += st->mean_nus[dc - 1] * (st->alloc_coeff ? st->alloc_coeff[size_t(i) * 5 + k] : 1.0);
+        k++;
+        dc = st->downstream[dc - 1];
+      }
+    }
+    mean_demand = mean_demand / 31536000.;
+    // routing.cpp:2972-2996 with dailyUse = 0
+    const double days = 31.;
+    double monthly_use = 0.0 * days;
+    monthly_use = monthly_use * 1000000000. / (days * 86400.);
+    if (mean_demand >= 0.5 * qmean)
+      out[i] = qmean / 2. * (1. + monthly_use / mean_demand);
+    else
+      out[i] = qmean + monthly_use - mean_demand;
+  }
+  return out;
+}
+
 // Cell-kernel tiles of block_threads_cell cells: block 8j + x covers tile j of XCD bin x.
 void build_tiles(wg_handle *h) {
   const int nx = h->n_xcd, T = h->block_threads_cell;
@@ -416,9 +468,9 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
                          wg_handle **out) {
   if (!out) return WG_E_ARG;
   *out = nullptr;
-  // rows of fewer than 2^28 doubles: the level walk addresses a member's row with
-  // 32-bit byte offsets below 2^31
-  if (n <= 0 || n >= (1 << 28) || !st || !opt || n_members <= 0 || n_members > 65535) return WG_E_ARG;
+  // rows of fewer than 2^27 cells: the chain kernels address a member's row of
+  // doubles or 16-byte pairs with 32-bit byte offsets below 2^31
+  if (n <= 0 || n >= (1 << 27) || !st || !opt || n_members <= 0 || n_members > 65535) return WG_E_ARG;
   const void *req[] = {st->row, st->cell_area, st->contfreq, st->aindex, st->ldd, st->land_cover, st->built_up,
                        st->smax, st->gw_factor, st->rgmax, st->texture, st->elevation, st->params, st->glo_lake,
                        st->loc_lake, st->glo_wetland, st->loc_wetland, st->lake_area, st->reservoir_area,
@@ -465,9 +517,12 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     int32_t biggest = 0;
     for (int32_t gg = 0; gg < H->n_groups; gg++)
       biggest = std::max(biggest, H->level_off[H->group_first_level[gg + 1]] - H->level_off[H->group_first_level[gg]]);
-    if (size_t(biggest) * sizeof(double) > size_t(lds)) {
+    // a routing group's transported volumes (8 B per cell) and chain flags (1 B)
+    // live in LDS while the chain kernels walk it
+    if ((size_t(biggest) * 9 + 15) / 16 * 16 > size_t(lds)) {
       *out = h.release();
-      return fail(*out, WG_E_INPUT, "a basin of %d cells exceeds the %d bytes of LDS per workgroup", biggest, lds);
+      return fail(*out, WG_E_INPUT, "a routing group of %d cells exceeds the %d bytes of LDS per workgroup "
+                  "(largest basin too large)", biggest, lds);
     }
   }
   const double *par = st->params;
@@ -527,6 +582,8 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     rw_c2[i] = bottom[i] / 1000.;
     rw_c3[i] = width_bf[i] / 1000.;
   }
+
+  const std::vector<double> prov_rel = reservoir_prov_rel(st, n);
 
   // initial state (one member, reference order)
   std::vector<double> s_river(n), s_loc_lake(n), s_loc_wetl(n), s_glo_lake(n), s_glo_wetl(n), s_res(n),
@@ -590,6 +647,7 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     UPR(d, st->reservoir_area); S.reservoir_area = d;
     UPR(d, st->stor_cap); S.stor_cap = d;
     UPR(d, st->mean_outflow); S.mean_outflow = d;
+    UP(d, prov_rel); S.res_prov_rel = d;
     UP(d, river_length); S.river_length = d;
     UP(d, river_slope); S.river_slope = d;
     UP(d, vel_c1); S.vel_c1 = d;
@@ -680,11 +738,7 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
       for (int32_t p = a; p < b; p++) gstart[p] = a;
       H->max_group_cells = std::max(H->max_group_cells, b - a);
     }
-    if (H->max_group_cells > 32767) {
-      *out = h.release();
-      return fail(*out, WG_E_INPUT, "a routing group of %d cells exceeds the 32767-cell limit of the "
-                  "16-bit upstream indices (largest basin too large)", H->max_group_cells);
-    }
+
     std::vector<int32_t> off(n + 1, 0), idx;
     for (int32_t p = 0; p < n; p++)
       if (ups[p].size() > 8) {
@@ -1127,11 +1181,11 @@ int enqueue_block(wg_handle *h, int days, hipEvent_t *evs) {
     if (h->chain_mode == 3) {
       // the wave walk runs 8 wavefronts with a 512-thread configuration, else 4
       if (h->block_threads_chain >= 512)
-        hipLaunchKernelGGL(wg_chain_wave_kernel<8>, chgrid, dim3(512), size_t(h->max_group_cells) * sizeof(double),
+        hipLaunchKernelGGL(wg_chain_wave_kernel<8>, chgrid, dim3(512), wave_lds_bytes(h),
                            h->stream, h->S, h->X, h->T, h->d_level_off, h->d_group_first, h->d_wave_wide,
                            h->d_wave_item_off[1], h->d_wave_items[1], h->d_clock, k, slot);
       else
-        hipLaunchKernelGGL(wg_chain_wave_kernel<4>, chgrid, dim3(256), size_t(h->max_group_cells) * sizeof(double),
+        hipLaunchKernelGGL(wg_chain_wave_kernel<4>, chgrid, dim3(256), wave_lds_bytes(h),
                            h->stream, h->S, h->X, h->T, h->d_level_off, h->d_group_first, h->d_wave_wide,
                            h->d_wave_item_off[0], h->d_wave_items[0], h->d_clock, k, slot);
     }

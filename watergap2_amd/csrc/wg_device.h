@@ -47,7 +47,7 @@ struct alignas(16) WgChainStatic {
 // Stored as pairs of fields, [M][WG_DEV_NREC / 2][N] x 16 bytes, so that the lanes
 // of a wavefront, which hold consecutive cells, read and write whole cache lines
 // with 16-byte accesses.
-#define WG_DEV_NREC 36
+#define WG_DEV_NREC 38
 
 // Device layout of the routing links, [2][N] x 16 bytes: (flags, station,
 // nu | nu_level << 16, level), then four words of two upstream indices (up[2k] in
@@ -81,7 +81,7 @@ struct WgStatic {
       *degday, *pcritgwa, *evarexp, *m_rivrgh;
   // routing statics
   const double *loc_lake, *loc_wetland, *glo_wetland, *lake_area, *reservoir_area, *stor_cap,
-      *mean_outflow, *river_length, *river_slope, *roughness, *width_bf, *bottom_width,
+      *mean_outflow, *res_prov_rel, *river_length, *river_slope, *roughness, *width_bf, *bottom_width,
       *river_stor_max, *lake_depth, *wetl_depth, *f_glo_lake, *fswb_init;
   const int8_t *res_type, *start_month;
   // exp(-k) terms of the surface-water and groundwater outflow coefficients

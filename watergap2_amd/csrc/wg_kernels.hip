@@ -445,7 +445,7 @@ struct CellIn {
   double s_m_netrad;
   double s_m_prec;
   double s_mcwh;
-  double s_mean_outflow;
+  double s_mean_outflow, s_prov_rel;
   double s_ome_gw;
   double s_pcritgwa;
   double s_pet_mxdy;
@@ -537,6 +537,7 @@ __device__ inline void load_cell_in_b(const WgStatic &S, const WgState &X, int64
   ci.s_loc_lake = S.loc_lake[p];
   ci.s_loc_wetland = S.loc_wetland[p];
   ci.s_mean_outflow = S.mean_outflow[p];
+  ci.s_prov_rel = S.res_prov_rel[p];
   ci.s_ome_gw = S.ome_gw[p];
   ci.s_pcritgwa = S.pcritgwa[p];
   ci.s_reservoir_area = S.reservoir_area[p];
@@ -1186,6 +1187,8 @@ __global__ __launch_bounds__(256, kSnowChunk == 5 ? 3 : 1) void wg_cell_kernel(W
     v[CF_RES_CAP] = res ? ci.s_stor_cap : 0.;
     v[CF_RES_QMEAN] = res ? ci.s_mean_outflow : 0.;
     v[CF_RES_AREA] = res ? ci.s_reservoir_area : 0.;
+    v[CF_RES_PROVREL] = res ? ci.s_prov_rel : 0.;
+    v[CF_RES_PAD] = 0.;
   }
   if (lk.nu == 0) {
     // no upstream cell: G_riverInflow[n] is still 0 when the cell is routed
@@ -1238,6 +1241,55 @@ __device__ inline void prefetch_slot(const WgStatic &S, const WgState &X, int64_
     const WgD2 v = src[k * n];
     q.a[2 * k] = v.x;
     q.a[2 * k + 1] = v.y;
+  }
+  q.inv_ksw = cs.inv_ksw;
+  q.ome_sw = cs.ome_sw;
+  q.inv_kgw = cs.inv_kgw;
+  q.ome_gw = cs.ome_gw;
+  q.area = cs.area;
+  q.cf100 = cs.cf100;
+  const WgI4 a = S.clink[p], b = S.clink[n + p];
+  q.flags = a.x;
+  q.station = a.y;
+  q.nu = a.z & 0xffff;
+  q.nu_level = a.z >> 16;
+  q.level = a.w;
+  q.up[0] = b.x;
+  q.up[1] = b.y;
+  q.up[2] = b.z;
+  q.up[3] = b.w;
+}
+
+typedef unsigned int WgU32x4 __attribute__((ext_vector_type(4)));
+
+// Wave-walk variant of prefetch_slot: the record suffix is fetched only for cells
+// with a global lake or reservoir (about 1.5 % of routed cells).  The cell's chain
+// flags `fl` come from LDS, so no global load waits for another.  The other lanes
+// issue the same suffix loads through raw buffers at an offset past the row's end,
+// which the hardware answers with zeros without a memory access: every lane issues
+// the same loads whatever its flags, so the count of outstanding loads, which the
+// compiler's waits rely on, does not depend on them.
+__device__ inline void prefetch_slot_wave(const WgStatic &S, const WgState &X, int64_t base, int p, int fl,
+                                          ChainSlot &q) {
+  const int64_t n = S.n;
+  const WgChainStatic cs = load_cstat(S, p);
+  const WgD2 *src = X.crec + base * (WG_DEV_NREC / 2) + p;
+  constexpr int kPrefixPairs = (CF_PREFIX + 1) / 2;
+#pragma unroll
+  for (int k = 0; k < kPrefixPairs; k++) {
+    const WgD2 v = src[k * n];
+    q.a[2 * k] = v.x;
+    q.a[2 * k + 1] = v.y;
+  }
+  const bool suffix = (fl & (CFL_LAKE | CFL_RES)) != 0;
+  const int off = suffix ? p * 16 : int(0x80000000u);
+  WgD2 *row0 = X.crec + base * (WG_DEV_NREC / 2);
+#pragma unroll
+  for (int k = kPrefixPairs; k < CF_COUNT / 2; k++) {
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(row0 + k * n, (short)0, int(n * 16), 0x00020000);
+    const WgU32x4 w = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    q.a[2 * k] = __builtin_bit_cast(double, (unsigned long long)w.x | ((unsigned long long)w.y << 32));
+    q.a[2 * k + 1] = __builtin_bit_cast(double, (unsigned long long)w.z | ((unsigned long long)w.w << 32));
   }
   q.inv_ksw = cs.inv_ksw;
   q.ome_sw = cs.ome_sw;
@@ -1441,11 +1493,12 @@ __device__ inline void wave_lds_order() {
 // batch's inputs, `fill` receives those of the batch starting at pnext.
 __device__ __forceinline__ void wave_batch(const WgStatic &S, const WgState &X, double *s_tv, int b0, int bend,
                                            int pnext, int ge, int gs, int64_t base, int m, const DayInfo &d,
-                                           int daily_slot, const ChainSlot &q, ChainSlot &fill) {
+                                           int daily_slot, const uint8_t *s_fl, const ChainSlot &q,
+                                           ChainSlot &fill) {
   const int lane = threadIdx.x & 63;
   {
-    const int pf = pnext + lane;
-    prefetch_slot(S, X, base, pf < ge ? pf : gs, fill);
+    const int pf = pnext + lane < ge ? pnext + lane : gs;
+    prefetch_slot_wave(S, X, base, pf, s_fl[pf - gs], fill);
   }
   const int p = b0 + lane;
   const bool have = p < bend;
@@ -1538,6 +1591,22 @@ __global__ __launch_bounds__(64 * kWaves) void wg_chain_wave_kernel(WgStatic S, 
       if (i < lvl0_end) s_tv[i - gs] = v[j];
     }
   }
+  // the chain flags of the group's cells, behind its transported volumes (the
+  // launch gives every workgroup 9 bytes of LDS per cell of the largest group)
+  uint8_t *s_fl = reinterpret_cast<uint8_t *>(s_tv + (ge - gs));
+  for (int i0 = gs + tid; i0 < ge; i0 += 4 * kThreads) {
+    int8_t v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int i = i0 + j * kThreads;
+      v[j] = (i < ge) ? S.cflags[i] : int8_t(0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int i = i0 + j * kThreads;
+      if (i < ge) s_fl[i - gs] = uint8_t(v[j]);
+    }
+  }
   __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   lds_barrier();
   ChainSlot s0, s1;
@@ -1551,8 +1620,8 @@ __global__ __launch_bounds__(64 * kWaves) void wg_chain_wave_kernel(WgStatic S, 
     };
     WgWaveItem ia = item(it0);
     {
-      const int pf = ia.start + lane;
-      prefetch_slot(S, X, base, pf < ge ? pf : gs, s0);
+      const int pf = ia.start + lane < ge ? ia.start + lane : gs;
+      prefetch_slot_wave(S, X, base, pf, s_fl[pf - gs], s0);
     }
     // the first slot's loads complete here on every path into the loop
     __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
@@ -1563,12 +1632,12 @@ __global__ __launch_bounds__(64 * kWaves) void wg_chain_wave_kernel(WgStatic S, 
         lds_barrier();
         lev++;
       }
-      wave_batch(S, X, s_tv, ia.start, ia.end, ib.start, ge, gs, base, m, d, daily_slot, s0, s1);
+      wave_batch(S, X, s_tv, ia.start, ia.end, ib.start, ge, gs, base, m, d, daily_slot, s_fl, s0, s1);
       while (lev < ib.level) {
         lds_barrier();
         lev++;
       }
-      wave_batch(S, X, s_tv, ib.start, ib.end, ic.start, ge, gs, base, m, d, daily_slot, s1, s0);
+      wave_batch(S, X, s_tv, ib.start, ib.end, ic.start, ge, gs, base, m, d, daily_slot, s_fl, s1, s0);
       ia = ic;
     }
     while (lev <= la) {
@@ -1580,14 +1649,14 @@ __global__ __launch_bounds__(64 * kWaves) void wg_chain_wave_kernel(WgStatic S, 
   // the narrow rest, wavefront 0 alone
   const int nb = (ge - pb + 127) / 128 * 2;
   {
-    const int pa = pb + lane;
-    prefetch_slot(S, X, base, pa < ge ? pa : gs, s0);
+    const int pa = pb + lane < ge ? pb + lane : gs;
+    prefetch_slot_wave(S, X, base, pa, s_fl[pa - gs], s0);
   }
   __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   for (int b = 0; b < nb; b += 2) {
     const int b0 = pb + b * 64;
-    wave_batch(S, X, s_tv, b0, ge, b0 + 64, ge, gs, base, m, d, daily_slot, s0, s1);
-    wave_batch(S, X, s_tv, b0 + 64, ge, b0 + 128, ge, gs, base, m, d, daily_slot, s1, s0);
+    wave_batch(S, X, s_tv, b0, ge, b0 + 64, ge, gs, base, m, d, daily_slot, s_fl, s0, s1);
+    wave_batch(S, X, s_tv, b0 + 64, ge, b0 + 128, ge, gs, base, m, d, daily_slot, s_fl, s1, s0);
   }
   (void)T;
 }

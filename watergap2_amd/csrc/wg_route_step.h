@@ -60,7 +60,7 @@ WG_RS_FN DayInfo day_info(int doy) {
 // Record of one routing cell: every term of routing.cpp:2623-3958 that does not
 // depend on the water arriving from upstream, computed by the cell kernel with
 // the reference's own expressions, so that the level walk only evaluates the
-// short chain that does.  Stored as one 240-byte record per (member, cell).
+// short chain that does.  Stored as one 304-byte record per (member, cell).
 enum ChainField : int {
   // prefix read by every routed cell: river, global wetland, arid groundwater
   CF_INFLOW = 0,  // outflow of the local lake/wetland system (routing.cpp:2616)
@@ -79,8 +79,11 @@ enum ChainField : int {
   CF_GL_USE, CF_GL_MAX, CF_GL_PREV, CF_GL_S0E, CF_GL_GWR,
   CF_OW_PREC, CF_OW_PET,
   // reservoir inputs (routing.cpp:2807-2860): previous storage, evaporation
-  // reduction factor, CFA, release coefficient, capacity, mean outflow, area
-  CF_RES_PREV, CF_RES_RED, CF_RES_CFA, CF_RES_KREL, CF_RES_CAP, CF_RES_QMEAN, CF_RES_AREA,
+  // reduction factor, CFA, release coefficient, capacity, mean outflow, area, the
+  // provisional release of an irrigation reservoir (routing.cpp:2972-2996 with
+  // the mean demand of routing.cpp:360-403, computed once on the host), padding
+  CF_RES_PREV, CF_RES_RED, CF_RES_CFA, CF_RES_KREL, CF_RES_CAP, CF_RES_QMEAN, CF_RES_AREA, CF_RES_PROVREL,
+  CF_RES_PAD,
   CF_COUNT
 };
 static_assert(CF_COUNT == WG_DEV_NREC, "routing record size");
@@ -239,14 +242,9 @@ WG_RS_FN ChainOut chain_step(const ChainRec &c, double inflow_up, const DayInfo 
     }
     const int rtype = flag_res_type(c.flags);
     if (rtype == 1) {
-      // irrigation reservoir without water use: zero demand and zero downstream use
-      const double mean_demand = 0.;
-      double monthlyUse = 0.0 * d.days_in_month;
-      monthlyUse = monthlyUse * 1000000000. / (d.days_in_month * 86400.);
-      if (mean_demand >= 0.5 * qmean)
-        prov_rel = qmean / 2. * (1. + monthlyUse / mean_demand);
-      else
-        prov_rel = qmean + monthlyUse - mean_demand;
+      // irrigation reservoir: without water use the month's use term is +0, so
+      // the provisional release is a constant of the cell
+      prov_rel = v[CF_RES_PROVREL];
     } else if (rtype == 2) {
       prov_rel = qmean;
     }
