@@ -317,7 +317,9 @@ int wg_set_forcing_perturbation(wg_handle *h, const uint64_t *member_seed, doubl
                                 double temp_sigma);
 
 /* Copies n_days of daily forcing [n_days][4][N] (temperature degC, precipitation
- * mm/d, shortwave W/m2, longwave W/m2; reference cell order) into device memory. */
+ * mm/d, shortwave W/m2, longwave W/m2; reference cell order) into device memory.
+ * WG_E_INPUT, with nothing changed, when a value is not finite or a precipitation
+ * is negative. */
 int wg_load_forcing(wg_handle *h, const float *forcing, int32_t n_days);
 
 /* Station (outlet) cells whose daily discharge is accumulated (0-based cells). */
@@ -328,9 +330,16 @@ int wg_set_stations(wg_handle *h, int32_t n_stations, const int32_t *cells);
 int wg_step_days(wg_handle *h, int32_t forcing_day0, int32_t first_doy, int32_t n_days);
 
 /* Station discharge summed over the days since the last reset [n_members][n_stations]
- * (km3), and daily series of the last wg_step_days call [n_members][n_days][n_stations]
- * when daily is not NULL. */
+ * (km3), and, when daily is not NULL, the daily series kept from the last
+ * wg_step_days / wg_step_days_profiled call: [n_members][max_days][n_stations], of
+ * which the first min(max_days, kept days) rows are written.  wg_step_days runs
+ * blocks of up to 365 days and keeps the series of its last block;
+ * wg_station_daily_window tells which days those are. */
 int wg_get_station_discharge(wg_handle *h, double *sum, double *daily, int32_t max_days);
+/* The days of the kept daily series: first_day counted from the first day of the
+ * last step call (0 for calls of up to 365 days), n_days of them.  No counterpart
+ * in the reference, whose daily series is a whole year (routing.cpp:4232-4238). */
+int wg_station_daily_window(wg_handle *h, int32_t *first_day, int32_t *n_days);
 int wg_reset_station_discharge(wg_handle *h);
 
 /* Diagnostics of the last simulated day for one member. */
@@ -343,7 +352,8 @@ int wg_synchronize(wg_handle *h);
 void *wg_stream(wg_handle *h);
 
 /* As wg_step_days, launched eagerly with a HIP event pair around every kernel
- * on the handle's stream, so that wg_last_timing reports per-kernel durations. */
+ * on the handle's stream, so that wg_last_timing reports per-kernel durations;
+ * at most 366 days per call. */
 int wg_step_days_profiled(wg_handle *h, int32_t forcing_day0, int32_t first_doy, int32_t n_days);
 
 /* Timing of the last wg_step_days / wg_step_days_profiled call (ms, HIP events on
@@ -376,8 +386,9 @@ int wg_set_launch_config(wg_handle *h, int32_t cell_threads, int32_t chain_threa
 int wg_schedule_info(wg_handle *h, int32_t *info);
 
 /* Diagnostic with no reference counterpart: evaluates the library's FP64 math on
- * the device, fn 0 exp(x), 1 pow(x, y), 2 sqrt(x), 3 log(x), for n arguments in
- * host arrays.  Used to check that the device and the host produce the same bits. */
+ * the device, fn 0 exp(x), 1 pow(x, y), 2 sqrt(x), 3 log(x), 4 sin(2 pi x),
+ * 5 cos(2 pi x), for n arguments in host arrays.  Used to check that the device
+ * and the host produce the same bits. */
 int wg_libm_eval(int32_t fn, int32_t n, const double *x, const double *y, double *out);
 
 /* ---- calibration: the per-basin search for the runoff coefficient gamma ---- */

@@ -1332,25 +1332,32 @@ static void philox4x32_10(uint32_t ctr[4], uint32_t k0, uint32_t k1) {
   }
 }
 
-/* Standard-normal deviate as the Irwin-Hall sum of twelve uniforms minus six.
- * Every operation is exact in double, so host and device agree bit for bit. */
-static double irwin_hall_normal(uint64_t seed, uint32_t cell, uint32_t day, uint32_t stream) {
-  double s = 0.;
-  for (uint32_t blk = 0; blk < 3; blk++) {
-    uint32_t c[4] = {cell, day, stream, blk};
-    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-    for (int k = 0; k < 4; k++) s += ((double)c[k] + 0.5) * 2.3283064365386963e-10;
-  }
-  return s - 6.;
+/* Two independent standard normal deviates for (cell, day) of one member by
+ * Box-Muller from one Philox block: u = (c + 1/2) 2^-32, R = sqrt(-2 ln u1),
+ * (R cos 2 pi u2, R sin 2 pi u2), with the log, sin and cos shared with the
+ * kernels (wg_libm.h) and the correctly rounded sqrt. */
+static void normal_pair(uint64_t seed, uint32_t cell, uint32_t day, double *z0, double *z1) {
+  uint32_t c[4] = {cell, day, 0x424f584du, 0u};
+  double u1, u2, r, sn, cs;
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  u1 = ((double)c[0] + 0.5) * 2.3283064365386963e-10;
+  u2 = ((double)c[1] + 0.5) * 2.3283064365386963e-10;
+  r = sqrt(-2.0 * wg_log(u1));
+  wg_sincos2pi(u2, &sn, &cs);
+  *z0 = r * cs;
+  *z1 = r * sn;
 }
 
+void wgo_normal_pair(uint64_t seed, uint32_t cell, uint32_t day, double *out2) {
+  normal_pair(seed, cell, day, out2, out2 + 1);
+}
 
 void wgo_perturb_forcing(uint64_t seed, int32_t day_index, int32_t n, double prec_sigma,
                          double temp_sigma, const float *temp_in, const float *prec_in,
                          float *temp_out, float *prec_out) {
   for (int32_t i = 0; i < n; i++) {
-    double zp = irwin_hall_normal(seed, (uint32_t)i, (uint32_t)day_index, 0x50524543u);
-    double zt = irwin_hall_normal(seed, (uint32_t)i, (uint32_t)day_index, 0x54454d50u);
+    double zp, zt;
+    normal_pair(seed, (uint32_t)i, (uint32_t)day_index, &zp, &zt);
     prec_out[i] = (float)((double)prec_in[i] * wg_exp(prec_sigma * zp));
     temp_out[i] = (float)((double)temp_in[i] + temp_sigma * zt);
   }
@@ -1363,3 +1370,4 @@ void wgo_philox(uint32_t ctr[4], const uint32_t key[2]) { philox4x32_10(ctr, key
 double wgo_exp(double x) { return EXP(x); }
 double wgo_pow(double x, double y) { return POW(x, y); }
 double wgo_log(double x) { return wg_log(x); }
+void wgo_sincos2pi(double u, double *s, double *c) { wg_sincos2pi(u, s, c); }

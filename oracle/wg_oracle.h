@@ -81,6 +81,10 @@ void wgo_perturb_forcing(uint64_t seed, int32_t day_index, int32_t n, double pre
                          double temp_sigma, const float *temp_in, const float *prec_in,
                          float *temp_out, float *prec_out);
 
+/* The two normal deviates of (seed, cell, day) the perturbation uses. */
+void wgo_normal_pair(uint64_t seed, uint32_t cell, uint32_t day, double *out2);
+void wgo_sincos2pi(double u, double *s, double *c);
+
 /* One Philox4x32-10 block: ctr is replaced by the output. */
 void wgo_philox(uint32_t ctr[4], const uint32_t key[2]);
 

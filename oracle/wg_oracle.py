@@ -62,6 +62,10 @@ def load(variant="wg"):
     lib.wgo_month_of_day.argtypes = [C.c_int]
     lib.wgo_set_threads.argtypes = [C.c_void_p, C.c_int]
     lib.wgo_set_threads.restype = None
+    lib.wgo_normal_pair.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, P_d]
+    lib.wgo_normal_pair.restype = None
+    lib.wgo_sincos2pi.argtypes = [C.c_double, P_d, P_d]
+    lib.wgo_sincos2pi.restype = None
     lib.wgo_philox.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     lib.wgo_philox.restype = None
     for fn in ("wgo_exp", "wgo_log"):
@@ -209,6 +213,24 @@ def philox(ctr, key):
     k = (C.c_uint32 * 2)(*[int(v) for v in key])
     load().wgo_philox(c, k)
     return [int(v) for v in c]
+
+
+def normal_pairs(seed, cells, day):
+    """[len(cells), 2] normal deviates of the perturbation for one member and day."""
+    lib = load()
+    out = np.zeros((len(cells), 2))
+    buf = (C.c_double * 2)()
+    for k, c in enumerate(cells):
+        lib.wgo_normal_pair(int(seed), int(c), int(day), C.cast(buf, _abi.P_d))
+        out[k] = buf[0], buf[1]
+    return out
+
+
+def sincos2pi(u):
+    lib = load()
+    s, c = C.c_double(), C.c_double()
+    lib.wgo_sincos2pi(float(u), C.byref(s), C.byref(c))
+    return s.value, c.value
 
 
 def perturb_forcing(seed, day_index, temp, prec, prec_sigma, temp_sigma):

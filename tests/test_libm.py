@@ -93,6 +93,8 @@ def test_device_bits_equal_host(wg, oracle):
         1: (base, y, lambda a, b: host.wgo_pow(a, b)),
         2: (base * 1e3, None, lambda a, b: math.sqrt(a)),
         3: (x_log, None, lambda a, b: host.wgo_log(a)),
+        4: (np.random.default_rng(4).random(20000), None, lambda a, b: oracle.sincos2pi(a)[0]),
+        5: (np.random.default_rng(5).random(20000), None, lambda a, b: oracle.sincos2pi(a)[1]),
     }.items():
         xs = np.ascontiguousarray(xs)
         ys_ = np.ascontiguousarray(ys) if ys is not None else None
@@ -104,3 +106,17 @@ def test_device_bits_equal_host(wg, oracle):
         expect = np.array([ref(xs[i], ys_[i] if ys_ is not None else 0.0) for i in range(len(xs))])
         mism = np.flatnonzero(out.view(np.int64) != expect.view(np.int64))
         assert len(mism) == 0, (fn, len(mism), xs[mism[:3]], out[mism[:3]], expect[mism[:3]])
+
+
+def test_sincos2pi_accuracy(oracle):
+    """sin and cos of 2 pi u within 3e-16 of an extended-precision evaluation
+    (x87 long double: the angle 2 pi u carries no double rounding there) on random
+    and on quadrant-boundary arguments."""
+    rng = np.random.default_rng(6)
+    us = np.concatenate([rng.random(20000), np.arange(0, 1, 1 / 64), [0.5 - 2 ** -40, 0.25 + 2 ** -45]])
+    got = np.array([oracle.sincos2pi(u) for u in us])
+    two_pi = np.longdouble(2) * np.longdouble("3.14159265358979323846264338327950288")
+    ang = two_pi * us.astype(np.longdouble)
+    assert np.finfo(np.longdouble).eps < 1e-18
+    np.testing.assert_allclose(got[:, 0], np.sin(ang).astype(np.float64), rtol=0, atol=3e-16)
+    np.testing.assert_allclose(got[:, 1], np.cos(ang).astype(np.float64), rtol=0, atol=3e-16)

@@ -179,3 +179,22 @@ def test_priestley_taylor_against_fao56_tables(wg, oracle):
     expect = alpha * delta / (delta + gamma) * rn
     land = w.contfreq > 0
     np.testing.assert_allclose(pet[land], expect[land], rtol=5e-3)
+
+
+def test_perturbation_deviates_are_standard_normal(oracle):
+    """The ensemble's Box-Muller deviates: 2 x 100,000 draws (cells x days of one
+    member) pass a Kolmogorov-Smirnov test against N(0, 1) at the 1 % level, with
+    the pair's two deviates uncorrelated and the 3-sigma tail at its normal share."""
+    from scipy import stats
+    cells = np.arange(10000, dtype=np.uint32)
+    z = np.concatenate([oracle.normal_pairs(12345, cells, day) for day in range(10)])
+    for k in (0, 1):
+        assert stats.kstest(z[:, k], "norm").pvalue > 0.01
+        assert abs(z[:, k].mean()) < 0.01 and abs(z[:, k].std() - 1.0) < 0.01
+        assert abs((np.abs(z[:, k]) > 3).mean() - 0.0026998) < 0.0006
+    assert abs(np.corrcoef(z[:, 0], z[:, 1])[0, 1]) < 0.01
+    # the perturbation applies them as P exp(sigma z0), T + sigma z1
+    t, p = oracle.perturb_forcing(12345, 3, np.full(4, 10.0, np.float32), np.full(4, 2.0, np.float32), 0.2, 1.0)
+    zz = oracle.normal_pairs(12345, np.arange(4), 3)
+    np.testing.assert_allclose(p, (2.0 * np.exp(0.2 * zz[:, 0])).astype(np.float32), rtol=1e-6)
+    np.testing.assert_allclose(t, (10.0 + zz[:, 1]).astype(np.float32), rtol=1e-6)

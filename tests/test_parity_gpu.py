@@ -94,6 +94,47 @@ def test_irrigation_reservoir_demand(wg, oracle):
     check_pair(m, o, 150)
 
 
+def test_handle_bookkeeping(wg, oracle, regional_world):
+    """Rejected forcing leaves the loaded forcing in place; stations can be set
+    again; a 400-day call keeps the daily series of its last block (days 365..399),
+    which equals the oracle's; the profiled path takes at most 366 days."""
+    print("case: handle bookkeeping, 400 days", flush=True)
+    w = regional_world
+    f = w.forcing(seed=9, day0=0, n_days=400)
+    m = wg.WaterBalanceModel(w)
+    m.load_forcing(f)
+    bad = f[:3].copy()
+    bad[1, 0, 17] = np.nan
+    with pytest.raises(wg.WaterGapError) as e:
+        m.load_forcing(bad)
+    assert e.value.code == -5
+    bad = f[:3].copy()
+    bad[2, 1, 5] = -1.0
+    with pytest.raises(wg.WaterGapError) as e:
+        m.load_forcing(bad)
+    assert e.value.code == -5
+    m.set_stations(np.array([3, 4, 5], np.int32))
+    cells = np.array([w.basin_outlet[0], 77, 1500, 77], dtype=np.int32)
+    m.set_stations(cells)
+    m.step_days(0, 1, 400)
+    assert m.station_daily_window() == (365, 35)
+    tot, daily = m.station_discharge(max_days=35)
+    o = oracle.Oracle(w)
+    o.set_stations(cells)
+    o.run(f[:365], 1)
+    prev = o.station_discharge()
+    for d in range(35):
+        o.step_day(f[365 + d, 0], f[365 + d, 1], f[365 + d, 2], f[365 + d, 3], d + 1)
+        now = o.station_discharge()
+        assert_close({"discharge": daily[0, d]}, {"discharge": now - prev}, label=f"day{365 + d}:")
+        prev = now
+    assert_close({"discharge": tot[0]}, {"discharge": o.station_discharge()})
+    assert_close(m.get_state(), o.get_state())
+    with pytest.raises(wg.WaterGapError) as e:
+        m.step_days(0, 1, 400, profiled=True)
+    assert e.value.code == -1
+
+
 def test_random_initial_state(wg, oracle, regional_edge_world):
     """Random storages drive rarely taken branches: deep snow (threshold elevation),
     empty and negative lakes, saturated soils, land area fraction zero."""

@@ -313,6 +313,13 @@ class WaterBalanceModel:
                                                    max_days))
         return total, daily
 
+    def station_daily_window(self):
+        """(first day, days) of the kept daily station series, counted from the
+        first day of the last step call."""
+        a, b = C.c_int32(), C.c_int32()
+        self._c(self._lib.wg_station_daily_window(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def reset_station_discharge(self):
         self._c(self._lib.wg_reset_station_discharge(self._h))
 

@@ -237,6 +237,7 @@ def _declare(lib):
     lib.wg_libm_eval.argtypes = [i32, i32, P_d, P_d, P_d]
     lib.wg_get_station_discharge.argtypes = [C.c_void_p, P_d, P_d, i32]
     lib.wg_reset_station_discharge.argtypes = [C.c_void_p]
+    lib.wg_station_daily_window.argtypes = [C.c_void_p, P_i32, P_i32]
     lib.wg_get_day_outputs.argtypes = [C.c_void_p, i32, C.POINTER(DayOutputs)]
     lib.wg_synchronize.argtypes = [C.c_void_p]
     lib.wg_stream.argtypes = [C.c_void_p]
@@ -279,7 +280,7 @@ ABI_SYMBOLS = [
     "wg_prepare_routing", "wg_create", "wg_destroy",
     "wg_last_error", "wg_get_state", "wg_set_state", "wg_set_member_params",
     "wg_set_forcing_perturbation", "wg_load_forcing", "wg_set_stations", "wg_step_days",
-    "wg_get_station_discharge", "wg_reset_station_discharge", "wg_get_day_outputs",
+    "wg_get_station_discharge", "wg_station_daily_window", "wg_reset_station_discharge", "wg_get_day_outputs",
     "wg_synchronize", "wg_stream", "wg_step_days_profiled", "wg_last_timing", "wg_schedule_info", "wg_libm_eval",
     "wg_autotune", "wg_set_launch_config",
     "wg_gamma_search_init", "wg_gamma_start", "wg_find_new_gamma",

@@ -68,7 +68,8 @@ struct wg_handle {
   // stations
   int32_t n_stations = 0;
   std::vector<int32_t> station_cells;
-  int32_t last_daily_days = 0;
+  int32_t last_daily_days = 0;   // days of the daily station series kept from the last step call
+  int32_t last_daily_first = 0;  // ... and the first of them, counted from that call's first day
   // graphs keyed by block length
   std::map<int, hipGraphExec_t> graphs;
   int block_threads_cell = 256, block_threads_chain = 256;
@@ -119,7 +120,10 @@ int dev_alloc(wg_handle *h, T **out, size_t count) {
   // uploads use: finish the zero fill before any copy can target the buffer.
   e = hipMemsetAsync(p, 0, bytes, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  if (e != hipSuccess) return fail(h, WG_E_HIP, "zero fill failed: %s", hipGetErrorString(e));
+  if (e != hipSuccess) {
+    (void)hipFree(p);
+    return fail(h, WG_E_HIP, "zero fill failed: %s", hipGetErrorString(e));
+  }
   h->bufs.push_back({p, bytes});
   *out = static_cast<T *>(p);
   return WG_OK;
@@ -1092,28 +1096,49 @@ extern "C" int wg_set_forcing_perturbation(wg_handle *h, const uint64_t *seeds, 
   return WG_OK;
 }
 
+namespace {
+void clear_graphs(wg_handle *h) {
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
+}
+}  // namespace
+
 extern "C" int wg_load_forcing(wg_handle *h, const float *forcing, int32_t n_days) {
   if (!h || !forcing || n_days <= 0) return WG_E_ARG;
+  const int32_t n = h->n;
+  const size_t day_elems = size_t(4) * n;
+  // the reference reads the forcing grids as given; values that no climate input
+  // holds (not finite, negative precipitation) are rejected before anything changes
+  for (int32_t d = 0; d < n_days; d++) {
+    const float *src = forcing + day_elems * d;
+    for (size_t k = 0; k < day_elems; k++)
+      if (!std::isfinite(src[k]))
+        return fail(h, WG_E_INPUT, "forcing day %d, variable %d, cell %d is not finite", d, int(k / n), int(k % n));
+    for (int32_t i = 0; i < n; i++)
+      if (src[size_t(n) + i] < 0.f)
+        return fail(h, WG_E_INPUT, "forcing day %d: negative precipitation %g at cell %d", d, src[size_t(n) + i], i);
+  }
   HIP_CHECK(h, hipSetDevice(h->device));
   HIP_CHECK(h, hipStreamSynchronize(h->stream));
-  const int32_t n = h->n;
-  if (h->d_forcing) {
-    HIP_CHECK(h, hipFree(h->d_forcing));
-    h->d_forcing = nullptr;
-  }
-  const size_t day_elems = size_t(4) * n;
-  HIP_CHECK(h, hipMalloc(&h->d_forcing, day_elems * n_days * sizeof(float)));
+  // the captured graphs hold the old buffer's address
+  clear_graphs(h);
+  float *fresh = nullptr;
+  hipError_t e = hipMalloc(&fresh, day_elems * n_days * sizeof(float));
+  if (e != hipSuccess) return fail(h, WG_E_NOMEM, "forcing buffer of %d days: %s", n_days, hipGetErrorString(e));
   std::vector<float> buf(day_elems);
-  for (int32_t d = 0; d < n_days; d++) {
+  for (int32_t d = 0; d < n_days && e == hipSuccess; d++) {
     const float *src = forcing + day_elems * d;
     for (int v = 0; v < 4; v++)
       for (int32_t p = 0; p < n; p++) buf[size_t(v) * n + p] = src[size_t(v) * n + h->perm[p]];
-    HIP_CHECK(h, hipMemcpy(h->d_forcing + day_elems * d, buf.data(), day_elems * sizeof(float),
-                           hipMemcpyHostToDevice));
+    e = hipMemcpy(fresh + day_elems * d, buf.data(), day_elems * sizeof(float), hipMemcpyHostToDevice);
   }
+  if (e != hipSuccess) {
+    (void)hipFree(fresh);
+    return fail(h, WG_E_HIP, "forcing upload: %s", hipGetErrorString(e));
+  }
+  if (h->d_forcing) (void)hipFree(h->d_forcing);
+  h->d_forcing = fresh;
   h->n_forcing_days = n_days;
-  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
-  h->graphs.clear();
   return WG_OK;
 }
 
@@ -1127,30 +1152,40 @@ extern "C" int wg_set_stations(wg_handle *h, int32_t n_st, const int32_t *cells)
     // the reference stops at the first station of a cell (routing.cpp:4232-4238)
     if (slot[h->inv[cells[b]]] < 0) slot[h->inv[cells[b]]] = b;
   }
-  HIP_CHECK(h, hipMemcpy(const_cast<int32_t *>(h->S.station), slot.data(), h->n * sizeof(int32_t),
-                         hipMemcpyHostToDevice));
+  // the new accumulators first: on failure the handle keeps its stations
+  auto release = [&](void *ptr) {
+    for (auto it = h->bufs.begin(); it != h->bufs.end(); ++it)
+      if (it->p == ptr) {
+        (void)hipFree(it->p);
+        h->bufs.erase(it);
+        return;
+      }
+  };
+  double *st_sum = nullptr, *st_daily = nullptr;
+  int rc = dev_alloc(h, &st_sum, size_t(h->m) * std::max(n_st, 1));
+  if (!rc) rc = dev_alloc(h, &st_daily, size_t(h->m) * WG_DAILY_SLOTS * std::max(n_st, 1));
+  if (rc) {
+    if (st_sum) release(st_sum);
+    return rc;
+  }
+  // the captured graphs hold the accumulators' addresses and the station count
+  clear_graphs(h);
   for (int32_t p = 0; p < h->n; p++) h->clink[p].station = slot[p];
   const std::vector<WgI4> packed = pack_links(h->clink);
-  HIP_CHECK(h, hipMemcpy(const_cast<WgI4 *>(h->S.clink), packed.data(), packed.size() * sizeof(WgI4),
-                         hipMemcpyHostToDevice));
-  // reallocate the accumulators
-  for (auto it = h->bufs.begin(); it != h->bufs.end();) {
-    if (it->p == h->X.st_sum || it->p == h->X.st_daily) {
-      (void)hipFree(it->p);
-      it = h->bufs.erase(it);
-    } else {
-      ++it;
-    }
-  }
+  hipError_t e = hipMemcpy(const_cast<int32_t *>(h->S.station), slot.data(), h->n * sizeof(int32_t),
+                           hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(const_cast<WgI4 *>(h->S.clink), packed.data(), packed.size() * sizeof(WgI4),
+                  hipMemcpyHostToDevice);
+  release(h->X.st_sum);
+  release(h->X.st_daily);
+  h->X.st_sum = st_sum;
+  h->X.st_daily = st_daily;
   h->n_stations = n_st;
-  h->station_cells.assign(cells, cells + n_st);
-  int rc = dev_alloc(h, &h->X.st_sum, size_t(h->m) * std::max(n_st, 1));
-  if (!rc) rc = dev_alloc(h, &h->X.st_daily, size_t(h->m) * WG_DAILY_SLOTS * std::max(n_st, 1));
-  if (rc) return rc;
   h->S.n_stations = n_st;
-  HIP_CHECK(h, hipStreamSynchronize(h->stream));
-  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
-  h->graphs.clear();
+  h->station_cells.assign(cells, cells + n_st);
+  h->last_daily_days = 0;
+  if (e != hipSuccess) return fail(h, WG_E_HIP, "station tables: %s", hipGetErrorString(e));
   return WG_OK;
 }
 
@@ -1270,8 +1305,9 @@ extern "C" int wg_step_days(wg_handle *h, int32_t forcing_day0, int32_t first_do
   int rc = check_step_args(h, forcing_day0, first_doy, n_days);
   if (rc) return rc;
   HIP_CHECK(h, hipSetDevice(h->device));
+  // a block's daily station series fills the WG_DAILY_SLOTS slots in day order
   int block = 365;
-  if (const char *v = getenv("WG_GRAPH_DAYS")) block = std::max(1, std::min(3650, atoi(v)));
+  if (const char *v = getenv("WG_GRAPH_DAYS")) block = std::max(1, std::min(WG_DAILY_SLOTS, atoi(v)));
   HIP_CHECK(h, hipEventRecord(h->ev0, h->stream));
   int64_t day = forcing_day0;
   int doy = first_doy;
@@ -1297,7 +1333,8 @@ extern "C" int wg_step_days(wg_handle *h, int32_t forcing_day0, int32_t first_do
   HIP_CHECK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
   h->last_ms[2] = ms;
   h->last_launches[2] = launches;
-  h->last_daily_days = std::min(WG_DAILY_SLOTS, (n_days - 1) % block + 1);
+  h->last_daily_days = (n_days - 1) % block + 1;
+  h->last_daily_first = n_days - h->last_daily_days;
   return check_flow_error(h);
 }
 
@@ -1307,6 +1344,8 @@ extern "C" int wg_step_days_profiled(wg_handle *h, int32_t forcing_day0, int32_t
   if (!h) return WG_E_ARG;
   int rc = check_step_args(h, forcing_day0, first_doy, n_days);
   if (rc) return rc;
+  if (n_days > WG_DAILY_SLOTS)
+    return fail(h, WG_E_ARG, "the profiled path runs at most %d days per call", WG_DAILY_SLOTS);
   HIP_CHECK(h, hipSetDevice(h->device));
   std::vector<hipEvent_t> evs(size_t(2) * n_days + 2);
   for (auto &e : evs) HIP_CHECK(h, hipEventCreate(&e));
@@ -1333,16 +1372,12 @@ extern "C" int wg_step_days_profiled(wg_handle *h, int32_t forcing_day0, int32_t
   h->last_launches[0] = n_days;
   h->last_launches[1] = n_days;
   h->last_launches[2] = 2 * int64_t(n_days) + 1;
-  h->last_daily_days = std::min(n_days, WG_DAILY_SLOTS);
+  h->last_daily_days = n_days;
+  h->last_daily_first = 0;
   return check_flow_error(h);
 }
 
 namespace {
-
-void clear_graphs(wg_handle *h) {
-  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
-  h->graphs.clear();
-}
 
 int apply_config(wg_handle *h, int cell_threads, int chain_threads, int chunk, int chain_mode) {
   if (chain_mode == 1 && h->max_depth > WG_MAX_GROUP_DEPTH)
@@ -1384,8 +1419,12 @@ extern "C" int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double
   };
   const bool started = h->started;
   const int32_t daily = h->last_daily_days;
+  const int c0 = h->block_threads_cell, ch0 = h->block_threads_chain, k0 = h->snow_chunk, m0 = h->chain_mode;
   int rc = WG_OK;
-  if (copy_all(true) != hipSuccess) rc = fail(h, WG_E_HIP, "autotune snapshot copy");
+  if (copy_all(true) != hipSuccess) {
+    (void)hipFree(snap);
+    return fail(h, WG_E_HIP, "autotune snapshot copy");
+  }
   const int cells[] = {128, 256}, chains[] = {64, 128, 256, 512}, chunks[] = {5, 10}, modes[] = {0, 1, 2, 3};
   double best = 1e300;
   int bc = h->block_threads_cell, bch = h->block_threads_chain, bk = h->snow_chunk, bm = h->chain_mode;
@@ -1414,13 +1453,19 @@ extern "C" int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double
           bm = md;
         }
       }
-  if (!rc) rc = apply_config(h, bc, bch, bk, bm);
-  if (!rc && copy_all(false) != hipSuccess) rc = fail(h, WG_E_HIP, "autotune restore");
-  if (!rc) rc = upload_tiles(h);
+  // the state and (after a failure) the configuration are restored on every path
+  const std::string err = h->err;
+  int rc2 = rc ? apply_config(h, c0, ch0, k0, m0) : apply_config(h, bc, bch, bk, bm);
+  if (copy_all(false) != hipSuccess && !rc2) rc2 = fail(h, WG_E_HIP, "autotune restore");
+  if (!rc2) rc2 = upload_tiles(h);
   h->started = started;
   h->last_daily_days = daily;
   (void)hipFree(snap);
-  if (rc) return rc;
+  if (rc) {
+    h->err = err;
+    return rc;
+  }
+  if (rc2) return rc2;
   if (config) {
     config[0] = bc;
     config[1] = bch;
@@ -1460,6 +1505,13 @@ extern "C" int wg_get_station_discharge(wg_handle *h, double *sum, double *daily
         for (int32_t b = 0; b < ns; b++)
           daily[(size_t(mm) * max_days + d) * ns + b] = buf[(size_t(mm) * WG_DAILY_SLOTS + d) * ns + b];
   }
+  return WG_OK;
+}
+
+extern "C" int wg_station_daily_window(wg_handle *h, int32_t *first_day, int32_t *n_days) {
+  if (!h || !first_day || !n_days) return WG_E_ARG;
+  *first_day = h->last_daily_first;
+  *n_days = h->last_daily_days;
   return WG_OK;
 }
 
@@ -1525,7 +1577,7 @@ extern "C" int wg_schedule_info(wg_handle *h, int32_t *info) {
 }
 
 extern "C" int wg_libm_eval(int32_t fn, int32_t n, const double *x, const double *y, double *out) {
-  if (n <= 0 || !x || !out || fn < 0 || fn > 3 || (fn == 1 && !y)) return WG_E_ARG;
+  if (n <= 0 || !x || !out || fn < 0 || fn > 5 || (fn == 1 && !y)) return WG_E_ARG;
   double *dx = nullptr, *dy = nullptr, *dout = nullptr;
   const size_t b = size_t(n) * sizeof(double);
   int rc = WG_OK;

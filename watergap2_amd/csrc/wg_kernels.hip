@@ -46,7 +46,7 @@ namespace {
 
 #include "wg_route_step.h"
 
-// ---- ensemble perturbation: Philox4x32-10 + Irwin-Hall normal deviates ----
+// ---- ensemble perturbation: Philox4x32-10 + Box-Muller normal deviates ----
 __device__ inline void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; r++) {
@@ -62,14 +62,21 @@ __device__ inline void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
   }
 }
 
-__device__ double irwin_hall(uint64_t seed, uint32_t cell, uint32_t day, uint32_t stream) {
-  double s = 0.;
-  for (uint32_t blk = 0; blk < 3; blk++) {
-    uint32_t c[4] = {cell, day, stream, blk};
-    philox(c, uint32_t(seed), uint32_t(seed >> 32));
-    for (int k = 0; k < 4; k++) s += (double(c[k]) + 0.5) * 2.3283064365386963e-10;
-  }
-  return s - 6.;
+// Two independent standard normal deviates for (cell, day) of one member: one
+// Philox block gives u1, u2 = (c + 1/2) 2^-32 in (0, 1); Box-Muller
+// R = sqrt(-2 ln u1), z = (R cos 2 pi u2, R sin 2 pi u2).  ln, sin and cos are the
+// deterministic ones shared with the host, sqrt is correctly rounded, so the
+// oracle draws the same bits.  |z| < 6.7 (u1 >= 2^-33).
+__device__ inline void normal_pair(uint64_t seed, uint32_t cell, uint32_t day, double &z0, double &z1) {
+  uint32_t c[4] = {cell, day, 0x424f584du, 0u};
+  philox(c, uint32_t(seed), uint32_t(seed >> 32));
+  const double u1 = (double(c[0]) + 0.5) * 2.3283064365386963e-10;
+  const double u2 = (double(c[1]) + 0.5) * 2.3283064365386963e-10;
+  const double r = sqrt(-2.0 * wg_log(u1));
+  double sn, cs;
+  wg_sincos2pi(u2, &sn, &cs);
+  z0 = r * cs;
+  z1 = r * sn;
 }
 
 // Device accessors of the routing links and chain constants ([k][N] x 16 bytes).
@@ -625,8 +632,8 @@ __global__ __launch_bounds__(256, kSnowChunk == 5 ? 3 : 1) void wg_cell_kernel(W
   if (X.perturb) {
     const uint64_t seed = X.member_seed[m];
     const uint32_t ref = uint32_t(S.ref_index[p]);
-    const double zp = irwin_hall(seed, ref, uint32_t(fday), 0x50524543u);
-    const double zt = irwin_hall(seed, ref, uint32_t(fday), 0x54454d50u);
+    double zp, zt;
+    normal_pair(seed, ref, uint32_t(fday), zp, zt);
     pf = float(double(pf) * wg_exp(X.prec_sigma * zp));
     tf = float(double(tf) + X.temp_sigma * zt);
   }
@@ -1811,7 +1818,13 @@ __global__ void wg_libm_kernel(int fn, int n, const double *__restrict__ x, cons
     case 0: r = wg_exp(x[i]); break;
     case 1: r = wg_pow(x[i], y[i]); break;
     case 2: r = sqrt(x[i]); break;
-    default: r = wg_log(x[i]); break;
+    case 3: r = wg_log(x[i]); break;
+    default: {
+      double sn, cs;
+      wg_sincos2pi(x[i], &sn, &cs);
+      r = (fn == 4) ? sn : cs;
+      break;
+    }
   }
   out[i] = r;
 }

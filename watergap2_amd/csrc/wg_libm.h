@@ -19,6 +19,8 @@
  *   log:  y0 from the atanh series of the mantissa, then one Newton step with the
  *         double-double exp: log x = y0 + t - t^2/2, t = x e^(-y0) - 1.
  *   pow:  exp(y log x) with the product carried as a double-double.
+ *   sin/cos of 2 pi u (the ensemble's Box-Muller normals): quadrant split and
+ *         Taylor series, see wg_sincos2pi.
  * Agreement with glibc is checked by tests/test_libm.py.
  */
 #ifndef WG_LIBM_H
@@ -173,6 +175,38 @@ WG_FN double wg_pow(double x, double y) {
   if (z.hi < -745.1332191019412) return sign * 0.0;
   wg_dd r = wg_exp_dd(z.hi, z.lo);
   return sign * (r.hi + r.lo);
+}
+
+/* sin(2 pi u) and cos(2 pi u) for u in [0, 1), from IEEE additions and
+ * multiplications only (no fused multiply-add), so that host and device produce
+ * the same bits.  4u = q + r splits off the quadrant q exactly; the angle within
+ * the quadrant is folded to x = (pi/2) min(r, 1 - r) <= pi/4, where the Taylor
+ * series of sin and cos, to x^17 and x^18, are below 1e-19 of their sums.
+ * Absolute error about 2e-16. */
+WG_FN void wg_sincos2pi(double u, double *s, double *c) {
+  const double t = 4.0 * u;
+  const double qf = WG_FLOOR(t);
+  const int q = ((int)qf) & 3;
+  const double r = t - qf;
+  const int flip = r > 0.5;
+  const double rr = flip ? 1.0 - r : r;
+  const double x = rr * 1.5707963267948966;
+  const double x2 = x * x;
+  const double sx =
+      x * (1.0 + x2 * (-1.0 / 6.0 + x2 * (1.0 / 120.0 + x2 * (-1.0 / 5040.0 + x2 * (1.0 / 362880.0 +
+           x2 * (-1.0 / 39916800.0 + x2 * (1.0 / 6227020800.0 + x2 * (-1.0 / 1307674368000.0 +
+           x2 * (1.0 / 355687428096000.0)))))))));
+  const double cx =
+      1.0 + x2 * (-0.5 + x2 * (1.0 / 24.0 + x2 * (-1.0 / 720.0 + x2 * (1.0 / 40320.0 + x2 * (-1.0 / 3628800.0 +
+            x2 * (1.0 / 479001600.0 + x2 * (-1.0 / 87178291200.0 + x2 * (1.0 / 20922789888000.0 +
+            x2 * (-1.0 / 6402373705728000.0)))))))));
+  const double sa = flip ? cx : sx, ca = flip ? sx : cx;
+  switch (q) {
+    case 0: *s = sa; *c = ca; break;
+    case 1: *s = ca; *c = -sa; break;
+    case 2: *s = -sa; *c = -ca; break;
+    default: *s = -ca; *c = sa; break;
+  }
 }
 
 #endif
