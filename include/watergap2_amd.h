@@ -202,6 +202,56 @@ int wg_inflow_outflow(int32_t n_cells, const int8_t *ldd, const int16_t *row, co
 int wg_rout_order(int32_t n_cells, const int32_t *inflow9, const int32_t *outflow,
                   int32_t *rout_order, int32_t *level);
 
+/* ---- WaterGAP input files (host) ----
+ * .UNF grids are raw big-endian arrays without a header (grid_io_adapters.h:43-129);
+ * the digit after .UNF is the element size: 0 float32, 1 int8, 2 int16, 4 int32.
+ * The reference keeps double grids in .UNF0 files as float32. */
+#define WG_UNF_F32 0
+#define WG_UNF_I8 1
+#define WG_UNF_I16 2
+#define WG_UNF_I32 4
+
+/* Element type of a file from its name (WG_UNF_*), or WG_E_ARG. */
+int wg_unf_type(const char *path);
+/* Reads count elements of the given type into out (host byte order); WG_E_INPUT
+ * when the file is missing or shorter.  Replaces GridUNFAdapter::readGrid for
+ * grids of the file's own element type. */
+int wg_unf_read(const char *path, int32_t type, int64_t count, void *out);
+int wg_unf_write(const char *path, int32_t type, int64_t count, const void *in);
+/* A double grid read from / written to a .UNF0 file (float32 on disk), as
+ * GridUNFAdapter does for Grid<double> (grid_io_adapters.h:60-75, 103-117). */
+int wg_unf_read_double(const char *path, int64_t count, double *out);
+int wg_unf_write_double(const char *path, int64_t count, const double *in);
+/* The year's daily forcing from climate_dir/G_TEMP_H08_int_<year>.365.UNF0,
+ * G_GPCC_H08day_V20110128_<year>.365.UNF0, G_SSRD_H08_int_<year>.365.UNF0 and
+ * G_SLRD_H08_int_<year>.365.UNF0 ([N][365] float each) into the layout of
+ * wg_load_forcing, forcing[365][4][N].  Replaces
+ * climateYearClass::read_climate_data_daily_per_year (climateYear.cpp:38-54). */
+int wg_read_climate_year(const char *climate_dir, int32_t year, int32_t n_cells, float *forcing);
+/* LCT_22.DAT and LAI_22.DAT of input_dir ('#' comment lines, then one row per
+ * class 1..18).  Replaces dailyWaterBalanceClass::readLCTdata (daily.cpp:1925-1959)
+ * and laiClass::readLAIdata (lai.cpp:110-149). */
+int wg_read_lct_tables(const char *input_dir, wg_lct_tables *out);
+/* Cell area from GAREA.UNF0 by row (geo.h:33-35) and the continental share raised
+ * to the sum of the surface-water fractions G_LOCLAK, G_GLOLAK, G_LOCRES,
+ * G_RES_FRAC (any may be NULL).  Replaces geo.cpp:7-44. */
+int wg_cell_geometry(int32_t n_cells, const double *area_by_row, const int16_t *row, const double *contfreq_in,
+                     const double *loc_lake, const double *glo_lake, const double *loc_res, const double *res_frac,
+                     double *cell_area, double *contfreq);
+/* Smax = G_TAWC x M_ROOT_D x rooting depth of the land-cover class (-9999 where
+ * TAWC < 0).  Replaces soilWatCapClass::createMaxSoilWaterCapacityGrid
+ * (s_max.cpp:43-74). */
+int wg_soil_capacity(int32_t n_cells, const float *tawc, const int8_t *land_cover, const double *m_root_d,
+                     const float *rooting_depth_lct, float *smax);
+/* Groundwater factor fg and Rgmax (mm/d x 100) from G_SLOPE_CLASS.UNF1,
+ * G_TEXTURE.UNF1, G_PERMAGLAC.UNF1, G_AQ_FACTOR.UNF1 and G_GW_FACTOR_CORR.UNF0 (may
+ * be NULL) with the M_GW_F and M_RG_MAX multipliers; time_series 0 selects the
+ * daily-WFD Rgmax table.  Replaces groundwaterFactorClass::createGrids
+ * (gw_frac.cpp:31-258). */
+int wg_gw_factor(int32_t n_cells, const int8_t *slope_class, const int8_t *texture, const int8_t *permaglac,
+                 const int8_t *aquifer_factor, const float *gw_factor_corr, const double *m_gw_f,
+                 const double *m_rg_max, int32_t time_series, float *gw_factor, int16_t *rgmax);
+
 /* ---- routing preprocessing (host): prepare_routing_files, rout_prepare.cpp:61-336 ----
  * Cell arrays in the reference's cell order; cell numbers in inflow9 / outflow are
  * 1-based with 0 = none, as in G_INFLC.9.UNF4 / G_OUTFLC.UNF4. */

@@ -216,6 +216,16 @@ def _declare(lib):
     lib.wg_alloc_coeff.argtypes = [i32, P_i32, P_d, P_d, P_d]
     lib.wg_reservoir_start_month.argtypes = [i32, P_d, P_d, P_i8]
     lib.wg_prepare_routing.argtypes = [C.POINTER(RoutingInputs), C.POINTER(RoutingFiles)]
+    lib.wg_unf_type.argtypes = [C.c_char_p]
+    lib.wg_unf_read.argtypes = [C.c_char_p, i32, C.c_int64, C.c_void_p]
+    lib.wg_unf_write.argtypes = [C.c_char_p, i32, C.c_int64, C.c_void_p]
+    lib.wg_unf_read_double.argtypes = [C.c_char_p, C.c_int64, P_d]
+    lib.wg_unf_write_double.argtypes = [C.c_char_p, C.c_int64, P_d]
+    lib.wg_read_climate_year.argtypes = [C.c_char_p, i32, i32, P_f]
+    lib.wg_read_lct_tables.argtypes = [C.c_char_p, C.POINTER(LctTables)]
+    lib.wg_cell_geometry.argtypes = [i32, P_d, P_i16, P_d, P_d, P_d, P_d, P_d, P_d, P_d]
+    lib.wg_soil_capacity.argtypes = [i32, P_f, P_i8, P_d, P_f, P_f]
+    lib.wg_gw_factor.argtypes = [i32, P_i8, P_i8, P_i8, P_i8, P_f, P_d, P_d, i32, P_f, P_i16]
     lib.wg_last_error.restype = C.c_char_p
     lib.wg_last_error.argtypes = [C.c_void_p]
     lib.wg_default_options.argtypes = [C.POINTER(OptionsProfile)]
@@ -277,7 +287,9 @@ ABI_SYMBOLS = [
     "wg_default_options", "wg_inflow_outflow", "wg_rout_order",
     "wg_ldd_from_flowdir", "wg_replace_flow_to_ocean", "wg_flow_accumulation", "wg_derive_basins", "wg_reindex_basins", "wg_neighbour_cells",
     "wg_cell_distances", "wg_river_slope", "wg_river_length", "wg_alloc_coeff", "wg_reservoir_start_month",
-    "wg_prepare_routing", "wg_create", "wg_destroy",
+    "wg_prepare_routing", "wg_unf_type", "wg_unf_read", "wg_unf_write", "wg_unf_read_double",
+    "wg_unf_write_double", "wg_read_climate_year", "wg_read_lct_tables", "wg_cell_geometry", "wg_soil_capacity",
+    "wg_gw_factor", "wg_create", "wg_destroy",
     "wg_last_error", "wg_get_state", "wg_set_state", "wg_set_member_params",
     "wg_set_forcing_perturbation", "wg_load_forcing", "wg_set_stations", "wg_step_days",
     "wg_get_station_discharge", "wg_station_daily_window", "wg_reset_station_discharge", "wg_get_day_outputs",
