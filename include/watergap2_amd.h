@@ -252,6 +252,37 @@ int wg_gw_factor(int32_t n_cells, const int8_t *slope_class, const int8_t *textu
                  const int8_t *aquifer_factor, const float *gw_factor_corr, const double *m_gw_f,
                  const double *m_rg_max, int32_t time_series, float *gw_factor, int16_t *rgmax);
 
+/* ---- restart files (host, text variants) ---- */
+#define WG_NSTORE 10        /* canopy, snow, soil, local lake, local wetland, global lake, global
+                               wetland, reservoir, river, groundwater (wghmStateFile.h) */
+#define WG_NADDITIONAL 53   /* columns of the additional out- and input file */
+
+/* WGHM water storage states, mm over each cell's continental area [N][WG_NSTORE]
+ * (TWS, their sum, is written first and skipped on reading).  Replace
+ * WghmStateFile::saveMean / load for .txt names (wghmStateFile.cpp:156-229, 357-413). */
+int wg_write_wghm_state_txt(const char *path, int32_t n_cells, const double *storages_mm);
+int wg_read_wghm_state_txt(const char *path, int32_t n_cells, double *storages_mm);
+/* Snow of the 101 elevation channels, mm over the continental area [N][101].  Replace
+ * SnowInElevationFile::save / load for .txt names (snowInElevationFile.cpp:80-210). */
+int wg_write_snow_elevation_txt(const char *path, int32_t n_cells, const double *snow);
+int wg_read_snow_elevation_txt(const char *path, int32_t n_cells, double *snow);
+/* The 53 columns of the additional out- and input file [N][53].  Replace
+ * AdditionalOutputInputFile::save / load for .txt names
+ * (additionalOutputInputFile.cpp:186-400). */
+int wg_write_additional_txt(const char *path, int32_t n_cells, const double *cols);
+int wg_read_additional_txt(const char *path, int32_t n_cells, double *cols);
+/* A member's state to the three files' values and back: storages in mm over the
+ * continental area (routing.cpp:4998-5020, integrateWGHM.cpp:830-846;
+ * daily.cpp:1896-1924, routing.cpp:851-882), the LAI state, release coefficient,
+ * land-area fractions and reduction factors in the additional columns (lai.cpp:55-64,
+ * 165-173; routing.cpp:388-393, 3080, 5209-5233).  Any output may be NULL; NULL
+ * state arrays are skipped.  The river's reduction factor and area fraction and the
+ * storage transfer are not in the reference's files and are left as they are. */
+int wg_state_to_files(int32_t n_cells, const wg_static_fields *st, const wg_state *state, double *storages_mm,
+                      double *snow_mm, double *additional);
+int wg_state_from_files(int32_t n_cells, const wg_static_fields *st, const double *storages_mm,
+                        const double *snow_mm, const double *additional, wg_state *state);
+
 /* ---- routing preprocessing (host): prepare_routing_files, rout_prepare.cpp:61-336 ----
  * Cell arrays in the reference's cell order; cell numbers in inflow9 / outflow are
  * 1-based with 0 = none, as in G_INFLC.9.UNF4 / G_OUTFLC.UNF4. */

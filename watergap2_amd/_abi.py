@@ -226,6 +226,11 @@ def _declare(lib):
     lib.wg_cell_geometry.argtypes = [i32, P_d, P_i16, P_d, P_d, P_d, P_d, P_d, P_d, P_d]
     lib.wg_soil_capacity.argtypes = [i32, P_f, P_i8, P_d, P_f, P_f]
     lib.wg_gw_factor.argtypes = [i32, P_i8, P_i8, P_i8, P_i8, P_f, P_d, P_d, i32, P_f, P_i16]
+    for fn in ("wg_write_wghm_state_txt", "wg_read_wghm_state_txt", "wg_write_snow_elevation_txt",
+               "wg_read_snow_elevation_txt", "wg_write_additional_txt", "wg_read_additional_txt"):
+        getattr(lib, fn).argtypes = [C.c_char_p, i32, P_d]
+    lib.wg_state_to_files.argtypes = [i32, C.POINTER(StaticFields), C.POINTER(StateStruct), P_d, P_d, P_d]
+    lib.wg_state_from_files.argtypes = [i32, C.POINTER(StaticFields), P_d, P_d, P_d, C.POINTER(StateStruct)]
     lib.wg_last_error.restype = C.c_char_p
     lib.wg_last_error.argtypes = [C.c_void_p]
     lib.wg_default_options.argtypes = [C.POINTER(OptionsProfile)]
@@ -289,7 +294,9 @@ ABI_SYMBOLS = [
     "wg_cell_distances", "wg_river_slope", "wg_river_length", "wg_alloc_coeff", "wg_reservoir_start_month",
     "wg_prepare_routing", "wg_unf_type", "wg_unf_read", "wg_unf_write", "wg_unf_read_double",
     "wg_unf_write_double", "wg_read_climate_year", "wg_read_lct_tables", "wg_cell_geometry", "wg_soil_capacity",
-    "wg_gw_factor", "wg_create", "wg_destroy",
+    "wg_gw_factor", "wg_write_wghm_state_txt", "wg_read_wghm_state_txt", "wg_write_snow_elevation_txt",
+    "wg_read_snow_elevation_txt", "wg_write_additional_txt", "wg_read_additional_txt", "wg_state_to_files",
+    "wg_state_from_files", "wg_create", "wg_destroy",
     "wg_last_error", "wg_get_state", "wg_set_state", "wg_set_member_params",
     "wg_set_forcing_perturbation", "wg_load_forcing", "wg_set_stations", "wg_step_days",
     "wg_get_station_discharge", "wg_station_daily_window", "wg_reset_station_discharge", "wg_get_day_outputs",
