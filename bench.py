@@ -14,16 +14,26 @@ Besides the JSON line's value, the run reports
     the same kernels over a profiling window after the timed region), against
     the 8 TB/s HBM peak, with the HBM traffic per launch when a rocprof counter
     summary is present under profiles/;
+  * deep: the same measurement on a global grid of 351 routing levels (60 ocean
+    outlets), where the routing chain is four times as deep;
+  * ensemble: 256 perturbed members of the global grid for one year;
+  * calibration: basins calibrated per hour at the 1 % criterion with the
+    reference's gamma search (120 synthetic basins, 30 years), and 64 candidates
+    per basin in one run;
+  * surrogate: basins per hour with the trained network's proposals (weights
+    in watergap2_amd/data) against the plain search on 500 unseen basins;
   * cpu_baseline: the CPU oracle built with the C library's exp/pow, vertical
-    balance on all host threads (up to 16) and serial routing, as the reference
-    runs it, on a bounded sample of the same workload on this host; plus its
-    single-thread rate.
+    balance on OpenMP threads and serial routing, as the reference runs it, on a
+    bounded sample of the same workload on this host, at 1, 8 (the reference's
+    setting) and all host threads.
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -204,52 +214,55 @@ def calibration_run(args):
 def surrogate_run(args):
     """Metric 2 with the learned part: basins calibrated per hour with surrogate
     proposals (verified by the plain model, plain search as the fallback) against
-    the plain search alone, on unseen basins, at the same 1 % criterion."""
-    import numpy as np
-    from watergap2_amd.calibration import BasinRunner, BasinSet, make_basins, plain_search, within_1pc
+    the plain search alone, on unseen basins, at the same 1 % criterion.  The
+    network comes from --sur-weights (written by watergap2_amd.train_surrogate)
+    unless --sur-data asks for training here."""
+    from watergap2_amd.calibration import BasinRunner, make_basins, plain_search
     from watergap2_amd import surrogate as sg
 
     def log(msg):
         print("# " + msg, file=sys.stderr, flush=True)
 
     out = {}
-    t0 = time.perf_counter()
-    if args.sur_data and all(os.path.exists(p) for p in args.sur_data.split(",")):
-        parts = [np.load(p) for p in args.sur_data.split(",")]
-        x = np.concatenate([d["x"] for d in parts])
-        t = np.concatenate([d["t"] for d in parts])
-        log(f"training set loaded from {args.sur_data}: {len(t)} samples")
-        gen_s = None
+    if not args.sur_data and os.path.exists(args.sur_weights):
+        model, winfo = sg.Surrogate.load(args.sur_weights)
+        out["training"] = {"weights": os.path.relpath(args.sur_weights, ROOT)}
+        out["training"].update({k: (v.item() if np.ndim(v) == 0 else v.tolist()) for k, v in winfo.items()})
+        spread = float(winfo.get("residual_sd", 0.024))
     else:
-        x, t = sg.generate_training_set(args.sur_train_basins, args.sur_candidates, seed=args.sur_seed, log=log)
-        gen_s = time.perf_counter() - t0
-        if args.sur_data:
-            os.makedirs(os.path.dirname(os.path.abspath(args.sur_data)), exist_ok=True)
-            np.savez(args.sur_data, x=x, t=t)
-    # The reference's land-area update can drive a cell's land fraction to ~0 and
-    # back; with CFA > 1 the rescaled soil storage then turns slightly negative and
-    # pow() of it gives NaN (the oracle does the same): such runs are dropped.
-    finite = np.isfinite(t) & np.isfinite(x).all(axis=1)
-    n_dropped = int((~finite).sum())
-    x, t = x[finite], t[finite]
-    out["training_set"] = {"samples": int(len(t)), "dropped_non_finite": n_dropped, "basins": args.sur_train_basins,
-                           "candidates_per_basin": args.sur_candidates, "generation_seconds":
-                           None if gen_s is None else round(gen_s, 1)}
-    if args.sur_generate_only:
-        return out
-    # hold out 5 % of the samples for validation
-    rng = np.random.default_rng(5)
-    perm = rng.permutation(len(t))
-    n_val = len(t) // 20
-    val, tr = perm[:n_val], perm[n_val:]
-    t1 = time.perf_counter()
-    model, losses = sg.Surrogate.fit(x[tr], t[tr], epochs=(args.sur_epochs, args.sur_epochs // 2), log=log)
-    train_s = time.perf_counter() - t1
-    pv = model.predict(x[val])
-    val_rmse = float(np.sqrt(np.mean((pv - t[val]) ** 2)))
-    out["training"] = {"seconds": round(train_s, 2), "final_loss": float(losses[-1]),
-                       "validation_rmse_log_runoff": round(val_rmse, 5)}
-    log(f"trained in {train_s:.1f}s, validation RMSE of log runoff {val_rmse:.4f}")
+        t0 = time.perf_counter()
+        if args.sur_data and all(os.path.exists(p) for p in args.sur_data.split(",")):
+            parts = [np.load(p) for p in args.sur_data.split(",")]
+            x = np.concatenate([d["x"] for d in parts])
+            t = np.concatenate([d["t"] for d in parts])
+            log(f"training set loaded from {args.sur_data}: {len(t)} samples")
+            gen_s = None
+        else:
+            x, t = sg.generate_training_set(args.sur_train_basins, args.sur_candidates, seed=args.sur_seed, log=log)
+            gen_s = time.perf_counter() - t0
+            if args.sur_data:
+                os.makedirs(os.path.dirname(os.path.abspath(args.sur_data)), exist_ok=True)
+                np.savez(args.sur_data, x=x, t=t)
+        # runs with a non-finite discharge are dropped (see train_surrogate)
+        finite = np.isfinite(t) & np.isfinite(x).all(axis=1)
+        n_dropped = int((~finite).sum())
+        x, t = x[finite], t[finite]
+        out["training_set"] = {"samples": int(len(t)), "dropped_non_finite": n_dropped,
+                               "basins": args.sur_train_basins, "candidates_per_basin": args.sur_candidates,
+                               "generation_seconds": None if gen_s is None else round(gen_s, 1)}
+        if args.sur_generate_only:
+            return out
+        perm = np.random.default_rng(5).permutation(len(t))
+        n_val = len(t) // 20
+        val, tr = perm[:n_val], perm[n_val:]
+        t1 = time.perf_counter()
+        model, losses = sg.Surrogate.fit(x[tr], t[tr], epochs=(args.sur_epochs, args.sur_epochs // 2), log=log)
+        train_s = time.perf_counter() - t1
+        res_v = model.predict(x[val]) - t[val]
+        spread = float(np.std(res_v))
+        out["training"] = {"seconds": round(train_s, 2), "final_loss": float(losses[-1]),
+                           "validation_rmse_log_runoff": round(float(np.sqrt(np.mean(res_v ** 2))), 5)}
+        log(f"trained in {train_s:.1f}s, validation RMSE of log runoff {out['training']['validation_rmse_log_runoff']}")
 
     # unseen basins, targets from a hidden gamma
     bs = make_basins(args.sur_eval_basins, seed=424242)
@@ -261,43 +274,151 @@ def surrogate_run(args):
     measured = runner.run(hidden[None, :], n_years=years)[0].astype(np.float32)
     log(f"evaluation set: {bs.n_basins} basins, {bs.n} cells")
 
-    # plain search
     t1 = time.perf_counter()
     res = plain_search(runner, measured, n_years=years)
     plain_s = time.perf_counter() - t1
     plain_ok = int((res["status"] == 1).sum())
     log(f"plain search: {res['model_runs']} runs in {plain_s:.1f}s, {plain_ok} within 1 %")
 
-    # surrogate-guided calibration: proposals verified by the plain model, a second
-    # round of corrected proposals, the plain search for what still misses
     t1 = time.perf_counter()
-    sres = sg.calibrate_with_surrogate(model, runner, feats, measured, log=log)
+    sres = sg.calibrate_with_surrogate(model, runner, feats, measured, first_members=args.sur_first_members,
+                                       first_spread=spread, log=log)
     sur_s = time.perf_counter() - t1
     sur_ok = int(sres["within_1pc"].sum())
     runner.close()
     by_round = np.bincount(sres["round"], minlength=7)[1:].tolist()
-    first_hits = by_round[0]
     fb = by_round[5]
-    log(f"surrogate: basins per round {by_round[:5]} (three surrogate, two bracketed), {fb} to the plain "
-        f"fallback ({sres['fallback_runs']} runs), {sur_s:.1f}s")
     plain_rate = plain_ok / plain_s * 3600.0
     sur_rate = sur_ok / sur_s * 3600.0
     out.update({
         "metric": "basins calibrated/hr at the 1% criterion, surrogate proposals vs plain search",
         "value": round(sur_rate, 1), "unit": "basins/hr", "higher_is_better": True,
         "plain_basins_per_hr": round(plain_rate, 1), "gain": round(sur_rate / max(plain_rate, 1e-9), 2),
+        "gain_ceiling": {"lock_step_runs_of_plain_search": res["model_runs"],
+                         "plain_runs_per_basin": round(float(res["runs"].mean()), 2)},
         "config": {"workload": f"{bs.n_basins} unseen synthetic basins (log-uniform 50-5,000 cells), {bs.n} "
                                f"cells, {years} years, targets from a hidden gamma U(0.1, 5)",
-                   "basins": bs.n_basins, "cells": bs.n, "years": years},
+                   "basins": bs.n_basins, "cells": bs.n, "years": years,
+                   "first_round_proposals_per_basin": args.sur_first_members},
         "plain": {"seconds": round(plain_s, 2), "model_runs": res["model_runs"], "within_1pc": plain_ok,
                   "runs_per_basin_mean": round(float(res["runs"].mean()), 2)},
-        "surrogate": {"seconds": round(sur_s, 2), "first_round_hits": first_hits,
+        "surrogate": {"seconds": round(sur_s, 2), "first_round_hits": by_round[0],
+                      "first_round_hit_rate": round(by_round[0] / bs.n_basins, 3),
                       "hits_per_round": by_round[:5], "fallback_basins": fb,
                       "fallback_runs": sres["fallback_runs"], "within_1pc": sur_ok,
                       "runs_per_basin_mean": round(float(sres["runs"].mean()), 2),
                       "median_abs_gamma_error": round(float(np.median(np.abs(sres["gamma"] - hidden))), 4)},
     })
     return out
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline_run(world, forcing, args):
+    """The reference's arithmetic on the host (the oracle with glibc's exp/pow):
+    vertical balance over cells on OpenMP threads (integrateWGHM.cpp:767), serial
+    routing, station sums through an index map; timed on bounded samples at 1, 8
+    and all host threads, plus the two scans of the reference as footnotes."""
+    from oracle import wg_oracle
+    n = world.n
+    nproc = host_threads()
+
+    def rate(threads, days, station_scan=False, aggr_scan=False):
+        days = max(1, min(days, forcing.shape[0]))
+        o = wg_oracle.Oracle(world, variant="glibc")
+        o.set_stations(world.basin_outlet)
+        o.set_threads(threads)
+        o.set_reference_scans(station_scan, aggr_scan)
+        t1 = time.perf_counter()
+        o.run(forcing[:days], 1)
+        sec = time.perf_counter() - t1
+        return n * days / sec, days, sec
+
+    by = {}
+    for th, days in ((1, args.cpu_days_single), (8, args.cpu_days), (nproc, args.cpu_days)):
+        v, d, sec = rate(th, days)
+        by[str(th)] = {"value": round(v, 1), "days": d, "seconds": round(sec, 2)}
+        print(f"# cpu oracle {v:.4g} cell-days/s on {th} threads ({d} days)", file=sys.stderr, flush=True)
+    scan_v, scan_d, _ = rate(nproc, args.cpu_days_footnote, station_scan=True)
+    aggr_v, aggr_d, _ = rate(nproc, args.cpu_days_footnote, station_scan=True, aggr_scan=True)
+    best = by[str(nproc)]
+    return {"value": best["value"], "unit": "cell-days/s", "cores": nproc, "kind": "port",
+            "sample": (f"CPU oracle (glibc libm, gcc -O2 -ffp-contract=off), calcNewDay over cells on {nproc} "
+                       f"OpenMP threads, serial routing; {n} cells x {best['days']} days of the same forcing in "
+                       f"{best['seconds']}s"),
+            "cpu_model": cpu_model(), "nproc": nproc, "by_threads": by,
+            "footnote": {"reference_station_scan": {"value": round(scan_v, 1), "days": scan_d,
+                                                    "note": "O(N x stations) station search of routing.cpp:4232-4238"},
+                         "reference_default_options": {"value": round(aggr_v, 1), "days": aggr_d,
+                                                       "note": "plus the O(N) riparian scan per lake/reservoir cell "
+                                                               "of aggrNUsGloLakResOpt=1 (routing.cpp:3083-3160)"}}}
+
+
+def day_loop_leg(wg, world, n_years, warmup_years, profile_days, autotune):
+    """Graph-replayed years of one member after warm-up; per-kernel HIP-event
+    durations over an eager window afterwards."""
+    n = world.n
+    dps = 365
+    n_days = max((warmup_years + n_years) * dps, profile_days, 1)
+    forcing = world.forcing(seed=1, day0=0, n_days=n_days)
+    model = wg.WaterBalanceModel(world)
+    model.set_stations(world.basin_outlet)
+    model.load_forcing(forcing)
+    info = model.schedule_info()
+    if autotune:
+        tuned, tuned_ms = model.autotune(n_days=5)
+    else:
+        tuned, tuned_ms = (256, 512, 5, 3), 0.0
+    for s in range(warmup_years):
+        model.step_days(s * dps, 1, dps)
+    model.synchronize()
+    t_start = time.perf_counter()
+    for s in range(warmup_years, warmup_years + n_years):
+        model.step_days(s * dps, 1, dps)
+    model.synchronize()
+    elapsed = time.perf_counter() - t_start
+    pd = max(1, min(profile_days, n_days, 366))
+    model.step_days(0, 1, pd, profiled=True)
+    ms, launches = model.timing()
+    model.close()
+    return {"forcing": forcing, "info": info, "tuned": tuned, "tuned_ms": tuned_ms, "elapsed": elapsed,
+            "cell_ms": ms[0] / max(launches[0], 1), "chain_ms": ms[1] / max(launches[1], 1)}
+
+
+def deep_run(wg, args):
+    """The day loop on a global grid of the survey's routing depth: the same
+    67,420-cell mask with 60 ocean outlets, 351 routing levels."""
+    world = wg.World(wg.global_spec(seed=1, n_outlets=60))
+    leg = day_loop_leg(wg, world, args.deep_years, 1, 30, autotune=not args.no_autotune)
+    n = world.n
+    days = 365 * args.deep_years
+    l0 = leg["info"]["level0_cells"]
+    chain_b = chain_kernel_bytes(n, 1, l0)
+    return {"metric": "cell-days/s at global 0.5deg (FP64), deep routing network",
+            "value": round(n * days / leg["elapsed"], 1), "unit": "cell-days/s",
+            "config": {"workload": f"global 0.5deg synthetic land mask, {n} cells, seed 1, 60 ocean outlets, "
+                                   f"{args.deep_years} years of daily steps", "cells": n, "basins": world.n_basins,
+                       "routing_levels": leg["info"]["max_depth"],
+                       "largest_basin_cells": int(np.bincount(world.basin).max())},
+            "us_per_day": round(leg["elapsed"] / days * 1e6, 2),
+            "wg_cell_kernel_us": round(leg["cell_ms"] * 1e3, 2),
+            "wg_chain_kernel_us": round(leg["chain_ms"] * 1e3, 2),
+            "chain_kernel_achieved_GBs": round(chain_b / (leg["chain_ms"] * 1e-3) / 1e9, 1)}
 
 
 def traffic_from_profiles(kernel):
@@ -320,12 +441,19 @@ def main():
     ap.add_argument("--days-per-step", type=int, default=365)
     ap.add_argument("--profile-days", type=int, default=60)
     ap.add_argument("--cpu-days", type=int, default=150,
-                    help="days of the multi-threaded CPU-oracle sample (bounded CPU baseline)")
+                    help="days of the multi-threaded CPU-oracle samples (bounded CPU baseline)")
     ap.add_argument("--cpu-days-single", type=int, default=60,
                     help="days of the single-thread CPU-oracle sample")
-    ap.add_argument("--cpu-threads", type=int, default=16,
-                    help="OpenMP threads of the CPU oracle's vertical loop (capped at the host's)")
+    ap.add_argument("--cpu-days-footnote", type=int, default=30,
+                    help="days of the samples with the reference's scans")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the day loop (no deep network, ensemble, calibration or surrogate legs)")
+    ap.add_argument("--deep-years", type=int, default=1, help="timed years of the deep-network leg (0 = skip)")
+    ap.add_argument("--sur-weights", default=os.path.join(ROOT, "watergap2_amd", "data", "surrogate_weights.npz"),
+                    help="trained surrogate (python -m watergap2_amd.train_surrogate)")
+    ap.add_argument("--sur-first-members", type=int, default=1,
+                    help="proposals per basin verified in the surrogate's first run")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch configuration (profiling runs)")
     ap.add_argument("--ensemble-members", type=int, default=256,
@@ -334,7 +462,7 @@ def main():
                     help="members timed one by one for the ensemble comparison")
     ap.add_argument("--calibration", action="store_true",
                     help="measure the calibration metric (basins/hr) instead of the day loop")
-    ap.add_argument("--cal-basins", type=int, default=120)
+    ap.add_argument("--cal-basins", type=int, default=120, help="(0 = skip the calibration leg)")
     ap.add_argument("--cal-years", type=int, default=30)
     ap.add_argument("--cal-candidates", type=int, default=64)
     ap.add_argument("--surrogate", action="store_true",
@@ -345,7 +473,7 @@ def main():
     ap.add_argument("--sur-seed", type=int, default=9001, help="seed of the training basins")
     ap.add_argument("--sur-candidates", type=int, default=40)
     ap.add_argument("--sur-epochs", type=int, default=30)
-    ap.add_argument("--sur-eval-basins", type=int, default=500)
+    ap.add_argument("--sur-eval-basins", type=int, default=500, help="(0 = skip the surrogate leg)")
     ap.add_argument("--ensemble-counters", type=int, default=0,
                     help="only run this many eagerly launched days of the batched ensemble (counter runs) "
                          "and exit")
@@ -378,56 +506,28 @@ def main():
                           "per-day exchange (see DESIGN.md), but only --gpus 1 is measured"}))
         sys.exit(2)
 
-    import numpy as np
     import watergap2_amd as wg
 
     t0 = time.time()
     world = wg.World(wg.global_spec(seed=1))
     n = world.n
-    dps = args.days_per_step
-    n_days = max((args.warmup + args.steps) * dps, args.profile_days, 1)
-    forcing = world.forcing(seed=1, day0=0, n_days=n_days)
-    model = wg.WaterBalanceModel(world)
-    model.set_stations(world.basin_outlet)
-    model.load_forcing(forcing)
-    info = model.schedule_info()
-    print(f"# setup {time.time() - t0:.1f}s: {n} cells, {world.n_basins} basins, "
-          f"{info['groups']} routing groups, depth {info['max_depth']} levels, "
-          f"{info['level0_cells']} headwater cells", file=sys.stderr, flush=True)
+    dps = 365
+    print(f"# setup {time.time() - t0:.1f}s: {n} cells, {world.n_basins} basins", file=sys.stderr, flush=True)
 
-    # launch configuration chosen by the library's autotuner (state restored after)
-    if args.no_autotune:
-        tuned, tuned_ms = (256, 512, 5, 3), 0.0  # the library default at one member
-    else:
-        tuned, tuned_ms = model.autotune(n_days=5)
-    print(f"# autotune: cell tile {tuned[0]}, chain workgroup {tuned[1]}, snow chunk {tuned[2]}, "
-          f"chain mode {tuned[3]} ({tuned_ms * 1e3:.1f} us/day eager)",
-          file=sys.stderr, flush=True)
-
-    # warmup (also builds the graphs)
-    for s in range(args.warmup):
-        model.step_days(s * dps, 1, dps)
-    model.synchronize()
-    # timed region: K steps bracketed by device synchronisation
-    model.synchronize()
-    t_start = time.perf_counter()
-    for s in range(args.warmup, args.warmup + args.steps):
-        model.step_days(s * dps, 1, dps)
-    model.synchronize()
-    elapsed = time.perf_counter() - t_start
+    # timed region: K steps (simulated years) of graph-replayed days, bracketed by
+    # device synchronisation, after W warm-up years (which also capture the graphs)
+    leg = day_loop_leg(wg, world, args.steps, args.warmup, args.profile_days, autotune=not args.no_autotune)
+    forcing, info, tuned = leg["forcing"], leg["info"], leg["tuned"]
+    elapsed = leg["elapsed"]
     timed_days = args.steps * dps
-    # --steps 0 (profiling runs) times nothing
     value = n * timed_days / elapsed if timed_days else 0.0
     ms_per_step = 1000.0 * elapsed / args.steps if args.steps else 0.0
-    print(f"# timed {timed_days} days in {elapsed:.3f}s -> {value:.4g} cell-days/s",
-          file=sys.stderr, flush=True)
+    print(f"# autotune: cell tile {tuned[0]}, chain workgroup {tuned[1]}, snow chunk {tuned[2]}, "
+          f"chain mode {tuned[3]} ({leg['tuned_ms'] * 1e3:.1f} us/day eager)", file=sys.stderr, flush=True)
+    print(f"# timed {timed_days} days in {elapsed:.3f}s -> {value:.4g} cell-days/s; depth {info['max_depth']} "
+          f"levels", file=sys.stderr, flush=True)
 
-    # per-kernel durations: eager launches with HIP events on the model's stream
-    pd = max(1, min(args.profile_days, n_days))
-    model.step_days(0, 1, pd, profiled=True)
-    ms, launches = model.timing()
-    cell_ms = ms[0] / max(launches[0], 1)
-    chain_ms = ms[1] / max(launches[1], 1)
+    cell_ms, chain_ms = leg["cell_ms"], leg["chain_ms"]
     l0 = info["level0_cells"]
     kern = {
         "wg_cell_kernel": (cell_ms, cell_kernel_bytes(n, 1, l0)),
@@ -457,44 +557,38 @@ def main():
         # the whole simulated day against the algorithmic bytes of both kernels
         "day": {"bytes_per_cell_day": round(day_bytes / n, 1), "us_per_day": round((cell_ms + chain_ms) * 1e3, 2),
                 "achieved_GBs": round(day_bytes / ((cell_ms + chain_ms) * 1e-3) / 1e9, 1),
-                "frac": round(day_bytes / ((cell_ms + chain_ms) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "note": ("wg_chain_kernel (the wave walk) routes each routing group level by level in "
-                 "batches of 64 cells, 104 dependent levels at this grid; its time is latency, "
-                 "not bandwidth"),
+                "frac": round(day_bytes / ((cell_ms + chain_ms) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "graph_us_per_day": round(elapsed / max(timed_days, 1) * 1e6, 2)},
+        "note": (f"wg_chain_kernel (the wave walk) routes each routing group level by level in "
+                 f"batches of 64 cells, {info['max_depth']} dependent levels at this grid; its time is "
+                 f"latency, not bandwidth"),
     }
 
-    cpu = None
-    if not args.no_cpu:
-        from oracle import wg_oracle
-        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
-        cd = max(1, min(args.cpu_days, n_days))
-        o = wg_oracle.Oracle(world, variant="glibc")
-        o.set_stations(world.basin_outlet)
-        o.set_threads(threads)
-        t1 = time.perf_counter()
-        o.run(forcing[:cd], 1)
-        cpu_s = time.perf_counter() - t1
-        cpu_value = n * cd / cpu_s
-        cd1 = max(1, min(args.cpu_days_single, n_days))
-        o1 = wg_oracle.Oracle(world, variant="glibc")
-        t1 = time.perf_counter()
-        o1.run(forcing[:cd1], 1)
-        single_s = time.perf_counter() - t1
-        cpu = {"value": round(cpu_value, 1), "unit": "cell-days/s", "cores": threads, "kind": "port",
-               "sample": (f"CPU oracle (glibc libm, -O2 -ffp-contract=off), calcNewDay over cells on "
-                          f"{threads} OpenMP threads, serial routing; {n} cells x {cd} days of the same "
-                          f"forcing in {cpu_s:.1f}s"),
-               "single_thread_value": round(n * cd1 / single_s, 1),
-               "single_thread_sample": f"{n} cells x {cd1} days in {single_s:.1f}s"}
-        print(f"# cpu oracle {cpu_value:.4g} cell-days/s on {threads} threads, "
-              f"{n * cd1 / single_s:.4g} on 1", file=sys.stderr, flush=True)
+    cpu = None if args.no_cpu else cpu_baseline_run(world, forcing, args)
+
+    deep = None
+    if not args.no_extras and args.deep_years > 0:
+        deep = deep_run(wg, args)
+        print(f"# deep network: {deep['value']:.4g} cell-days/s, {deep['config']['routing_levels']} levels, "
+              f"chain {deep['wg_chain_kernel_us']} us/day", file=sys.stderr, flush=True)
 
     ens = None
-    if args.ensemble_members > 0:
-        ens = ensemble_run(wg, world, forcing if n_days >= 365 else world.forcing(seed=1, day0=0, n_days=365),
+    if not args.no_extras and args.ensemble_members > 0:
+        ens = ensemble_run(wg, world, forcing if forcing.shape[0] >= 365 else world.forcing(seed=1, day0=0, n_days=365),
                            args.ensemble_members, max(1, args.serial_members))
         print(f"# ensemble {ens['members']} members x 1 year: {ens['cell_days_per_s']:.4g} cell-days/s, "
               f"{ens['speedup_vs_serial']}x the serial member runs", file=sys.stderr, flush=True)
+
+    cal = None
+    if not args.no_extras and args.cal_basins > 0:
+        cal = calibration_run(args)
+        print(f"# calibration: {cal['value']:.4g} basins/hr (plain search)", file=sys.stderr, flush=True)
+
+    sur = None
+    if not args.no_extras and args.sur_eval_basins > 0 and os.path.exists(args.sur_weights):
+        sur = surrogate_run(args)
+        print(f"# surrogate: {sur['value']:.4g} basins/hr, {sur['gain']}x the plain search", file=sys.stderr,
+              flush=True)
 
     line = {
         "metric": "cell-days/s at global 0.5deg (FP64)",
@@ -509,8 +603,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "global 0.5deg synthetic land mask, 67,420 cells, seed 1, 100 elevation "
-                               "sub-cells, 10 years of daily steps, hipGraph-captured days",
+        "config": {"workload": (f"global 0.5deg synthetic land mask, {n} cells, seed 1, 100 elevation "
+                                f"sub-cells, {args.steps} years of daily steps, hipGraph-captured days"),
                    "cells": n, "days_per_step": dps, "members": 1, "basins": world.n_basins,
                    "routing_levels": info["max_depth"],
                    "launch": {"cell_tile": tuned[0], "chain_workgroup": tuned[1], "snow_chunk": tuned[2],
@@ -519,7 +613,10 @@ def main():
                                                "wave walk in 64-cell batches"][tuned[3]]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "deep": deep,
         "ensemble": ens,
+        "calibration": cal,
+        "surrogate": sur,
     }
     print(json.dumps(line))
 

@@ -75,6 +75,10 @@ struct wgo_model {
   /* stations */
   int32_t n_st;
   int32_t *st_cell;
+  int32_t *st_slot;        /* first station of each cell, -1 = none (index map) */
+  int station_scan;        /* 1: the reference's scan of all stations per cell */
+  int aggr_scan;           /* 1: the O(N) riparian scan of aggrNUsGloLakResOpt = 1 */
+  int64_t aggr_matches;
   double *st_sum;
 };
 
@@ -311,7 +315,7 @@ void wgo_destroy(wgo_model *m) {
                   &m->river_inflow, &m->discharge, &m->st_sum};
   for (size_t i = 0; i < sizeof(d) / sizeof(d[0]); i++) free(*d[i]);
   free(m->smax); free(m->lai_max); free(m->contcell); free(m->routing_cell);
-  free(m->lai_days); free(m->lai_status); free(m->st_cell);
+  free(m->lai_days); free(m->lai_status); free(m->st_cell); free(m->st_slot);
   free(m);
 }
 
@@ -1020,6 +1024,16 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
     if (m->red_res[i] > 1.) m->red_res[i] = 1.;
   }
 
+  /* routing.cpp:3083-3160: with aggrNUsGloLakResOpt = 1 every global lake or
+   * reservoir cell scans all cells for riparian cells of its lake unit; without
+   * water use nothing is assigned.  The scan's cost is kept for the baseline's
+   * footnote (each cell is its own unit here, so the scan finds only itself). */
+  if (m->aggr_scan && (st->reservoir_area[i] > 0. || st->lake_area[i] > 0.)) {
+    int64_t hits = 0;
+    for (int32_t k = 0; k < n; k++) hits += (m->routing_cell[k] == i + 1);
+    m->aggr_matches += hits;
+  }
+
   /* routing.cpp:3178-3297 global wetland */
   if (st->glo_wetland[i] > 0) {
     double prev = m->glo_wetl[i], rf = m->red_gw[i], evap, PETgwr, PETgwrMax;
@@ -1120,12 +1134,17 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
   m->discharge[i] = tv;
   /* routing.cpp:3955-3958 the only write to another cell */
   if ((st->downstream[i] - 1) >= 0) m->river_inflow[st->downstream[i] - 1] += tv;
-  /* routing.cpp:4232-4238 station discharge */
-  for (int32_t b = 0; b < m->n_st; b++) {
-    if (i == m->st_cell[b]) {
-      if (ldd >= 0) m->st_sum[b] += tv;
-      break;
+  /* routing.cpp:4232-4238 station discharge: the first station of the cell; the
+   * reference finds it by scanning all stations (kept as an option for timing) */
+  if (m->station_scan) {
+    for (int32_t b = 0; b < m->n_st; b++) {
+      if (i == m->st_cell[b]) {
+        if (ldd >= 0) m->st_sum[b] += tv;
+        break;
+      }
     }
+  } else if (m->st_slot && m->st_slot[i] >= 0) {
+    if (ldd >= 0) m->st_sum[m->st_slot[i]] += tv;
   }
   /* routing.cpp:4517 */
   m->river_inflow[i] = 0.;
@@ -1295,10 +1314,21 @@ int wgo_get_day_outputs(const wgo_model *m, wg_day_outputs *o) {
 void wgo_set_stations(wgo_model *m, int32_t n_st, const int32_t *cells) {
   free(m->st_cell);
   free(m->st_sum);
+  free(m->st_slot);
   m->n_st = n_st;
   m->st_cell = (int32_t *)malloc((size_t)(n_st > 0 ? n_st : 1) * sizeof(int32_t));
   m->st_sum = (double *)calloc((size_t)(n_st > 0 ? n_st : 1), sizeof(double));
-  for (int32_t b = 0; b < n_st; b++) m->st_cell[b] = cells[b];
+  m->st_slot = (int32_t *)malloc((size_t)m->n * sizeof(int32_t));
+  for (int32_t i = 0; i < m->n; i++) m->st_slot[i] = -1;
+  for (int32_t b = 0; b < n_st; b++) {
+    m->st_cell[b] = cells[b];
+    if (cells[b] >= 0 && cells[b] < m->n && m->st_slot[cells[b]] < 0) m->st_slot[cells[b]] = b;
+  }
+}
+
+void wgo_set_reference_scans(wgo_model *m, int station_scan, int aggr_scan) {
+  m->station_scan = station_scan;
+  m->aggr_scan = aggr_scan;
 }
 
 void wgo_station_discharge(const wgo_model *m, double *sum) {

@@ -75,6 +75,10 @@ int wgo_get_day_outputs(const wgo_model *m, wg_day_outputs *out);
 void wgo_set_stations(wgo_model *m, int32_t n_st, const int32_t *cells);
 void wgo_station_discharge(const wgo_model *m, double *sum);
 void wgo_reset_station_discharge(wgo_model *m);
+/* Timing options of the CPU baseline: the reference's O(N x stations) station
+ * search instead of the index map, and the O(N) riparian scan per lake/reservoir
+ * cell of aggrNUsGloLakResOpt = 1.  Neither changes a result. */
+void wgo_set_reference_scans(wgo_model *m, int station_scan, int aggr_scan);
 
 /* Member forcing perturbation shared with the ensemble generator. */
 void wgo_perturb_forcing(uint64_t seed, int32_t day_index, int32_t n, double prec_sigma,

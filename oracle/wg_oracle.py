@@ -56,6 +56,8 @@ def load(variant="wg"):
     lib.wgo_station_discharge.argtypes = [C.c_void_p, P_d]
     lib.wgo_station_discharge.restype = None
     lib.wgo_reset_station_discharge.argtypes = [C.c_void_p]
+    lib.wgo_set_reference_scans.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.wgo_set_reference_scans.restype = None
     lib.wgo_reset_station_discharge.restype = None
     lib.wgo_perturb_forcing.argtypes = [C.c_uint64, i32, i32, C.c_double, C.c_double, P_f, P_f, P_f, P_f]
     lib.wgo_perturb_forcing.restype = None
@@ -263,6 +265,9 @@ class Oracle:
                 self._m = None
         except Exception:
             pass
+
+    def set_reference_scans(self, station_scan=False, aggr_scan=False):
+        self._lib.wgo_set_reference_scans(self._m, int(station_scan), int(aggr_scan))
 
     def set_threads(self, threads):
         self._lib.wgo_set_threads(self._m, int(threads))

@@ -51,12 +51,13 @@ def regional_spec(seed=0, nrows=40, ncols=50, row0=81, col0=361, water_frac_max=
 
 
 def global_spec(seed=1, n_land=67420, n_outlets=600, n_sinks=20, water_frac_max=0.1,
-                glolake_every=100, res_every=200, edge_cases=False):
+                glolake_every=100, res_every=200, edge_cases=False, relief_scale=0.0):
     """Global land mask (the 67,420-cell 0.5-degree workload)."""
     return _abi.WorldSpec(kind=1, nrows=0, ncols=0, row0=0, col0=0, n_land=n_land,
                           n_outlets=n_outlets, n_sinks=n_sinks, seed=seed,
                           water_frac_max=water_frac_max, glolake_every=glolake_every,
-                          res_every=res_every, edge_cases=1 if edge_cases else 0)
+                          res_every=res_every, edge_cases=1 if edge_cases else 0,
+                          relief_scale=relief_scale)
 
 
 class World:

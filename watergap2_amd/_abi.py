@@ -109,6 +109,7 @@ class WorldSpec(C.Structure):
         ("col0", C.c_int32), ("n_land", C.c_int32), ("n_outlets", C.c_int32),
         ("n_sinks", C.c_int32), ("seed", C.c_uint64), ("water_frac_max", C.c_double),
         ("glolake_every", C.c_int32), ("res_every", C.c_int32), ("edge_cases", C.c_int32),
+        ("relief_scale", C.c_double),
     ]
 
 

@@ -268,7 +268,7 @@ struct Builder {
         int r = p / kCols + 1, c = p % kCols + 1;
         cells_r.push_back(r);
         cells_c.push_back(c);
-        double v = rel.fbm(c - 0.5, r - 0.5, 40.0, 6, kCols);
+        double v = rel.fbm(c - 0.5, r - 0.5, sp.relief_scale > 0 ? sp.relief_scale : 40.0, 6, kCols);
         dem.push_back(float(v));
       }
     }

@@ -30,6 +30,7 @@ typedef struct wg_world_spec {
   int32_t glolake_every; /* one global lake per this many cells (0: none) */
   int32_t res_every;     /* one reservoir per this many cells (0: none) */
   int32_t edge_cases;    /* 1: add cells that exercise rarely taken branches */
+  double relief_scale;   /* kind 1: length scale (cells) of the relief's largest octave; 0 = 40 */
 } wg_world_spec;
 
 typedef struct wg_world {

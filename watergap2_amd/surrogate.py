@@ -215,11 +215,13 @@ def bracket_proposals(points, target, n=3):
 
 
 def calibrate_with_surrogate(model, runner, feats, measured, spread=0.006, rounds=3, bracket_rounds=2,
-                             log=None):
+                             first_members=1, first_spread=0.0, log=None):
     """Surrogate-guided calibration of every basin of the runner's set.
 
     Round 1: the network's proposal for each basin, verified by one model run of
-    the set.  Rounds 2..`rounds`, for the basins still missing the 1 % criterion:
+    the set; with `first_members` > 1 the proposals aim at the target shifted
+    across +- `first_spread` (log runoff, e.g. the network's residual spread), as
+    that many members of the one run.  Rounds 2..`rounds`, for the basins still missing the 1 % criterion:
     the network's curve corrected through the verified points (shifted by the
     miss after one point; shifted and scaled through the two closest points after
     two or more), aimed at the target and at the target +- `spread` (log runoff),
@@ -250,14 +252,32 @@ def calibrate_with_surrogate(model, runner, feats, measured, spread=0.006, round
 
     every = np.arange(nb)
     target = log_runoff(measured, every)
-    gam = model.propose_gamma(feats, target)
-    q = runner.run(gam[None, :], n_years=n_years)[0]
     rnd = np.zeros(nb, dtype=np.int32)
     runs = np.ones(nb, dtype=np.int32)
-    hit = np.array([within_1pc(q[b], measured[b]) for b in range(nb)])
+    if first_members <= 1:
+        gam = model.propose_gamma(feats, target)
+        q = runner.run(gam[None, :], n_years=n_years)[0]
+        hit = np.array([within_1pc(q[b], measured[b]) for b in range(nb)])
+        # verified points per basin: (gamma, log runoff)
+        points = [[(float(gam[b]), float(log_runoff(q[b:b + 1], [b])[0]))] for b in range(nb)]
+    else:
+        offs = np.linspace(-first_spread, first_spread, first_members)
+        # the central proposal first, so that it is kept when several hit
+        offs = offs[np.argsort(np.abs(offs), kind="stable")]
+        props = np.stack([model.propose_gamma(feats, target + o) for o in offs])
+        mr = BasinRunner(bs, n_members=first_members, forcing_years=runner.forcing_years, forcing=runner.forcing)
+        qm = mr.run(props, n_years=n_years)
+        mr.close()
+        gam = props[0].copy()
+        hit = np.zeros(nb, dtype=bool)
+        points = [[] for _ in range(nb)]
+        for b in range(nb):
+            for k in range(first_members):
+                points[b].append((float(props[k, b]), float(log_runoff(qm[k, b:b + 1], [b])[0])))
+                if not hit[b] and within_1pc(qm[k, b], measured[b]):
+                    hit[b] = True
+                    gam[b] = props[k, b]
     rnd[hit] = 1
-    # verified points per basin: (gamma, log runoff), kept sorted by distance to the target
-    points = [[(float(gam[b]), float(log_runoff(q[b:b + 1], [b])[0]))] for b in range(nb)]
     if log:
         log(f"surrogate round 1: {int(hit.sum())}/{nb} within 1 %")
     shifts = (0.0, -spread, spread)
@@ -350,6 +370,21 @@ class Surrogate:
             if log:
                 log(f"surrogate training: epoch {len(losses)}, loss {losses[-1]:.5f}")
         return cls(mlp, mean, std), np.array(losses)
+
+    def save(self, path, **info):
+        """Weights (FP32 master copy), the feature standardisation and any
+        training facts given as keywords, in one .npz file."""
+        np.savez(path, params=self.mlp.params(), mean=self.mean, std=self.std,
+                 **{k: np.asarray(v) for k, v in info.items()})
+
+    @classmethod
+    def load(cls, path, seed=0):
+        """A network from a file written by save(); returns (surrogate, info)."""
+        d = np.load(path, allow_pickle=False)
+        mlp = MLP(seed=seed)
+        mlp.set_params(d["params"])
+        info = {k: d[k] for k in d.files if k not in ("params", "mean", "std")}
+        return cls(mlp, d["mean"], d["std"]), info
 
     def predict_at(self, feats, gamma, cfa=None):
         """Predicted log runoff of each basin at its own gamma."""
