@@ -24,8 +24,18 @@ def test_global_grid(global_world):
     sizes = np.bincount(w.basin)
     assert 500 <= w.n_basins <= 1500
     assert (w.ldd == -1).sum() >= 1  # inland sinks
-    assert w.max_level >= 80
+    assert w.max_level >= 100  # 104 routing levels
     assert sizes.max() > 2000
+
+
+def test_deep_global_grid(wg):
+    """The same land mask drained by 60 ocean outlets: a routing network of the
+    depth the design plans for (about 350-400 levels)."""
+    w = wg.World(wg.global_spec(seed=1, n_outlets=60))
+    sizes = np.bincount(w.basin)
+    assert w.n == 67420
+    assert w.max_level + 1 >= 350
+    assert sizes.max() > 10000
 
 
 def test_deterministic(wg):

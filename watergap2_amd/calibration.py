@@ -14,8 +14,8 @@ Here every basin of a :class:`BasinSet` is one part of a single grid on the GPU:
 * :func:`run_candidates` evaluates K (gamma, CFA) candidates per basin in one run,
   candidates as model members.
 
-Basins are synthetic: each is a regional lattice cut from its own seeded fractal
-DEM with a single outlet, sized log-uniformly.  The forcing of a basin is a cycle
+Basins are synthetic: disjoint station catchments cut from a seeded global
+fractal DEM, sized log-uniformly.  The forcing of a basin is a cycle
 of ``forcing_years`` synthetic years repeated over the run (the work per year is
 unchanged; forcing memory stays at a few years).
 """
@@ -25,7 +25,7 @@ import math
 import numpy as np
 
 from . import _abi
-from . import World, WaterBalanceModel, regional_spec, PARAM_INDEX
+from . import World, WaterBalanceModel, global_spec, regional_spec, PARAM_INDEX
 
 
 class BasinSet:
@@ -84,7 +84,76 @@ class BasinSet:
         return np.repeat(np.asarray(values, dtype=np.float64), np.diff(self.offsets))
 
 
-def make_basins(n_basins, seed, min_cells=50, max_cells=5000):
+def upstream_cells(world):
+    """Number of cells draining through each cell, itself included."""
+    order = np.argsort(world.rout_order)  # upstream cells come first
+    count = np.ones(world.n, dtype=np.int64)
+    ds = world.downstream.astype(np.int64)
+    for i in order:
+        if ds[i] > 0:
+            count[ds[i] - 1] += count[i]
+    return count
+
+
+def catchment(world, station):
+    """0-based cells upstream of (and including) a station cell, ascending."""
+    inflow = world.inflow9
+    stack, out = [int(station)], []
+    while stack:
+        c = stack.pop()
+        out.append(c)
+        stack.extend(int(u) - 1 for u in inflow[c] if u > 0)
+    return np.sort(np.array(out, dtype=np.int32))
+
+
+def cut_basins(dem, n_basins, rng, min_cells, max_cells):
+    """Up to n_basins disjoint catchments of one grid with cell counts log-uniform
+    in [min_cells, max_cells]: for each drawn size (largest first) the station is
+    the free cell whose catchment count is nearest.  A cell is free when it lies in
+    no basin taken before and no such basin lies upstream of it (cells on the flow
+    path below a taken station are marked as they are reached).  Returns lists of
+    0-based cells."""
+    count = upstream_cells(dem)
+    ds = dem.downstream.astype(np.int64)
+    taken = np.zeros(dem.n, dtype=bool)
+    sizes = np.exp(rng.uniform(math.log(min_cells), math.log(max_cells), n_basins))
+    in_range = (count >= min_cells) & (count <= max_cells)
+    out = []
+    for size in sorted(sizes, reverse=True):
+        cand = np.flatnonzero(in_range & ~taken)
+        if len(cand) == 0:
+            break
+        st = int(cand[np.argmin(np.abs(np.log(count[cand] / size)))])
+        cells = catchment(dem, st)
+        taken[cells] = True
+        c = ds[st] - 1
+        while c >= 0 and not taken[c]:
+            taken[c] = True
+            c = ds[c] - 1
+        out.append(cells)
+    return out
+
+
+def make_basins(n_basins, seed, min_cells=50, max_cells=5000, dem_cells=200000):
+    """Synthetic single-outlet basins with log-uniform cell counts in
+    [min_cells, max_cells], cut as disjoint station catchments from a seeded global
+    0.5-degree DEM of dem_cells land cells (a further DEM, seed + 1, ..., when one
+    has too few).  Basins come in random order."""
+    rng = np.random.default_rng(seed)
+    worlds = []
+    dem_seed = int(seed) * 7 + 3
+    while len(worlds) < n_basins:
+        dem = World(global_spec(seed=dem_seed, n_land=dem_cells, n_outlets=max(40, dem_cells // 500),
+                                n_sinks=max(4, dem_cells // 5000)))
+        need = n_basins - len(worlds)
+        for cells in cut_basins(dem, need, rng, min_cells, max_cells):
+            worlds.append(dem.subset(cells))
+        dem_seed += 1
+    order = rng.permutation(len(worlds))
+    return BasinSet([worlds[i] for i in order])
+
+
+def make_lattice_basins(n_basins, seed, min_cells=50, max_cells=5000):
     """Synthetic single-outlet basins with log-uniform cell counts in
     [min_cells, max_cells], each on its own DEM seed and latitude band."""
     rng = np.random.default_rng(seed)
