@@ -516,6 +516,17 @@ int wg_mlp_get_params(wg_mlp *h, float *out);
 int wg_mlp_set_params(wg_mlp *h, const float *in);
 /* y[r] for rows x[r][16] (host arrays). */
 int wg_mlp_forward(wg_mlp *h, int32_t rows, const float *x, float *y);
+/* wg_mlp_forward without any allocation, for callers that cannot allocate (code
+ * under HIP graph capture, tight proposal loops).  No counterpart in the
+ * reference.  x [rows][16] and y [rows] are device pointers on the handle's
+ * device; the work is queued on `stream` (a hipStream_t of that device; NULL: the
+ * handle's own stream) and not waited for.  The activations live in one static
+ * scratch array per GPU of WG_MLP_NOALLOC_MAX_ROWS rows (6.5 MiB, held until the
+ * process ends): 1 <= rows <= WG_MLP_NOALLOC_MAX_ROWS, else WG_E_ARG.  Every call
+ * on a GPU shares that array, so calls that overlap on one GPU (other streams,
+ * other handles) race.  Results are wg_mlp_forward's bits. */
+#define WG_MLP_NOALLOC_MAX_ROWS 8192
+int wg_mlp_forward_noalloc(wg_mlp *h, int32_t rows, const float *x, float *y, void *stream);
 /* Mean-squared-error loss and its gradient (flat, like the parameters) over rows. */
 int wg_mlp_gradient(wg_mlp *h, int32_t rows, const float *x, const float *t, float *loss, float *grad);
 /* Adam training over shuffled mini-batches (batch a multiple of 32); mean batch

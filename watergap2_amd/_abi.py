@@ -276,6 +276,7 @@ def _declare(lib):
     lib.wg_mlp_get_params.argtypes = [C.c_void_p, P_f]
     lib.wg_mlp_set_params.argtypes = [C.c_void_p, P_f]
     lib.wg_mlp_forward.argtypes = [C.c_void_p, i32, P_f, P_f]
+    lib.wg_mlp_forward_noalloc.argtypes = [C.c_void_p, i32, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.wg_mlp_gradient.argtypes = [C.c_void_p, i32, P_f, P_f, P_f, P_f]
     lib.wg_mlp_train.argtypes = [C.c_void_p, i32, P_f, P_f, i32, i32, C.c_float, C.c_uint64, P_f]
     lib.wg_world_create.argtypes = [C.POINTER(WorldSpec), C.POINTER(C.POINTER(WorldStruct))]
@@ -305,5 +306,5 @@ ABI_SYMBOLS = [
     "wg_autotune", "wg_set_launch_config",
     "wg_gamma_search_init", "wg_gamma_start", "wg_find_new_gamma",
     "wg_mlp_create", "wg_mlp_destroy", "wg_mlp_last_error", "wg_mlp_n_params", "wg_mlp_get_params",
-    "wg_mlp_set_params", "wg_mlp_forward", "wg_mlp_gradient", "wg_mlp_train",
+    "wg_mlp_set_params", "wg_mlp_forward", "wg_mlp_forward_noalloc", "wg_mlp_gradient", "wg_mlp_train",
 ]

@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -51,11 +52,11 @@ __device__ inline float gelu_grad(float z) {
 
 struct Epilogue {
   int mode;           // 0: C; 1: hidden forward; 2: delta of the previous layer
-  float *out_f32;     // mode 0: C; 1: z; 2: dz
+  float *out_f32;     // mode 0: C; 1: z (may be null: forward only); 2: dz
   const float *bias;  // mode 1
   const float *zprev; // mode 2: pre-activation whose GELU derivative scales C
   bf16 *out_b;        // modes 1, 2: bf16 [M][N]
-  bf16 *out_bt;       // modes 1, 2: bf16 [N][M]
+  bf16 *out_bt;       // modes 1, 2: bf16 [N][M] (mode 1: may be null)
 };
 
 // One wavefront per 32 x 32 tile of C (2 x 2 MFMA blocks); M, N, K multiples of 32.
@@ -95,10 +96,10 @@ __global__ __launch_bounds__(64) void mlp_gemm(const bf16 *__restrict__ A, const
           ep.out_f32[o] = c;
         } else if (ep.mode == 1) {
           const float z = c + ep.bias[col];
-          ep.out_f32[o] = z;
+          if (ep.out_f32) ep.out_f32[o] = z;
           const bf16 h = bf16(gelu(z));
           ep.out_b[o] = h;
-          ep.out_bt[int64_t(col) * M + row] = h;
+          if (ep.out_bt) ep.out_bt[int64_t(col) * M + row] = h;
         } else {
           const float dz = c * gelu_grad(ep.zprev[o]);
           ep.out_f32[o] = dz;
@@ -110,27 +111,29 @@ __global__ __launch_bounds__(64) void mlp_gemm(const bf16 *__restrict__ A, const
 }
 
 // Batch rows from the resident sample set: X [R][32] bf16 (features padded with
-// zeros), its transpose [32][R], targets.  Rows beyond n_valid are zero.
+// zeros), its transpose [32][R], targets.  Rows beyond n_valid are zero.  A null
+// idx takes rows 0 .. n_valid - 1 in order; a null xt or tb is not written.
 __global__ void mlp_gather(const float *__restrict__ xs, const float *__restrict__ ts, const int32_t *__restrict__ idx,
                            int n_valid, int R, bf16 *__restrict__ xb, bf16 *__restrict__ xt, float *__restrict__ tb) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R) return;
   const bool ok = r < n_valid;
-  const int64_t s = ok ? idx[r] : 0;
+  const int64_t s = ok ? (idx ? idx[r] : r) : 0;
   for (int k = 0; k < kInPad; k++) {
     const float v = (ok && k < kIn) ? xs[s * kIn + k] : 0.f;
     const bf16 b = bf16(v);
     xb[int64_t(r) * kInPad + k] = b;
-    xt[int64_t(k) * R + r] = b;
+    if (xt) xt[int64_t(k) * R + r] = b;
   }
   if (tb) tb[r] = ok ? ts[s] : 0.f;
 }
 
-// Output layer forward: y = h3 . w4 + b4 (FP32, sequential over units).
+// Output layer forward: y = h3 . w4 + b4 (FP32, sequential over units), rows
+// 0 .. n_out - 1 of the batch.
 __global__ void mlp_out_fwd(const bf16 *__restrict__ h3, const bf16 *__restrict__ w4, const float *__restrict__ b4,
-                            int R, float *__restrict__ y) {
+                            int n_out, float *__restrict__ y) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= R) return;
+  if (r >= n_out) return;
   float s = 0.f;
   for (int k = 0; k < kHid; k++) s += float(h3[int64_t(r) * kHid + k]) * float(w4[k]);
   y[r] = s + b4[0];
@@ -229,11 +232,22 @@ __global__ void mlp_pack(const float *__restrict__ p, bf16 *__restrict__ w1, bf1
   if (i < kHid) w4[i] = bf16(p[kOffW4 + i]);
 }
 
+// Activations of wg_mlp_forward_noalloc: X [R][32] bf16 and the three hidden
+// layers [R][128] bf16 for up to WG_MLP_NOALLOC_MAX_ROWS rows (6.5 MiB), one copy
+// per GPU for the life of the process.  Host code uses the per-device address
+// that hipGetSymbolAddress returns, never this name.
+constexpr size_t kScratchXBytes = size_t(WG_MLP_NOALLOC_MAX_ROWS) * kInPad * sizeof(bf16);
+constexpr size_t kScratchHBytes = size_t(WG_MLP_NOALLOC_MAX_ROWS) * kHid * sizeof(bf16);
+constexpr size_t kScratchBytes = kScratchXBytes + 3 * kScratchHBytes;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 }  // namespace
 
 struct wg_mlp {
   std::string err;
   hipStream_t stream = nullptr;
+  int device = 0;
+  unsigned char *scratch = nullptr;  // this device's address of the scratch array
   float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr, *gpad = nullptr;
   bf16 *w1 = nullptr, *w1t = nullptr, *w2 = nullptr, *w2t = nullptr, *w3 = nullptr, *w3t = nullptr, *w4 = nullptr;
   int64_t adam_t = 0;
@@ -297,10 +311,32 @@ int ensure_batch(wg_mlp *h, int R) {
   return WG_OK;
 }
 
-int launch_gemm(wg_mlp *h, const bf16 *A, const bf16 *Bt, int M, int N, int K, const Epilogue &ep) {
+int launch_gemm_on(wg_mlp *h, hipStream_t stream, const bf16 *A, const bf16 *Bt, int M, int N, int K,
+                   const Epilogue &ep) {
   if (M % 32 || N % 32 || K % 32) return mlp_fail(h, WG_E_ARG, "GEMM shape is not a multiple of 32");
-  hipLaunchKernelGGL(mlp_gemm, dim3(N / 32, M / 32), dim3(64), 0, h->stream, A, Bt, M, N, K, ep);
+  hipLaunchKernelGGL(mlp_gemm, dim3(N / 32, M / 32), dim3(64), 0, stream, A, Bt, M, N, K, ep);
   MLP_CHECK(h, hipGetLastError());
+  return WG_OK;
+}
+
+int launch_gemm(wg_mlp *h, const bf16 *A, const bf16 *Bt, int M, int N, int K, const Epilogue &ep) {
+  return launch_gemm_on(h, h->stream, A, Bt, M, N, K, ep);
+}
+
+// Per-device address of the scratch array, resolved once per device with that
+// device current (each GPU holds its own copy at its own address).
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_addr[64] = {};
+
+int scratch_address(wg_mlp *h) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (h->device < 0 || h->device >= 64) return mlp_fail(h, WG_E_ARG, "device ordinal out of range");
+  if (!g_scratch_addr[h->device]) {
+    void *ptr = nullptr;
+    MLP_CHECK(h, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_addr[h->device] = static_cast<unsigned char *>(ptr);
+  }
+  h->scratch = g_scratch_addr[h->device];
   return WG_OK;
 }
 
@@ -376,8 +412,10 @@ extern "C" int wg_mlp_create(uint64_t seed, wg_mlp **out) {
   wg_mlp *h = new (std::nothrow) wg_mlp();
   if (!h) return WG_E_NOMEM;
   *out = h;
+  MLP_CHECK(h, hipGetDevice(&h->device));
   MLP_CHECK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-  int rc = mlp_alloc(h, &h->p, kNParams);
+  int rc = scratch_address(h);
+  if (!rc) rc = mlp_alloc(h, &h->p, kNParams);
   if (!rc) rc = mlp_alloc(h, &h->g, kNParams);
   if (!rc) rc = mlp_alloc(h, &h->m1, kNParams);
   if (!rc) rc = mlp_alloc(h, &h->m2, kNParams);
@@ -470,6 +508,35 @@ extern "C" int wg_mlp_forward(wg_mlp *h, int32_t rows, const float *x, float *y)
   (void)hipStreamSynchronize(h->stream);
   (void)hipFree(dx);
   return rc;
+}
+
+extern "C" int wg_mlp_forward_noalloc(wg_mlp *h, int32_t rows, const float *x, float *y, void *stream) {
+  if (!h || !x || !y) return WG_E_ARG;
+  if (rows <= 0 || rows > WG_MLP_NOALLOC_MAX_ROWS)
+    return mlp_fail(h, WG_E_ARG, "wg_mlp_forward_noalloc: rows must be 1.." + std::to_string(WG_MLP_NOALLOC_MAX_ROWS) +
+                                     " (the size of its scratch array), got " + std::to_string(rows));
+  if (!h->scratch) return mlp_fail(h, WG_E_STATE, "wg_mlp_forward_noalloc: scratch address not resolved");
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : h->stream;
+  const int R = round32(rows);
+  bf16 *xb = reinterpret_cast<bf16 *>(h->scratch);
+  bf16 *hb[3];
+  for (int l = 0; l < 3; l++) hb[l] = reinterpret_cast<bf16 *>(h->scratch + kScratchXBytes + l * kScratchHBytes);
+  // every row of the batch, padding included, is written before it is read:
+  // nothing in the scratch array needs clearing between calls
+  hipLaunchKernelGGL(mlp_gather, dim3((R + 255) / 256), dim3(256), 0, s, x, nullptr, nullptr, rows, R, xb, nullptr,
+                     nullptr);
+  MLP_CHECK(h, hipGetLastError());
+  const bf16 *w[3] = {h->w1, h->w2, h->w3};
+  const float *b[3] = {h->p + kOffB1, h->p + kOffB2, h->p + kOffB3};
+  const int K[3] = {kInPad, kHid, kHid};
+  for (int l = 0; l < 3; l++) {
+    Epilogue ep{1, nullptr, b[l], nullptr, hb[l], nullptr};
+    int rc = launch_gemm_on(h, s, l == 0 ? xb : hb[l - 1], w[l], R, kHid, K[l], ep);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(mlp_out_fwd, dim3((rows + 255) / 256), dim3(256), 0, s, hb[2], h->w4, h->p + kOffB4, rows, y);
+  MLP_CHECK(h, hipGetLastError());
+  return WG_OK;
 }
 
 extern "C" int wg_mlp_gradient(wg_mlp *h, int32_t rows, const float *x, const float *t, float *loss, float *grad) {
