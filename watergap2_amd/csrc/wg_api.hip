@@ -823,8 +823,13 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
             for (int32_t L = 1; L <= wide[gg]; L++) {
               const int32_t a = H->level_off[l0 + L], e = H->level_off[l0 + L + 1];
               int32_t j = 0;
-              for (int32_t st0 = a; st0 < e; st0 += 64, j++)
-                if (j % nw == w) items.push_back(WgWaveItem{st0, std::min(st0 + 64, e), L, 0});
+              for (int32_t st0 = a; st0 < e; st0 += 64, j++) {
+                if (j % nw != w) continue;
+                const int32_t end = std::min(st0 + 64, e);
+                int32_t lr = 0;
+                for (int32_t q = st0; q < end; q++) lr |= (H->clink[q].flags & (CFL_LAKE | CFL_RES)) ? 1 : 0;
+                items.push_back(WgWaveItem{st0, end, L, lr});
+              }
             }
           }
         }

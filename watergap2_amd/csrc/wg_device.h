@@ -30,9 +30,10 @@ struct alignas(16) WgChainLink {
 };
 
 // A batch of the wave walk's wide levels: cells [start, end) of one level (counted
-// from the group's first level), up to 64.
+// from the group's first level), up to 64; lakeres is 1 when one of them has a
+// global lake or a reservoir (the batch then needs the routing record's suffix).
 struct alignas(16) WgWaveItem {
-  int32_t start, end, level, pad;
+  int32_t start, end, level, lakeres;
 };
 
 // Outflow constants of the storages on the routing chain (global lake / wetland,

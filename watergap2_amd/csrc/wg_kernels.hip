@@ -1316,6 +1316,53 @@ __device__ inline void prefetch_slot_wave(const WgStatic &S, const WgState &X, i
   q.up[3] = b.w;
 }
 
+// Wide-level variant: a batch of one level takes the record suffix when one of
+// its cells has a global lake or reservoir (`suffix`, uniform over the wavefront,
+// from the batch table), and issues no suffix loads otherwise.  Lake and reservoir
+// cells come last in their level, so most batches load the prefix alone.  The
+// workgroup's wavefronts share one CU's vector memory pipeline, which bounds the
+// wide levels: every load instruction left out shortens them.
+__device__ inline void prefetch_slot_batch(const WgStatic &S, const WgState &X, int64_t base, int p, bool suffix,
+                                           ChainSlot &q) {
+  const int64_t n = S.n;
+  const WgChainStatic cs = load_cstat(S, p);
+  const WgD2 *src = X.crec + base * (WG_DEV_NREC / 2) + p;
+  constexpr int kPrefixPairs = (CF_PREFIX + 1) / 2;
+#pragma unroll
+  for (int k = 0; k < kPrefixPairs; k++) {
+    const WgD2 v = src[k * n];
+    q.a[2 * k] = v.x;
+    q.a[2 * k + 1] = v.y;
+  }
+  if (suffix) {
+#pragma unroll
+    for (int k = kPrefixPairs; k < CF_COUNT / 2; k++) {
+      const WgD2 v = src[k * n];
+      q.a[2 * k] = v.x;
+      q.a[2 * k + 1] = v.y;
+    }
+  } else {
+#pragma unroll
+    for (int k = 2 * kPrefixPairs; k < CF_COUNT; k++) q.a[k] = 0.;
+  }
+  q.inv_ksw = cs.inv_ksw;
+  q.ome_sw = cs.ome_sw;
+  q.inv_kgw = cs.inv_kgw;
+  q.ome_gw = cs.ome_gw;
+  q.area = cs.area;
+  q.cf100 = cs.cf100;
+  const WgI4 a = S.clink[p], b = S.clink[n + p];
+  q.flags = a.x;
+  q.station = a.y;
+  q.nu = a.z & 0xffff;
+  q.nu_level = a.z >> 16;
+  q.level = a.w;
+  q.up[0] = b.x;
+  q.up[1] = b.y;
+  q.up[2] = b.z;
+  q.up[3] = b.w;
+}
+
 // The routing record of a slot (the suffix of a cell without a global lake or
 // reservoir holds no data; chain_step does not read it for such cells).
 __device__ inline void slot_record(const ChainSlot &q, ChainRec &c) {
@@ -1498,14 +1545,22 @@ __device__ inline void wave_lds_order() {
 
 // One batch: cells [b0, bend) (b0 >= bend: a padding batch); `q` holds the
 // batch's inputs, `fill` receives those of the batch starting at pnext.
+// kWide: a batch of the wide levels, where next_lr / cur_lr (uniform) are the
+// batches' lakeres flags: the next batch's suffix is loaded, and this batch's
+// lake and reservoir stores issued, only when it is 1.  Otherwise a batch of the
+// narrow rest: per-cell record suffix through s_fl, every store issued.
+template <bool kWide>
 __device__ __forceinline__ void wave_batch(const WgStatic &S, const WgState &X, double *s_tv, int b0, int bend,
                                            int pnext, int ge, int gs, int64_t base, int m, const DayInfo &d,
-                                           int daily_slot, const uint8_t *s_fl, const ChainSlot &q,
-                                           ChainSlot &fill) {
+                                           int daily_slot, const uint8_t *s_fl, int next_lr, int cur_lr,
+                                           const ChainSlot &q, ChainSlot &fill) {
   const int lane = threadIdx.x & 63;
   {
     const int pf = pnext + lane < ge ? pnext + lane : gs;
-    prefetch_slot_wave(S, X, base, pf, s_fl[pf - gs], fill);
+    if (kWide)
+      prefetch_slot_batch(S, X, base, pf, next_lr != 0, fill);
+    else
+      prefetch_slot_wave(S, X, base, pf, s_fl[pf - gs], fill);
   }
   const int p = b0 + lane;
   const bool have = p < bend;
@@ -1544,9 +1599,11 @@ __device__ __forceinline__ void wave_batch(const WgStatic &S, const WgState &X, 
     }
   }
   const int n = S.n;
-  store_or_drop(X.glo_lake + base, n, run && (q.flags & CFL_LAKE), p, o.glo_lake);
-  store_or_drop(X.res + base, n, run && (q.flags & CFL_RES), p, o.res);
-  store_or_drop(X.k_release + base, n, run && (q.flags & CFL_RES), p, o.k_release);
+  if (cur_lr != 0) {
+    store_or_drop(X.glo_lake + base, n, run && (q.flags & CFL_LAKE), p, o.glo_lake);
+    store_or_drop(X.res + base, n, run && (q.flags & CFL_RES), p, o.res);
+    store_or_drop(X.k_release + base, n, run && (q.flags & CFL_RES), p, o.k_release);
+  }
   store_or_drop(X.glo_wetl + base, n, run && (q.flags & CFL_GWET), p, o.glo_wetl);
   store_or_drop(X.gw + base, n, run && (q.flags & CFL_ARID), p, o.gw);
   store_or_drop(X.river + base, n, run, p, o.river);
@@ -1585,53 +1642,45 @@ __global__ __launch_bounds__(64 * kWaves) void wg_chain_wave_kernel(WgStatic S, 
   // level 0 was routed by the cell kernel: its transported volumes, four loads
   // in flight per thread
   constexpr int kThreads = 64 * kWaves;
-  for (int i0 = gs + tid; i0 < lvl0_end; i0 += 4 * kThreads) {
-    double v[4];
+  // the chain flags of the group's cells, behind its transported volumes (the
+  // launch gives every workgroup 9 bytes of LDS per cell of the largest group);
+  // both are loaded together, 16 of each in flight per thread
+  uint8_t *s_fl = reinterpret_cast<uint8_t *>(s_tv + (ge - gs));
+  constexpr int kFill = 16;
+  for (int i0 = gs + tid; i0 < ge; i0 += kFill * kThreads) {
+    double v[kFill];
+    int8_t f[kFill];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < kFill; j++) {
       const int i = i0 + j * kThreads;
       v[j] = (i < lvl0_end) ? X.tv[base + i] : 0.;
+      f[j] = (i < ge) ? S.cflags[i] : int8_t(0);
     }
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < kFill; j++) {
       const int i = i0 + j * kThreads;
       if (i < lvl0_end) s_tv[i - gs] = v[j];
+      if (i < ge) s_fl[i - gs] = uint8_t(f[j]);
     }
   }
-  // the chain flags of the group's cells, behind its transported volumes (the
-  // launch gives every workgroup 9 bytes of LDS per cell of the largest group)
-  uint8_t *s_fl = reinterpret_cast<uint8_t *>(s_tv + (ge - gs));
-  for (int i0 = gs + tid; i0 < ge; i0 += 4 * kThreads) {
-    int8_t v[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int i = i0 + j * kThreads;
-      v[j] = (i < ge) ? S.cflags[i] : int8_t(0);
-    }
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int i = i0 + j * kThreads;
-      if (i < ge) s_fl[i - gs] = uint8_t(v[j]);
-    }
+  ChainSlot s0, s1;
+  // wide levels: this wavefront's batches in level order; a padding batch (past
+  // the list) routes nothing and passes no barrier.  Every wavefront passes
+  // exactly la barriers, one after each wide level.  The first batch's loads go
+  // out with the fill's.
+  const int it0 = la > 0 ? wave_item_off[g * kWaves + w] : 0;
+  const int it1 = la > 0 ? wave_item_off[g * kWaves + w + 1] : 0;
+  auto item = [&](int i) {
+    return (i < it1) ? wave_items[i] : WgWaveItem{ge, ge, 0, 0};
+  };
+  WgWaveItem ia = item(it0);
+  if (la > 0) {
+    const int pf = ia.start + lane < ge ? ia.start + lane : gs;
+    prefetch_slot_batch(S, X, base, pf, ia.lakeres != 0, s0);
   }
   __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   lds_barrier();
-  ChainSlot s0, s1;
   if (la > 0) {
-    // wide levels: this wavefront's batches in level order; a padding batch
-    // (past the list) routes nothing and passes no barrier.  Every wavefront
-    // passes exactly la barriers, one after each wide level.
-    const int it0 = wave_item_off[g * kWaves + w], it1 = wave_item_off[g * kWaves + w + 1];
-    auto item = [&](int i) {
-      return (i < it1) ? wave_items[i] : WgWaveItem{ge, ge, 0, 0};
-    };
-    WgWaveItem ia = item(it0);
-    {
-      const int pf = ia.start + lane < ge ? ia.start + lane : gs;
-      prefetch_slot_wave(S, X, base, pf, s_fl[pf - gs], s0);
-    }
-    // the first slot's loads complete here on every path into the loop
-    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     int lev = 1;
     for (int i = it0; i < it1; i += 2) {
       const WgWaveItem ib = item(i + 1), ic = item(i + 2);
@@ -1639,12 +1688,14 @@ __global__ __launch_bounds__(64 * kWaves) void wg_chain_wave_kernel(WgStatic S, 
         lds_barrier();
         lev++;
       }
-      wave_batch(S, X, s_tv, ia.start, ia.end, ib.start, ge, gs, base, m, d, daily_slot, s_fl, s0, s1);
+      wave_batch<true>(S, X, s_tv, ia.start, ia.end, ib.start, ge, gs, base, m, d, daily_slot, s_fl, ib.lakeres,
+                 ia.lakeres, s0, s1);
       while (lev < ib.level) {
         lds_barrier();
         lev++;
       }
-      wave_batch(S, X, s_tv, ib.start, ib.end, ic.start, ge, gs, base, m, d, daily_slot, s_fl, s1, s0);
+      wave_batch<true>(S, X, s_tv, ib.start, ib.end, ic.start, ge, gs, base, m, d, daily_slot, s_fl, ic.lakeres,
+                 ib.lakeres, s1, s0);
       ia = ic;
     }
     while (lev <= la) {
@@ -1662,8 +1713,8 @@ __global__ __launch_bounds__(64 * kWaves) void wg_chain_wave_kernel(WgStatic S, 
   __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   for (int b = 0; b < nb; b += 2) {
     const int b0 = pb + b * 64;
-    wave_batch(S, X, s_tv, b0, ge, b0 + 64, ge, gs, base, m, d, daily_slot, s_fl, s0, s1);
-    wave_batch(S, X, s_tv, b0 + 64, ge, b0 + 128, ge, gs, base, m, d, daily_slot, s_fl, s1, s0);
+    wave_batch<false>(S, X, s_tv, b0, ge, b0 + 64, ge, gs, base, m, d, daily_slot, s_fl, 1, 1, s0, s1);
+    wave_batch<false>(S, X, s_tv, b0 + 64, ge, b0 + 128, ge, gs, base, m, d, daily_slot, s_fl, 1, 1, s1, s0);
   }
   (void)T;
 }
