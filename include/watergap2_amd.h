@@ -501,6 +501,35 @@ float wg_gamma_start(const wg_gamma_search *s, float gamma_from_file);
 float wg_find_new_gamma(wg_gamma_search *s, float gamma_old, int32_t n_years, const float *measured,
                         const float *simulated, const float *sim_inflow, const float *sim_water_use);
 
+/* Station correction factor after a search (calibGammaClass::writeCorrFactors,
+ * calibration.cpp:656-728): with the cell correction factor computed
+ * (cell_corr_factor_ind 99) the measured sum over the years with data is scaled by
+ * 1.1 (gamma at its upper limit) or 0.9 (lower limit) and CFS = measured / simulated;
+ * within 1 % of 1 it is 1.  Sets calib_status 4 when CFS differs from 1. */
+float wg_station_corr_factor(wg_gamma_search *s, float gamma, int32_t n_years, const float *measured,
+                             const float *simulated);
+
+/* Calibration basins from station cells (1-based) in the order of the station file
+ * (cbasinClass::prepare with upstreamBasin, calib_basins.cpp:25-300, 302-378): each
+ * station claims the cells upstream of it that hold its own cell's value (no basin
+ * yet, or the earlier basin it lies in, which it then splits).  Station cells <= 1
+ * (outside the land mask) and repeats are skipped (used[k] = 0).  calib_basin[N]
+ * receives the basin number (1-based, in the order of the used stations), 0 outside
+ * every basin, and -1 for the cell below a station that lies in no basin
+ * (calib_basins.cpp:270-285).  overlap[k]: 0 none, 1 the station lies in an earlier
+ * basin, 2 earlier basins lie upstream, 3 both (-1 when skipped).  used and overlap
+ * may be NULL.  Returns the number of basins, or a negative WG_E_* code. */
+int wg_calib_basins(int32_t n_cells, const int32_t *inflow9, const int32_t *outflow, int32_t n_stations,
+                    const int32_t *station_cells, int32_t *used, int16_t *calib_basin, int32_t *overlap);
+/* The stations upstream of each basin's station (upstreamStationClass::findStations,
+ * upstream_stations.cpp:66-140): all_stations 0 stops at the first station on each
+ * path (directUpstSt), 1 goes through (allUpstSt).  basin_cells are the basins'
+ * station cells (1-based) in basin order.  Basin b's list is stations[offsets[b] ..
+ * offsets[b+1]) (1-based basin numbers, in the reference's list order); WG_E_ARG when
+ * more than max_entries entries are found. */
+int wg_upstream_stations(int32_t n_cells, const int32_t *inflow9, int32_t n_basins, const int32_t *basin_cells,
+                         int32_t all_stations, int32_t *offsets, int32_t *stations, int32_t max_entries);
+
 /* ---- calibration surrogate: 16-128-128-128-1 GELU perceptron on BF16 MFMA ----
  * No counterpart in the reference: it proposes gamma from basin features so that
  * the plain model only has to verify (the search it replaces is

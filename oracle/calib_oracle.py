@@ -152,3 +152,108 @@ def find_new_gamma(st, gamma_old, measured, simulated, inflow=None, water_use=No
     if gamma < 0 and abs(float(f32(st.cell_corr_factor - f32(1)))) > 0.01:
         st.calib_status = 3
     return gamma
+
+
+def station_corr_factor(st, gamma, measured, simulated):
+    """calibration.cpp:656-728 (writeCorrFactors): the station correction factor
+    CFS in float; sets calib_status 4 when it differs from 1."""
+    meas_sum = f32(0)
+    sim_sum = f32(0)
+    for m, s in zip(np.asarray(measured, np.float32), np.asarray(simulated, np.float32)):
+        if m > -1:
+            meas_sum = f32(meas_sum + m)
+            sim_sum = f32(sim_sum + s)
+    gamma = f32(gamma)
+    if gamma == st.upper and st.cell_corr_factor_ind == 99:   # calibration.cpp:684-687
+        meas_sum = f32(float(meas_sum) * 1.1)
+    if gamma == st.lower and st.cell_corr_factor_ind == 99:   # calibration.cpp:688-691
+        meas_sum = f32(float(meas_sum) * 0.9)
+    cfs = f32(meas_sum / sim_sum) if st.cell_corr_factor_ind == 99 else f32(1.0)   # calibration.cpp:694-700
+    if (1.0 < cfs < 1.01) or (0.99 < cfs < 1.0):              # calibration.cpp:704-709
+        cfs = f32(1.0)
+    if cfs > 1.0 or cfs < 1.0:                                 # calibration.cpp:710-711
+        st.calib_status = 4
+    return cfs
+
+
+def _upstream_basin(cell, basin_no, inflow9, grid):
+    """calib_basins.cpp:302-378 (upstreamBasin), sweep by sweep over all cells."""
+    ng = len(grid)
+    upstream = downstream = False
+    base = 0
+    if grid[cell - 1] != 0:
+        upstream = True
+        base = grid[cell - 1]
+    grid[cell - 1] = -1
+    while True:
+        not_ready = False
+        for n in range(ng):
+            if grid[n] == -1:
+                for j in range(9):
+                    grid[n] = -2
+                    u = inflow9[n][j]
+                    if u != 0:
+                        if grid[u - 1] == base:
+                            not_ready = True
+                            grid[u - 1] = -1
+                        else:
+                            downstream = True
+        if not not_ready:
+            break
+    for n in range(ng):
+        if grid[n] == -2:
+            grid[n] = basin_no
+    return (1 if upstream else 0) + (2 if downstream else 0)
+
+
+def calib_basins(inflow9, outflow, station_cells):
+    """calib_basins.cpp:25-300 (cbasinClass::prepare) on the station cells in file
+    order; returns (grid, used, overlap)."""
+    ng = len(outflow)
+    grid = [0] * ng
+    used, overlap, accepted = [], [], []
+    for cn in station_cells:
+        if cn <= 1 or cn in accepted:          # calib_basins.cpp:117-131
+            used.append(0)
+            overlap.append(-1)
+            continue
+        accepted.append(cn)
+        overlap.append(_upstream_basin(cn, len(accepted), inflow9, grid))
+        used.append(1)
+    for cn in accepted:                        # calib_basins.cpp:270-285
+        o = outflow[cn - 1]
+        if o != 0 and grid[o - 1] == 0:
+            grid[o - 1] = -1
+    return np.array(grid, np.int16), used, overlap
+
+
+def upstream_stations(inflow9, basin_cells, all_stations):
+    """upstream_stations.cpp:66-140 (findStations), sweep by sweep; returns one
+    list per basin in the reference's order (new entries at the front)."""
+    ng = len(inflow9)
+    done = [0] * ng
+    lists = []
+    for cb, cell in enumerate(basin_cells):
+        lst = []
+        done[cell - 1] = 1
+        while True:
+            for n in range(ng):
+                if done[n] == 1:
+                    done[n] = 2
+                    for i in range(9):
+                        u = inflow9[n][i]
+                        if u != 0:
+                            st = 0
+                            for cb2, c2 in enumerate(basin_cells):
+                                if u == c2:
+                                    st = cb2 + 1
+                                    break
+                            if st:
+                                done[u - 1] = all_stations
+                                lst.insert(0, st)
+                            else:
+                                done[u - 1] = 1
+            if not any(d == 1 for d in done):
+                break
+        lists.append(lst)
+    return lists

@@ -79,3 +79,41 @@ def test_plain_search_matches_oracle_driven_search(wg, oracle):
             traj.append(float(g))
         assert traj == res["trajectories"][b], (b, traj, res["trajectories"][b])
         assert np.float32(res["gamma"][b]) == g
+
+
+def test_nested_stations_search(wg, oracle):
+    """Four nested stations on the 2,000-cell regional grid: the station runner's
+    discharges equal the oracle's with the same per-basin gamma, and the tiered
+    search (upstream stations first, simulated upstream inflow passed to
+    findNewGamma) brings every station within 1 % of its hidden-gamma target."""
+    from watergap2_amd.calibration import StationRunner, nested_search, upstream_cells, within_1pc
+    w = wg.World(wg.regional_spec(seed=0))
+    count = upstream_cells(w)
+    outlet = int(np.argmax(count))
+    # three interior stations with catchments of 150-700 cells, nested where the network allows
+    cand = np.flatnonzero((count > 150) & (count < 700))
+    rng = np.random.default_rng(8)
+    picks = [int(c) for c in rng.choice(cand, 3, replace=False)]
+    stations = [p + 1 for p in picks] + [outlet + 1]
+    print(f"case: nested stations {stations} on the regional grid, 3 years", flush=True)
+    years = 3
+    r = StationRunner(w, stations, forcing_years=3)
+    nb = r.n_basins
+    assert nb == 4 and (r.calib_basin > 0).sum() == w.n
+    hidden = rng.uniform(0.3, 4.0, nb)
+    q = r.run(hidden[None, :], n_years=years)[0]
+    # the same per-cell parameters in the oracle
+    o = oracle.Oracle(w)
+    o.set_params(gamma=r.per_cell(hidden, r.base_gamma))
+    o.set_stations(r.cells - 1)
+    for y in range(years):
+        o.run(r.forcing[365 * y:365 * (y + 1)], 1)
+        assert np.array_equal(q[:, y], o.station_discharge()), y
+        o.reset_station_discharge()
+    measured = q.astype(np.float32)
+    res = nested_search(r, measured, n_years=years)
+    final = r.run(res["gamma"].astype(np.float64)[None, :], n_years=years)[0]
+    r.close()
+    assert (res["status"] >= 1).all()
+    assert all(within_1pc(final[b], measured[b]) for b in range(nb)), res
+    assert res["tier"].max() >= 1  # the outlet waits for the stations above it

@@ -277,3 +277,189 @@ def within_1pc(sim, measured):
     ny = int(ok.sum())
     avg = np.float32(ms / np.float32(ny))
     return abs(float(np.float32(d / np.float32(ny * avg)))) < 0.01
+
+
+def calib_basins(inflow9, outflow, station_cells):
+    """Calibration basins of station cells (1-based, station-file order) on a
+    routing network (cbasinClass::prepare, calib_basins.cpp:25-300): returns
+    (calib_basin [N] int16, used [K], overlap [K], n_basins).  See
+    wg_calib_basins in include/watergap2_amd.h."""
+    lib = _abi.load_library()
+    inflow9 = np.ascontiguousarray(inflow9, dtype=np.int32)
+    outflow = np.ascontiguousarray(outflow, dtype=np.int32)
+    cells = np.ascontiguousarray(station_cells, dtype=np.int32)
+    n = len(outflow)
+    grid = np.zeros(n, np.int16)
+    used = np.zeros(len(cells), np.int32)
+    overlap = np.zeros(len(cells), np.int32)
+    rc = lib.wg_calib_basins(n, _abi.ptr(inflow9, _abi.P_i32), _abi.ptr(outflow, _abi.P_i32), len(cells),
+                             _abi.ptr(cells, _abi.P_i32), _abi.ptr(used, _abi.P_i32), _abi.ptr(grid, _abi.P_i16),
+                             _abi.ptr(overlap, _abi.P_i32))
+    if rc < 0:
+        raise RuntimeError(f"wg_calib_basins failed: {_abi.ERRORS.get(rc, rc)}")
+    return grid, used, overlap, rc
+
+
+def upstream_stations(inflow9, basin_cells, all_stations=False):
+    """Per basin, the basins whose stations lie upstream of its station
+    (upstream_stations.cpp:66-140): directly upstream, or all with all_stations.
+    Returns a list of lists of 1-based basin numbers in the reference's order."""
+    lib = _abi.load_library()
+    inflow9 = np.ascontiguousarray(inflow9, dtype=np.int32)
+    cells = np.ascontiguousarray(basin_cells, dtype=np.int32)
+    nb = len(cells)
+    cap = max(1, nb * nb)
+    offsets = np.zeros(nb + 1, np.int32)
+    out = np.zeros(cap, np.int32)
+    rc = lib.wg_upstream_stations(len(inflow9), _abi.ptr(inflow9, _abi.P_i32), nb, _abi.ptr(cells, _abi.P_i32),
+                                  1 if all_stations else 0, _abi.ptr(offsets, _abi.P_i32), _abi.ptr(out, _abi.P_i32),
+                                  cap)
+    if rc != 0:
+        raise RuntimeError(f"wg_upstream_stations failed: {_abi.ERRORS.get(rc, rc)}")
+    return [out[offsets[b]:offsets[b + 1]].tolist() for b in range(nb)]
+
+
+def station_corr_factor(search, gamma, measured, simulated):
+    """The station correction factor CFS after a search (writeCorrFactors,
+    calibration.cpp:656-728); `search` is the basin's _abi.GammaSearch."""
+    lib = _abi.load_library()
+    meas = np.ascontiguousarray(measured, dtype=np.float32)
+    sim = np.ascontiguousarray(simulated, dtype=np.float32)
+    return float(lib.wg_station_corr_factor(C.byref(search), float(gamma), len(meas), _abi.ptr(meas, _abi.P_f),
+                                            _abi.ptr(sim, _abi.P_f)))
+
+
+class StationRunner:
+    """One model handle over a routing network with calibration stations, which may
+    be nested: each station's calibration basin is the part of its catchment below
+    the stations upstream of it (calib_basins.cpp:25-300).  gamma and CFA are set per
+    calibration basin; cells in no basin keep the grid's own values.  run() returns
+    the annual discharge at every station."""
+
+    def __init__(self, world, station_cells, n_members=1, forcing_years=3, seed=11, forcing=None):
+        self.world = world
+        grid, used, _overlap, nb = calib_basins(world.inflow9, world.downstream, station_cells)
+        self.calib_basin = grid
+        self.cells = np.array([c for c, u in zip(station_cells, used) if u], dtype=np.int32)  # 1-based
+        self.n_basins = nb
+        self.direct_upstream = upstream_stations(world.inflow9, self.cells, all_stations=False)
+        self.n_members = n_members
+        self.forcing_years = forcing_years
+        self.model = WaterBalanceModel(world, n_members=n_members)
+        self.model.set_stations(self.cells - 1)
+        if forcing is None:
+            forcing = world.forcing(seed=seed, day0=0, n_days=365 * forcing_years)
+        self.forcing = forcing
+        self.model.load_forcing(forcing)
+        self.initial = self.model.get_state(0)
+        self.base_gamma = world.params[PARAM_INDEX["P_GAMRUN_C"]].copy()
+        self.base_cfa = world.params[PARAM_INDEX["P_CFA"]].copy()
+
+    def per_cell(self, values, base):
+        """Per-basin values (1-based basins) on the cells of each calibration basin."""
+        out = base.copy()
+        inside = self.calib_basin > 0
+        out[inside] = np.asarray(values, dtype=np.float64)[self.calib_basin[inside] - 1]
+        return out
+
+    def run(self, gamma, cfa=None, n_years=30):
+        """gamma, cfa: [n_members][n_basins].  Returns annual station discharge
+        [n_members][n_basins][n_years] (km3/yr, float64: each the sum of the day's
+        values in day order, routing.cpp:7232-7239)."""
+        M, nb = self.n_members, self.n_basins
+        gamma = np.asarray(gamma, dtype=np.float64).reshape(M, nb)
+        g = np.stack([self.per_cell(gamma[m], self.base_gamma) for m in range(M)])
+        if cfa is None:
+            c = np.tile(self.base_cfa, (M, 1))
+        else:
+            cfa = np.asarray(cfa, dtype=np.float64).reshape(M, nb)
+            c = np.stack([self.per_cell(cfa[m], self.base_cfa) for m in range(M)])
+        self.model.set_member_params(gamma=g, cfa=c)
+        self.model.set_state(self.initial, member=-1)
+        self.model.reset_station_discharge()
+        out = np.zeros((M, nb, n_years))
+        for y in range(n_years):
+            self.model.step_days((y % self.forcing_years) * 365, 1, 365)
+            tot, _ = self.model.station_discharge()
+            out[:, :, y] = tot
+            self.model.reset_station_discharge()
+        return out
+
+    def upstream_inflow(self, annual):
+        """Annual inflow from each basin's direct upstream stations
+        (getAnnualUpstreamInflow, routing.cpp:7241-7258): the stations' annual
+        discharges added in list order, as float (calibGammaClass::setUpstInflow)."""
+        nb, ny = annual.shape
+        out = np.zeros((nb, ny), dtype=np.float32)
+        for b in range(nb):
+            for y in range(ny):
+                total = 0.0
+                for u in self.direct_upstream[b]:
+                    total += annual[u - 1, y]
+                out[b, y] = np.float32(total)
+        return out
+
+    def close(self):
+        self.model.close()
+
+
+def nested_search(runner, measured, n_years=30, max_iter=40):
+    """The reference's calibration of every station of a network, upstream first:
+    a station's search starts once the stations directly upstream of it are done,
+    and it compares its station's discharge with the observations given the
+    simulated inflow from those stations (calibration.cpp:266-636 with
+    integrateWGHM.cpp:967-974).  Stations of one tier search side by side, one
+    model run of the whole network per step; a finished station keeps its last
+    gamma.  After a station's search the test run's discharge gives its station
+    correction factor (calibration.cpp:656-728).  Returns the final gamma, status,
+    CFA and CFS per station, the runs each needed, the tiers and the number of
+    model runs."""
+    assert runner.n_members == 1
+    lib = _abi.load_library()
+    nb = runner.n_basins
+    measured = np.asarray(measured, dtype=np.float32).reshape(nb, n_years)
+    states = [_abi.GammaSearch() for _ in range(nb)]
+    gamma = np.ones(nb, dtype=np.float32)
+    for i, s in enumerate(states):
+        lib.wg_gamma_search_init(C.byref(s))
+        gamma[i] = lib.wg_gamma_start(C.byref(s), -1.0)
+    # tier of a station: 0 without upstream stations, else 1 + the deepest tier above
+    tier = np.full(nb, -1)
+    while (tier < 0).any():
+        for b in range(nb):
+            ups = runner.direct_upstream[b]
+            if tier[b] < 0 and all(tier[u - 1] >= 0 for u in ups):
+                tier[b] = 1 + max((tier[u - 1] for u in ups), default=-1)
+    status = np.zeros(nb, dtype=np.int32)
+    runs = np.zeros(nb, dtype=np.int32)
+    cfs = np.ones(nb, dtype=np.float32)
+    n_runs = 0
+    for t in range(int(tier.max()) + 1):
+        active = tier == t
+        for _ in range(max_iter):
+            if not active.any():
+                break
+            q = runner.run(gamma.astype(np.float64)[None, :], n_years=n_years)[0]
+            n_runs += 1
+            inflow = runner.upstream_inflow(q)
+            for i in np.flatnonzero(active):
+                runs[i] += 1
+                sim = np.ascontiguousarray(q[i].astype(np.float32))
+                meas = np.ascontiguousarray(measured[i])
+                infl = np.ascontiguousarray(inflow[i])
+                g_new = lib.wg_find_new_gamma(C.byref(states[i]), float(gamma[i]), n_years, _abi.ptr(meas, _abi.P_f),
+                                              _abi.ptr(sim, _abi.P_f), _abi.ptr(infl, _abi.P_f), None)
+                if g_new < 0:
+                    active[i] = False
+                else:
+                    gamma[i] = np.float32(g_new)
+        # the test runs with the final gammas give the station correction factors
+        q = runner.run(gamma.astype(np.float64)[None, :], n_years=n_years)[0]
+        n_runs += 1
+        for i in np.flatnonzero(tier == t):
+            sim = np.ascontiguousarray(q[i].astype(np.float32))
+            cfs[i] = station_corr_factor(states[i], gamma[i], measured[i], sim)
+    status = np.array([s.calib_status for s in states], dtype=np.int32)
+    cfa = np.array([s.cell_corr_factor for s in states], dtype=np.float32)
+    return {"gamma": gamma, "status": status, "cfa": cfa, "cfs": cfs, "runs": runs, "tier": tier,
+            "model_runs": n_runs}
