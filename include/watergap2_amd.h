@@ -403,6 +403,27 @@ int wg_set_forcing_perturbation(wg_handle *h, const uint64_t *member_seed, doubl
  * is negative. */
 int wg_load_forcing(wg_handle *h, const float *forcing, int32_t n_days);
 
+/* routingClass::annualInit for a year after the first (routing.cpp:979-1495), at
+ * a year boundary between step calls.  st holds the statics of wg_create with this
+ * year's reservoir grids as Parts 1-2 of annualInit leave them (routing.cpp:1015-1100:
+ * G_RES of the year to use, and the area and capacity of the reservoirs in operation;
+ * wg_reservoir_year computes the latter); every other field must be the one given to
+ * wg_create.  With apply_changes (the reference's condition start month == 1 or a
+ * year after the first, routing.cpp:1313), a cell whose reservoir fraction grew
+ * against the previous call (or wg_create) loses that fraction of land, and the
+ * canopy, soil and snow water of the lost land moves to the reservoir storage of
+ * outflow_cell[i] (1-based, G_OUTFLOW_CELL_ASSIGNMENT.UNF4), every member
+ * (routing.cpp:1300-1421).  WG_E_INPUT, with nothing changed, when such a cell's land
+ * fraction does not fall against the day before (routing.cpp:1354-1357, a branch this
+ * state does not represent) or a reservoir has an unknown type. */
+int wg_annual_init(wg_handle *h, const wg_static_fields *st, const int32_t *outflow_cell, int32_t apply_changes);
+/* Parts 1-2 of annualInit (routing.cpp:1015-1100) for the year to use (resYearCurrentToUse):
+ * regulated lakes (reg_lake_status 1) and reservoirs with start_year <= year_to_use
+ * operate with their full area and capacity, the rest have none. */
+int wg_reservoir_year(int32_t n_cells, int32_t year_to_use, const double *reservoir_area_full,
+                      const double *stor_cap_full, const int32_t *start_year, const int8_t *reg_lake_status,
+                      double *reservoir_area, double *stor_cap);
+
 /* Station (outlet) cells whose daily discharge is accumulated (0-based cells). */
 int wg_set_stations(wg_handle *h, int32_t n_stations, const int32_t *cells);
 

@@ -180,6 +180,28 @@ int wgo_rout_order(int32_t n, const int32_t *inflow9, const int32_t *outflow, in
 
 #define ALLOC_D(p) (p) = (double *)calloc((size_t)n, sizeof(double))
 
+/* routing.cpp:360-403: mean demand of the cell and of up to reservoir_dsc = 5
+ * downstream cells (m3/yr -> m3/s), from the reservoirs in operation. */
+static void mean_demand(wgo_model *m, const wg_static_fields *st) {
+  const int32_t n = m->n;
+  for (int32_t i = 0; i < n; i++) {
+    m->mean_demand[i] = 0;
+    if (st->mean_nus) {
+      short k = 0;
+      int32_t downstreamCell = st->downstream[i];
+      m->mean_demand[i] += st->mean_nus[i];
+      while (k < 5 && downstreamCell > 0 &&
+             (downstreamCell < n ? st->reservoir_area[downstreamCell] : 0.) <= 0) {
+        const double coeff = st->alloc_coeff ? st->alloc_coeff[(size_t)i * 5 + k] : 1.0;
+        k++;
+        m->mean_demand[i] += st->mean_nus[downstreamCell - 1] * coeff;
+        downstreamCell = st->downstream[downstreamCell - 1];
+      }
+    }
+    m->mean_demand[i] = m->mean_demand[i] / 31536000.;
+  }
+}
+
 wgo_model *wgo_create(int32_t n, const wg_static_fields *st) {
   wgo_model *m = (wgo_model *)calloc(1, sizeof(wgo_model));
   if (m) m->threads = 1;
@@ -280,26 +302,58 @@ wgo_model *wgo_create(int32_t n, const wg_static_fields *st) {
    * i.e. the cell after the downstream cell; an index past the last cell is taken
    * as a cell without a reservoir.  Allocation coefficients default to 1
    * (rout_prepare.cpp:906-909). */
-  for (int32_t i = 0; i < n; i++) {
-    m->mean_demand[i] = 0;
-    if (st->mean_nus) {
-      short k = 0;
-      int32_t downstreamCell = st->downstream[i];
-      m->mean_demand[i] += st->mean_nus[i];
-      while (k < 5 && downstreamCell > 0 &&
-             (downstreamCell < n ? st->reservoir_area[downstreamCell] : 0.) <= 0) {
-        const double coeff = st->alloc_coeff ? st->alloc_coeff[(size_t)i * 5 + k] : 1.0;
-        k++;
-        m->mean_demand[i] += st->mean_nus[downstreamCell - 1] * coeff;
-        downstreamCell = st->downstream[downstreamCell - 1];
-      }
-    }
-    m->mean_demand[i] = m->mean_demand[i] / 31536000.;
-  }
+  mean_demand(m, st);
   /* routing.cpp:530-538 */
   for (int32_t i = 0; i < n; i++) m->routing_cell[st->rout_order[i] - 1] = i + 1;
   m->started = 0;
   return m;
+}
+
+/* routingClass::annualInit for a later year (routing.cpp:979-1495), parity profile:
+ * st holds this year's reservoir grids (Parts 1-2, routing.cpp:1015-1100, as the
+ * caller derives them); Part 4 (routing.cpp:1300-1421) moves the land of a growing
+ * reservoir fraction into the reservoir storage of the cell's outflow cell
+ * assignment.  The branch of routing.cpp:1354-1357 (land not shrunk against the
+ * day before) is refused with -5, as the library refuses it. */
+int wgo_annual_init(wgo_model *m, const wg_static_fields *st, const int32_t *outflow_cell, int apply_changes,
+                    double *glo_res_prev) {
+  const int32_t n = m->n;
+  if (apply_changes) {
+    for (int32_t i = 0; i < n; i++) {
+      double change = st->glo_res[i] - glo_res_prev[i];           /* routing.cpp:1315 */
+      if (change > 0.) {
+        double laf = (glo_res_prev[i] > 0.) ? m->laf[i] - change : m->laf[i] - st->glo_res[i];
+        if (laf < 0.) laf = 0.;
+        if (laf - m->laf_prev[i] >= 0.) return -5;
+      }
+    }
+    for (int32_t i = 0; i < n; i++) {
+      double change = st->glo_res[i] - glo_res_prev[i];
+      if (change > 0.) {
+        /* routing.cpp:1326-1332 */
+        if (glo_res_prev[i] > 0.)
+          m->laf[i] = m->laf[i] - change;
+        else
+          m->laf[i] = m->laf[i] - st->glo_res[i];
+        if (m->laf[i] < 0.) m->laf[i] = 0.;
+        {
+          /* routing.cpp:1346-1400 */
+          double laf_change = m->laf[i] - m->laf_prev[i];
+          double cellArea = st->cell_area[i];
+          double canopy_km3 = m->canopy[i] * cellArea / 1000000.0 * ((-laf_change) / 100.0);
+          double soil_km3 = m->soil[i] * cellArea / 1000000.0 * ((-laf_change) / 100.0);
+          double snow_km3 = m->snow[i] * cellArea / 1000000.0 * ((-laf_change) / 100.0);
+          m->res[outflow_cell[i] - 1] += canopy_km3 + soil_km3 + snow_km3;
+        }
+      }
+    }
+  }
+  /* this year's reservoirs: the statics, and the demand terms that depend on them
+   * (routing.cpp:360-403) */
+  m->st = st;
+  mean_demand(m, st);
+  for (int32_t i = 0; i < n; i++) glo_res_prev[i] = st->glo_res[i];  /* routing.cpp:5479-5483 at year end */
+  return 0;
 }
 
 void wgo_destroy(wgo_model *m) {

@@ -50,6 +50,12 @@ void wgo_reservoir_start_month(int32_t n, const double *monthly, const double *a
  * setLakeWetlToMaximum + annualInit (first year) + laiClass::init. */
 wgo_model *wgo_create(int32_t n, const wg_static_fields *st);
 void wgo_destroy(wgo_model *m);
+/* routingClass::annualInit for a later year (routing.cpp:979-1495): this year's
+ * reservoir statics in st (kept by pointer), Part 4's land-to-reservoir transfer;
+ * glo_res_prev [N] is the previous year's G_RES and receives this year's.
+ * Returns -5 for the unsupported branch of routing.cpp:1354-1357. */
+int wgo_annual_init(wgo_model *m, const wg_static_fields *st, const int32_t *outflow_cell, int apply_changes,
+                    double *glo_res_prev);
 
 /* Per-cell overrides of gamma, CFA and Smax (NULL = keep). */
 void wgo_set_params(wgo_model *m, const double *gamma, const double *cfa, const float *smax);

@@ -42,6 +42,9 @@ def load(variant="wg"):
     lib.wgo_create.restype = C.c_void_p
     lib.wgo_destroy.argtypes = [C.c_void_p]
     lib.wgo_destroy.restype = None
+    lib.wgo_annual_init.argtypes = [C.c_void_p, C.POINTER(_abi.StaticFields), C.POINTER(C.c_int32), C.c_int,
+                                    C.POINTER(C.c_double)]
+    lib.wgo_annual_init.restype = C.c_int
     lib.wgo_set_params.argtypes = [C.c_void_p, P_d, P_d, P_f]
     lib.wgo_set_params.restype = None
     lib.wgo_step_day.argtypes = [C.c_void_p, P_f, P_f, P_f, P_f, C.c_int, C.c_int]
@@ -265,6 +268,20 @@ class Oracle:
                 self._m = None
         except Exception:
             pass
+
+    def annual_init(self, year_statics, outflow_cell=None, apply_changes=True):
+        """wgo_annual_init: routingClass::annualInit for a later year."""
+        import watergap2_amd as _wg
+        if not hasattr(self, "_glo_res_prev"):
+            self._glo_res_prev = np.ascontiguousarray(self.world.glo_res, dtype=np.float64).copy()
+        self._year_st = _wg.World.static_fields(year_statics)   # kept: the oracle reads it by pointer
+        self._year_obj = year_statics
+        oc = None if outflow_cell is None else np.ascontiguousarray(outflow_cell, dtype=np.int32)
+        self._year_oc = oc
+        rc = self._lib.wgo_annual_init(self._m, C.byref(self._year_st), None if oc is None else _p(oc, C.POINTER(C.c_int32)),
+                                       1 if apply_changes else 0, _p(self._glo_res_prev, C.POINTER(C.c_double)))
+        if rc != 0:
+            raise RuntimeError(f"wgo_annual_init returned {rc}")
 
     def set_reference_scans(self, station_scan=False, aggr_scan=False):
         self._lib.wgo_set_reference_scans(self._m, int(station_scan), int(aggr_scan))

@@ -229,6 +229,15 @@ class WaterBalanceModel:
         assert forcing.ndim == 3 and forcing.shape[1:] == (4, self.n)
         self._c(self._lib.wg_load_forcing(self._h, _abi.ptr(forcing, _abi.P_f), forcing.shape[0]))
 
+    def annual_init(self, year_statics, outflow_cell=None, apply_changes=True):
+        """routingClass::annualInit for a year after the first (wg_annual_init):
+        year_statics is the world with this year's glo_res, reservoir_area and
+        stor_cap (any object with the World's static arrays)."""
+        st = World.static_fields(year_statics)
+        oc = None if outflow_cell is None else np.ascontiguousarray(outflow_cell, dtype=np.int32)
+        self._c(self._lib.wg_annual_init(self._h, C.byref(st), None if oc is None else _abi.ptr(oc, _abi.P_i32),
+                                         1 if apply_changes else 0))
+
     def set_stations(self, cells):
         cells = np.ascontiguousarray(cells, dtype=np.int32)
         self.n_stations = len(cells)
@@ -326,3 +335,18 @@ class WaterBalanceModel:
 
     def synchronize(self):
         self._c(self._lib.wg_synchronize(self._h))
+
+
+def reservoir_year(year_to_use, reservoir_area_full, stor_cap_full, start_year, reg_lake_status=None):
+    """Parts 1-2 of annualInit (wg_reservoir_year): the area and capacity of the
+    reservoirs and regulated lakes in operation in the year to use."""
+    lib = _abi.load_library()
+    af = np.ascontiguousarray(reservoir_area_full, dtype=np.float64)
+    cf = np.ascontiguousarray(stor_cap_full, dtype=np.float64)
+    sy = np.ascontiguousarray(start_year, dtype=np.int32)
+    rl = None if reg_lake_status is None else np.ascontiguousarray(reg_lake_status, dtype=np.int8)
+    area, cap = np.zeros(len(af)), np.zeros(len(af))
+    _check(lib.wg_reservoir_year(len(af), int(year_to_use), _abi.ptr(af, _abi.P_d), _abi.ptr(cf, _abi.P_d),
+                                 _abi.ptr(sy, _abi.P_i32), None if rl is None else _abi.ptr(rl, _abi.P_i8),
+                                 _abi.ptr(area, _abi.P_d), _abi.ptr(cap, _abi.P_d)))
+    return area, cap

@@ -271,6 +271,8 @@ def _declare(lib):
     lib.wg_station_corr_factor.restype = C.c_float
     lib.wg_calib_basins.argtypes = [i32, P_i32, P_i32, i32, P_i32, P_i32, P_i16, P_i32]
     lib.wg_upstream_stations.argtypes = [i32, P_i32, i32, P_i32, i32, P_i32, P_i32, i32]
+    lib.wg_annual_init.argtypes = [C.c_void_p, C.POINTER(StaticFields), P_i32, i32]
+    lib.wg_reservoir_year.argtypes = [i32, i32, P_d, P_d, P_i32, P_i8, P_d, P_d]
     lib.wg_mlp_create.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
     lib.wg_mlp_destroy.argtypes = [C.c_void_p]
     lib.wg_mlp_last_error.argtypes = [C.c_void_p]
@@ -310,6 +312,6 @@ ABI_SYMBOLS = [
     "wg_autotune", "wg_set_launch_config",
     "wg_gamma_search_init", "wg_gamma_start", "wg_find_new_gamma",
     "wg_mlp_create", "wg_mlp_destroy", "wg_mlp_last_error", "wg_mlp_n_params", "wg_mlp_get_params",
-    "wg_station_corr_factor", "wg_calib_basins", "wg_upstream_stations",
+    "wg_station_corr_factor", "wg_calib_basins", "wg_upstream_stations", "wg_annual_init", "wg_reservoir_year",
     "wg_mlp_set_params", "wg_mlp_forward", "wg_mlp_forward_noalloc", "wg_mlp_gradient", "wg_mlp_train",
 ]

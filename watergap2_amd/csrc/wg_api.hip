@@ -82,6 +82,8 @@ struct wg_handle {
   std::vector<int32_t> tile_start, tile_end, block_group;
   int32_t tile_capacity = 0;      // entries of the device tile tables
   std::vector<WgChainLink> clink;
+  std::vector<int32_t> wave_wide;    // wide levels of each group (host copy of d_wave_wide)
+  std::vector<double> glo_res_prev;  // G_glores_prevyear, reference order (routing.cpp:5479-5483)
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms[3] = {0, 0, 0};
@@ -457,6 +459,33 @@ int build_schedule(wg_handle *h, const wg_static_fields *st, int group_cells) {
   return WG_OK;
 }
 
+// The wave walk's batch table for W wavefronts: the batches of the wide levels of
+// every group dealt round-robin, each flagged when one of its cells has a global
+// lake or reservoir (from the chain links' flags).
+void wave_tables(const wg_handle *H, int nw, std::vector<int32_t> &item_off, std::vector<WgWaveItem> &items) {
+  item_off.assign(size_t(H->n_groups) * nw + 1, 0);
+  items.clear();
+  for (int32_t gg = 0; gg < H->n_groups; gg++) {
+    const int32_t l0 = H->group_first_level[gg];
+    for (int w = 0; w < nw; w++) {
+      item_off[size_t(gg) * nw + w] = int32_t(items.size());
+      for (int32_t L = 1; L <= H->wave_wide[gg]; L++) {
+        const int32_t a = H->level_off[l0 + L], e = H->level_off[l0 + L + 1];
+        int32_t j = 0;
+        for (int32_t st0 = a; st0 < e; st0 += 64, j++) {
+          if (j % nw != w) continue;
+          const int32_t end = std::min(st0 + 64, e);
+          int32_t lr = 0;
+          for (int32_t q = st0; q < end; q++) lr |= (H->clink[q].flags & (CFL_LAKE | CFL_RES)) ? 1 : 0;
+          items.push_back(WgWaveItem{st0, end, L, lr});
+        }
+      }
+    }
+  }
+  item_off[size_t(H->n_groups) * nw] = int32_t(items.size());
+  if (items.empty()) items.push_back(WgWaveItem{0, 0, 0, 0});
+}
+
 int upload_tiles(wg_handle *h) {
   if (int32_t(h->tile_start.size()) > h->tile_capacity) return fail(h, WG_E_ARG, "tile table too large");
   HIP_CHECK(h, hipMemcpy(const_cast<int32_t *>(h->S.tile_start), h->tile_start.data(),
@@ -588,6 +617,7 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
   }
 
   const std::vector<double> prov_rel = reservoir_prov_rel(st, n);
+  H->glo_res_prev.assign(st->glo_res, st->glo_res + n);  // routing.cpp:1085-1088
 
   // initial state (one member, reference order)
   std::vector<double> s_river(n), s_loc_lake(n), s_loc_wetl(n), s_glo_lake(n), s_glo_wetl(n), s_res(n),
@@ -812,29 +842,11 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
       }
       r_ = upload(H, &H->d_wave_wide, wide);
       if (r_) { *out = h.release(); return r_; }
+      H->wave_wide = wide;
       for (int t = 0; t < 2; t++) {
-        const int nw = t == 0 ? 4 : 8;
-        std::vector<int32_t> item_off(size_t(H->n_groups) * nw + 1, 0);
+        std::vector<int32_t> item_off;
         std::vector<WgWaveItem> items;
-        for (int32_t gg = 0; gg < H->n_groups; gg++) {
-          const int32_t l0 = H->group_first_level[gg];
-          for (int w = 0; w < nw; w++) {
-            item_off[size_t(gg) * nw + w] = int32_t(items.size());
-            for (int32_t L = 1; L <= wide[gg]; L++) {
-              const int32_t a = H->level_off[l0 + L], e = H->level_off[l0 + L + 1];
-              int32_t j = 0;
-              for (int32_t st0 = a; st0 < e; st0 += 64, j++) {
-                if (j % nw != w) continue;
-                const int32_t end = std::min(st0 + 64, e);
-                int32_t lr = 0;
-                for (int32_t q = st0; q < end; q++) lr |= (H->clink[q].flags & (CFL_LAKE | CFL_RES)) ? 1 : 0;
-                items.push_back(WgWaveItem{st0, end, L, lr});
-              }
-            }
-          }
-        }
-        item_off[size_t(H->n_groups) * nw] = int32_t(items.size());
-        if (items.empty()) items.push_back(WgWaveItem{0, 0, 0, 0});
+        wave_tables(H, t == 0 ? 4 : 8, item_off, items);
         r_ = upload(H, &H->d_wave_item_off[t], item_off);
         if (r_) { *out = h.release(); return r_; }
         r_ = upload(H, &H->d_wave_items[t], items);
@@ -1517,6 +1529,140 @@ extern "C" int wg_station_daily_window(wg_handle *h, int32_t *first_day, int32_t
   if (!h || !first_day || !n_days) return WG_E_ARG;
   *first_day = h->last_daily_first;
   *n_days = h->last_daily_days;
+  return WG_OK;
+}
+
+namespace {
+
+// Chain flags of reference cell i (the links' flags, with the reservoir type and
+// start month in bits 8-23 when with_res_bits).
+int chain_flags(const wg_static_fields *st, int32_t i, bool with_res_bits) {
+  int f = 0;
+  if (st->lake_area[i] > 0.) f |= CFL_LAKE;
+  if (st->reservoir_area[i] > 0.) f |= CFL_RES;
+  if (st->glo_wetland[i] > 0) f |= CFL_GWET;
+  if (st->aindex[i] == 1 && st->ldd[i] >= 0) f |= CFL_ARID;
+  if (st->ldd[i] >= 0) f |= CFL_LDD;
+  if (!(st->contfreq[i] < 0.000001)) f |= CFL_CONT;
+  if (with_res_bits && st->reservoir_area[i] > 0.)
+    f |= (int(st->res_type[i]) & 0xff) << 8 | (int(st->start_month[i]) & 0xff) << 16;
+  return f;
+}
+
+template <class T>
+int to_host(wg_handle *h, std::vector<T> &out, const T *dev, size_t count) {
+  out.resize(count);
+  HIP_CHECK(h, hipMemcpy(out.data(), dev, count * sizeof(T), hipMemcpyDeviceToHost));
+  return WG_OK;
+}
+
+template <class T>
+int to_device(wg_handle *h, const T *dev, const std::vector<T> &in) {
+  HIP_CHECK(h, hipMemcpy(const_cast<T *>(dev), in.data(), in.size() * sizeof(T), hipMemcpyHostToDevice));
+  return WG_OK;
+}
+
+}  // namespace
+
+extern "C" int wg_annual_init(wg_handle *h, const wg_static_fields *st, const int32_t *outflow_cell,
+                              int32_t apply_changes) {
+  if (!h || !st) return WG_E_ARG;
+  const int32_t n = h->n, M = h->m;
+  const void *req[] = {st->cell_area, st->contfreq, st->aindex, st->ldd, st->glo_wetland, st->lake_area,
+                       st->reservoir_area, st->glo_res, st->stor_cap, st->mean_outflow, st->res_type,
+                       st->start_month, st->downstream};
+  for (const void *q : req)
+    if (!q) return fail(h, WG_E_ARG, "wg_annual_init: a required static field is NULL");
+  for (int32_t i = 0; i < n; i++) {
+    if (st->reservoir_area[i] > 0. && !(st->res_type[i] == 1 || st->res_type[i] == 2))
+      return fail(h, WG_E_INPUT, "cell %d: reservoir of unknown type (routing.cpp:1106-1115)", i);
+    if (st->reservoir_area[i] > 0. && (st->start_month[i] < 1 || st->start_month[i] > 12))
+      return fail(h, WG_E_INPUT, "cell %d: reservoir start month out of range", i);
+  }
+  HIP_CHECK(h, hipSetDevice(h->device));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  // Part 4 of routingClass::annualInit (routing.cpp:1300-1421): a cell whose
+  // reservoir fraction grows gives the new reservoir its land: the land-area
+  // fraction falls by the growth, and the water of the lost land (canopy, soil and
+  // snow, mm over the land area) moves to the reservoir storage of the cell's
+  // outflow cell assignment.  Cells in reference order, every member.
+  std::vector<int32_t> grown;
+  if (apply_changes) {
+    for (int32_t i = 0; i < n; i++)
+      if (st->glo_res[i] - h->glo_res_prev[i] > 0.) grown.push_back(i);
+  }
+  if (!grown.empty()) {
+    if (!outflow_cell) return fail(h, WG_E_ARG, "wg_annual_init: outflow cell assignment missing");
+    for (int32_t i : grown)
+      if (outflow_cell[i] < 1 || outflow_cell[i] > n)
+        return fail(h, WG_E_INPUT, "cell %d: outflow cell assignment %d out of range", i, outflow_cell[i]);
+    const size_t NM = size_t(n) * M;
+    std::vector<double> canopy, soil, snow, laf, laf_prev, res;
+    int rc = to_host(h, canopy, h->X.canopy, NM);
+    if (!rc) rc = to_host(h, soil, h->X.soil, NM);
+    if (!rc) rc = to_host(h, snow, h->X.snow, NM);
+    if (!rc) rc = to_host(h, laf, h->X.laf, NM);
+    if (!rc) rc = to_host(h, laf_prev, h->X.laf_prev, NM);
+    if (!rc) rc = to_host(h, res, h->X.res, NM);
+    if (rc) return rc;
+    // every member's new fractions first: a cell whose land did not shrink against
+    // the day before takes a branch (routing.cpp:1349-1357) that keeps the next
+    // day's fraction apart from today's, which this state does not hold
+    for (int32_t m = 0; m < M; m++) {
+      for (int32_t i : grown) {
+        const size_t mp = size_t(m) * n + h->inv[i];
+        const double change = st->glo_res[i] - h->glo_res_prev[i];
+        double f = h->glo_res_prev[i] > 0. ? laf[mp] - change : laf[mp] - st->glo_res[i];
+        if (f < 0.) f = 0.;
+        if (f - laf_prev[mp] >= 0.)
+          return fail(h, WG_E_INPUT, "cell %d member %d: the land fraction did not fall against the day before "
+                      "(routing.cpp:1354-1357 is not supported)", i, m);
+      }
+    }
+    for (int32_t m = 0; m < M; m++) {
+      for (int32_t i : grown) {
+        const size_t mp = size_t(m) * n + h->inv[i];
+        const size_t ma = size_t(m) * n + h->inv[outflow_cell[i] - 1];
+        const double change = st->glo_res[i] - h->glo_res_prev[i];
+        double f = h->glo_res_prev[i] > 0. ? laf[mp] - change : laf[mp] - st->glo_res[i];
+        if (f < 0.) f = 0.;
+        const double lost = f - laf_prev[mp];
+        const double area = st->cell_area[i];
+        const double c = canopy[mp] * area / 1000000.0 * ((-lost) / 100.0);
+        const double so = soil[mp] * area / 1000000.0 * ((-lost) / 100.0);
+        const double sn = snow[mp] * area / 1000000.0 * ((-lost) / 100.0);
+        res[ma] += c + so + sn;
+        laf[mp] = f;
+      }
+    }
+    rc = to_device(h, h->X.laf, laf);
+    if (!rc) rc = to_device(h, h->X.res, res);
+    if (rc) return rc;
+  }
+  // Parts 1-2 (routing.cpp:1015-1100): this year's reservoirs, as the caller derived
+  // them from the full grids and the start years; flags, tables and the release
+  // terms follow them.  The device order stays (cells with a reservoir last in
+  // their level is a speed choice, not a condition).
+  int rc = to_device(h, h->S.reservoir_area, permute(h, st->reservoir_area));
+  if (!rc) rc = to_device(h, h->S.stor_cap, permute(h, st->stor_cap));
+  if (!rc) rc = to_device(h, h->S.res_prov_rel, permute(h, reservoir_prov_rel(st, n).data()));
+  if (rc) return rc;
+  std::vector<int8_t> cflags(n);
+  for (int32_t p = 0; p < n; p++) {
+    const int32_t i = h->perm[p];
+    cflags[p] = int8_t(chain_flags(st, i, false));
+    h->clink[p].flags = chain_flags(st, i, true);
+  }
+  rc = to_device(h, h->S.cflags, cflags);
+  if (!rc) rc = to_device(h, h->S.clink, pack_links(h->clink));
+  for (int t = 0; t < 2 && !rc; t++) {
+    std::vector<int32_t> item_off;
+    std::vector<WgWaveItem> items;
+    wave_tables(h, t == 0 ? 4 : 8, item_off, items);
+    rc = to_device(h, h->d_wave_items[t], items);
+  }
+  if (rc) return rc;
+  h->glo_res_prev.assign(st->glo_res, st->glo_res + n);
   return WG_OK;
 }
 

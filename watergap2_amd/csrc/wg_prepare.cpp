@@ -435,3 +435,23 @@ extern "C" int wg_prepare_routing(const wg_routing_inputs *in, wg_routing_files 
   }
   return WG_OK;
 }
+
+// routingClass::annualInit, Parts 1-2 (routing.cpp:1015-1100): the reservoirs and
+// regulated lakes in operation in the year to use.
+extern "C" int wg_reservoir_year(int32_t n, int32_t year_to_use, const double *area_full, const double *cap_full,
+                                 const int32_t *start_year, const int8_t *reg_lake, double *area, double *cap) {
+  if (n <= 0 || !area_full || !cap_full || !start_year || !area || !cap) return WG_E_ARG;
+  for (int32_t i = 0; i < n; i++) {
+    area[i] = 0.;  // routing.cpp:1015-1016
+    cap[i] = 0.;
+    if (reg_lake && reg_lake[i] == 1) {  // routing.cpp:1040-1048
+      area[i] = area_full[i];
+      cap[i] = cap_full[i];
+    }
+    if (year_to_use >= start_year[i]) {  // routing.cpp:1091-1095
+      area[i] = area_full[i];
+      cap[i] = cap_full[i];
+    }
+  }
+  return WG_OK;
+}
