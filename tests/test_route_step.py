@@ -39,7 +39,7 @@ int main(int argc, char **argv) {
   const long n = std::atol(argv[1]);
   std::mt19937_64 rng(7);
   std::uniform_real_distribution<double> U(0., 1.);
-  long n_fast = 0, n_bad = 0, n_res = 0, n_dry = 0, n_wet0 = 0, n_lake0 = 0;
+  long n_fast = 0, n_bad = 0, n_res = 0, n_dry = 0;
   for (long it = 0; it < n; it++) {
     ChainRec c;
     int flags = CFL_CONT;
@@ -70,8 +70,6 @@ int main(int argc, char **argv) {
     ChainOut b = step_fast(c, pre, up, slow);
     if (slow) b = chain_step(c, up, d);
     else n_fast++;
-    if (!slow && (flags & CFL_GWET) && a.glo_wetl == 0. && c.v[CF_GW_PET] > 0.) n_wet0++;  // emptied wetland, fast path
-    if (!slow && (flags & CFL_LAKE) && a.glo_lake < 0.) n_lake0++;                           // lake below zero, fast path
     if (flags & CFL_RES) n_res++;
     if (a.river == 0.) n_dry++;
     bool ok = same(a.tv, b.tv) && same(a.river, b.river) && same(a.gw, b.gw) && same(a.glo_wetl, b.glo_wetl);
@@ -80,7 +78,7 @@ int main(int argc, char **argv) {
     if (!ok && n_bad++ < 5)
       std::printf("mismatch %ld flags %x: tv %.17g %.17g river %.17g %.17g\n", it, flags, a.tv, b.tv, a.river, b.river);
   }
-  std::printf("%ld %ld %ld %ld %ld %ld %ld\n", n, n_fast, n_bad, n_res, n_dry, n_wet0, n_lake0);
+  std::printf("%ld %ld %ld %ld %ld\n", n, n_fast, n_bad, n_res, n_dry);
   return 0;
 }
 """
@@ -118,9 +116,7 @@ def test_day_info_matches_the_calendar(route_step_exe):
 def test_split_step_gives_chain_step_bits(route_step_exe):
     out = subprocess.run([route_step_exe, "1000000"], check=True, capture_output=True, text=True).stdout
     lines = out.strip().splitlines()
-    n, n_fast, n_bad, n_res, n_dry, n_wet0, n_lake0 = map(int, lines[-1].split())
+    n, n_fast, n_bad, n_res, n_dry = map(int, lines[-1].split())
     assert n_bad == 0, out
-    # the draw covers both paths, reservoirs, dry rivers, and wetlands and lakes
-    # emptied on the fast path
+    # the draw covers both paths, reservoirs and dry rivers
     assert n_fast > n // 2 and n - n_fast > n // 20 and n_res > n // 20 and n_dry > n // 100, out
-    assert n_wet0 > n // 100 and n_lake0 > n // 1000, out

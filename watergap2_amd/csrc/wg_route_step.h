@@ -314,14 +314,14 @@ WG_RS_FN ChainOut chain_step(const ChainRec &c, double inflow_up, const DayInfo 
 
 // The common case of the routing step, split so that what does not depend on the
 // upstream inflow is done beforehand: a cell without a reservoir whose global lake
-// and global wetland keep water, or lie in a humid cell.  In that case the arid groundwater step (routing.cpp:3305-3386) does not depend on
-// the inflow (step_pre), and the inflow-dependent rest (step_fast) is the lake,
-// wetland and river steps, the latter two with selects in place of branches.
-// Every operation is the one chain_step performs, in the same order, so the
-// results are chain_step's bits.  step_fast flags the cells that need chain_step:
-// a reservoir, or a lake or wetland that runs dry in an arid cell (their scaled
-// groundwater recharge enters the arid groundwater step, routing.cpp:2708 and
-// 3235).
+// and global wetland (if any) keep water.  In that case the arid groundwater step
+// (routing.cpp:3305-3386) does not depend on the inflow (step_pre), and the
+// inflow-dependent rest (step_fast) is the lake, wetland and river steps, the
+// latter two with selects in place of branches.  Every operation is the one
+// chain_step performs, in the same order, so the results are chain_step's bits.
+// step_fast flags the cells that need chain_step: a reservoir, or a lake or
+// wetland that runs dry (which rescales its groundwater recharge,
+// routing.cpp:2708 and 3235).
 struct StepPre {
   double lgwrir, gw;
 };
@@ -350,14 +350,11 @@ WG_RS_FN ChainOut step_fast(const ChainRec &c, const StepPre &pre, double inflow
   inflow += inflow_up;  // routing.cpp:2623
   slow = (c.flags & CFL_RES) != 0;
   if (c.flags & CFL_LAKE) {
-    // global lake, routing.cpp:2630-2804.  A lake whose use exceeds what it holds
-    // ends at minus its maximum storage with no outflow; only its scaled
-    // groundwater recharge, which feeds the arid groundwater step, needs chain_step.
+    // global lake, routing.cpp:2630-2804, the branch that keeps water
     const double prev = v[CF_GL_PREV], use = v[CF_GL_USE], maxStorage = v[CF_GL_MAX];
     const double totalInflow = inflow + v[CF_GL_LP];
     const double useMax = totalInflow + maxStorage + prev;
-    const bool dry = use > useMax;
-    slow = slow || (dry && (c.flags & CFL_ARID));
+    slow = slow || (use > useMax);
     double st = v[CF_GL_S0E] + c.inv_ksw * (totalInflow - use) * c.ome_sw;
     double outflow = totalInflow + prev - st - use;
     const bool over = st > maxStorage;
@@ -366,17 +363,12 @@ WG_RS_FN ChainOut step_fast(const ChainRec &c, const StepPre &pre, double inflow
     const bool neg = outflow < 0.;
     st = neg ? prev + totalInflow - use : st;
     outflow = neg ? 0. : outflow;
-    st = dry ? (-1.) * maxStorage : st;
-    outflow = dry ? 0. : outflow;
     st = (fabs(st) <= kMinStorVol) ? 0. : st;
     o.glo_lake = st;
     inflow = outflow;
   }
   {
-    // global wetland, routing.cpp:3178-3297.  A wetland that runs dry ends empty
-    // with no outflow (its maximum storage is not negative, so the overflow test
-    // leaves the zeros alone); only its scaled groundwater recharge, which feeds the
-    // arid groundwater step, needs chain_step.
+    // global wetland, routing.cpp:3178-3297, the branch that keeps water
     const double prev = v[CF_GW_PREV], PETgwr = v[CF_GW_PET], maxStorage = v[CF_GW_MAX];
     const double totalInflow = inflow + v[CF_GW_WP];
     const double PETgwrMax = totalInflow + prev;
@@ -387,10 +379,7 @@ WG_RS_FN ChainOut step_fast(const ChainRec &c, const StepPre &pre, double inflow
     st = over ? maxStorage : st;
     st = (fabs(st) <= kMinStorVol) ? 0. : st;
     const bool gwet = (c.flags & CFL_GWET) != 0;
-    const bool dry = PETgwr > PETgwrMax;
-    st = dry ? 0. : st;
-    outflow = dry ? 0. : outflow;
-    slow = slow || (gwet && dry && (c.flags & CFL_ARID));
+    slow = slow || (gwet && PETgwr > PETgwrMax);
     o.glo_wetl = gwet ? st : prev;
     inflow = gwet ? outflow : inflow;
   }
