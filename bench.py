@@ -502,8 +502,8 @@ def main():
         return
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the path shards by basin with no "
-                          "per-day exchange (see DESIGN.md), but only --gpus 1 is measured"}))
+        print(json.dumps({"error": "this build runs on one GPU; the path would shard by basin with no "
+                          "per-day exchange, but only --gpus 1 is measured"}))
         sys.exit(2)
 
     import watergap2_amd as wg

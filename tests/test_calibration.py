@@ -164,6 +164,26 @@ def test_basin_set_equals_separate_basins(wg, oracle):
         assert q_whole[b] == ob.station_discharge()[0]
 
 
+def test_cut_basins_puts_no_station_on_an_inland_sink(wg):
+    """A station at an inland sink records no discharge (routing.cpp:4232-4238), so
+    cut_basins never takes one, although sinks with catchments in range exist."""
+    from watergap2_amd.calibration import cut_basins, upstream_cells
+    dem = wg.World(wg.global_spec(seed=3, n_land=20000, n_outlets=40, n_sinks=60))
+    ldd = np.asarray(dem.ldd)
+    count = upstream_cells(dem)
+    sinks = np.flatnonzero(ldd == -1)
+    assert ((count[sinks] >= 20) & (count[sinks] <= 2000)).sum() >= 5
+    basins = cut_basins(dem, 80, np.random.default_rng(1), 20, 2000)
+    assert len(basins) >= 40
+    ds = dem.downstream.astype(np.int64) - 1
+    for cells in basins:
+        inside = np.zeros(dem.n, dtype=bool)
+        inside[cells] = True
+        station = [c for c in cells if ds[c] < 0 or not inside[ds[c]]]
+        assert len(station) == 1
+        assert ldd[station[0]] != -1
+
+
 def test_thirty_year_mean_from_nine_years():
     from watergap2_amd import surrogate as sg
     # years 1-9 as run, then seven more copies of the last cycle (years 7-9)

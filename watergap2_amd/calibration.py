@@ -111,13 +111,14 @@ def cut_basins(dem, n_basins, rng, min_cells, max_cells):
     in [min_cells, max_cells]: for each drawn size (largest first) the station is
     the free cell whose catchment count is nearest.  A cell is free when it lies in
     no basin taken before and no such basin lies upstream of it (cells on the flow
-    path below a taken station are marked as they are reached).  Returns lists of
-    0-based cells."""
+    path below a taken station are marked as they are reached).  Inland sinks are
+    never stations: their discharge is not recorded (routing.cpp:4232-4238), so
+    such a basin has nothing to calibrate against.  Returns lists of 0-based cells."""
     count = upstream_cells(dem)
     ds = dem.downstream.astype(np.int64)
     taken = np.zeros(dem.n, dtype=bool)
     sizes = np.exp(rng.uniform(math.log(min_cells), math.log(max_cells), n_basins))
-    in_range = (count >= min_cells) & (count <= max_cells)
+    in_range = (count >= min_cells) & (count <= max_cells) & (np.asarray(dem.ldd) != -1)
     out = []
     for size in sorted(sizes, reverse=True):
         cand = np.flatnonzero(in_range & ~taken)
