@@ -444,6 +444,17 @@ int wg_get_station_discharge(wg_handle *h, double *sum, double *daily, int32_t m
 int wg_station_daily_window(wg_handle *h, int32_t *first_day, int32_t *n_days);
 int wg_reset_station_discharge(wg_handle *h);
 
+/* G_potCellRunoff (routing.h:286), the per-cell sum of the daily potential cell
+ * runoff in km3 (routing.cpp:1962-1970 (semi-)arid cells through 3372-3379, 2045-2053
+ * humid cells, 2180-2187 inland sinks): the cell's local runoff terms plus the
+ * open-water balance (daily.cpp:773) over its lake, reservoir and wetland area, the
+ * quantity the cell correction factors are computed from.  An output the day loop
+ * keeps only when asked: enable 1 allocates it (or, when on, restarts the sums at
+ * 0), 0 frees it.  wg_annual_init restarts the sums (routing.cpp:1009). */
+int wg_set_pot_cell_runoff(wg_handle *h, int32_t enable);
+/* The sums since the last restart for one member, pot[N] in reference cell order. */
+int wg_get_pot_cell_runoff(wg_handle *h, int32_t member, double *pot);
+
 /* Diagnostics of the last simulated day for one member. */
 int wg_get_day_outputs(wg_handle *h, int32_t member, wg_day_outputs *out);
 
@@ -509,6 +520,7 @@ typedef struct wg_gamma_search {
   float cell_corr_factor;        /* basin-wide CFA */
   float gamma_upper_limit, gamma_lower_limit;  /* 5 and 0.1 (calibration.cpp:45-48) */
   float last_nash_sutcliffe, last_sum_diff, measured_avg_adapt;  /* CALIBRATION.OUT columns */
+  float sim_runoff_avg;          /* simRunoffSum / numberOfYears of the last call */
 } wg_gamma_search;
 
 /* Fresh search state (calibGammaClass constructor + function statics). */
@@ -529,6 +541,19 @@ float wg_find_new_gamma(wg_gamma_search *s, float gamma_old, int32_t n_years, co
  * within 1 % of 1 it is 1.  Sets calib_status 4 when CFS differs from 1. */
 float wg_station_corr_factor(wg_gamma_search *s, float gamma, int32_t n_years, const float *measured,
                              const float *simulated);
+
+/* calibGammaClass::createCorrectionGrid (calibration.cpp:730-801): the cell
+ * correction factors of one calibration basin.  annual_pot[n_years][N] holds the
+ * annual potential cell runoff of the evaluation years as the reference stores it
+ * (G_POT_CELL_RUNOFF_<year>.UNF0, float).  Their mean (float, mean_pot when not NULL)
+ * weighs the basin's discharge difference: for each cell with calib_basin[i] ==
+ * station_number, corr[i] = 1 - sign(mean) * (simulated - measured) / sum |mean|,
+ * limited to 0.5 .. 1.5; simulated is the mean simulated discharge of the years and
+ * measured the adapted measured mean (findNewGamma's arguments, calibration.cpp:627,
+ * wg_gamma_search.sim_runoff_avg and .measured_avg_adapt).  Other cells of corr keep
+ * their values. */
+int wg_cell_corr_factor_grid(int32_t n_cells, const int16_t *calib_basin, int32_t station_number, int32_t n_years,
+                             const float *annual_pot, float simulated, float measured, float *mean_pot, double *corr);
 
 /* Calibration basins from station cells (1-based) in the order of the station file
  * (cbasinClass::prepare with upstreamBasin, calib_basins.cpp:25-300, 302-378): each

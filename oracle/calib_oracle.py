@@ -25,6 +25,8 @@ class GammaSearchState:
         self.cell_corr_factor = f32(1)       # calibration.h:48
         self.upper = f32(5.)                 # calibration.cpp:47
         self.lower = f32(0.1)                # calibration.cpp:48
+        self.measured_avg_adapt = f32(0)     # measuredRunoffAvgAdapt of the last call
+        self.sim_runoff_avg = f32(0)         # simRunoffSum / numberOfYears of the last call
 
 
 def gamma_start(state, gamma_from_file):
@@ -151,7 +153,38 @@ def find_new_gamma(st, gamma_old, measured, simulated, inflow=None, water_use=No
     st.sum_diff_old = sum_diff
     if gamma < 0 and abs(float(f32(st.cell_corr_factor - f32(1)))) > 0.01:
         st.calib_status = 3
+    st.measured_avg_adapt = adapt
+    st.sim_runoff_avg = mean_sim
     return gamma
+
+
+def correction_grid(calib_basin, station_number, annual_pot, simulated, measured, corr):
+    """calibration.cpp:730-801 (createCorrectionGrid) in float: the mean of the
+    annual grids, the basin's sum of absolute means (in cell order), and for each
+    basin cell 1 - sign(mean) * (simulated - measured) / sum, limited to 0.5 .. 1.5.
+    Returns (corr updated, mean)."""
+    annual = np.asarray(annual_pot, dtype=np.float32)
+    ny = annual.shape[0]
+    mean = np.zeros(annual.shape[1], dtype=np.float32)
+    for y in range(ny):
+        mean = (mean + annual[y]).astype(np.float32)
+    mean = (mean / np.float32(ny)).astype(np.float32)
+    cells = np.flatnonzero(np.asarray(calib_basin) == station_number)
+    total = f32(0)
+    for i in cells:
+        total = f32(total + abs(mean[i]))
+    corr = np.array(corr, dtype=np.float64)
+    diff = f32(f32(simulated) - f32(measured))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for i in cells:
+            sg = 1 if mean[i] > 0 else (0 if mean[i] == 0 else -1)   # sign(), calibration.cpp:32-43
+            c = float(f32(f32(1) - f32(f32(f32(sg) * diff) / total)))
+            if c > 1.5:
+                c = 1.5
+            if c < 0.5:
+                c = 0.5
+            corr[i] = c
+    return corr, mean
 
 
 def station_corr_factor(st, gamma, measured, simulated):

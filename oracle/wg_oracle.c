@@ -71,6 +71,7 @@ struct wgo_model {
   double *red_ll, *red_lw, *red_gl, *red_gw, *red_res, *red_river;
   double *river_frac_next, *river_frac_change, *fswb_next, *laf, *laf_prev, *laf_next, *k_release;
   double *river_inflow, *discharge;
+  double *pot_acc;         /* G_potCellRunoff (routing.h:286), km3 since the last annual init */
   int32_t started;
   /* stations */
   int32_t n_st;
@@ -230,7 +231,7 @@ wgo_model *wgo_create(int32_t n, const wg_static_fields *st) {
   ALLOC_D(m->red_river);
   ALLOC_D(m->river_frac_next); ALLOC_D(m->river_frac_change); ALLOC_D(m->fswb_next); ALLOC_D(m->laf);
   ALLOC_D(m->laf_prev); ALLOC_D(m->laf_next); ALLOC_D(m->k_release);
-  ALLOC_D(m->river_inflow); ALLOC_D(m->discharge);
+  ALLOC_D(m->river_inflow); ALLOC_D(m->discharge); ALLOC_D(m->pot_acc);
 
   /* lai.cpp:86-89: lai_factor_a = 0.1*decid, lai_factor_b = (1-decid)*evergreen (float) */
   for (int i = 0; i < WG_NLCT; i++) {
@@ -352,6 +353,7 @@ int wgo_annual_init(wgo_model *m, const wg_static_fields *st, const int32_t *out
    * (routing.cpp:360-403) */
   m->st = st;
   mean_demand(m, st);
+  for (int32_t i = 0; i < n; i++) m->pot_acc[i] = 0.;                /* routing.cpp:1009 */
   for (int32_t i = 0; i < n; i++) glo_res_prev[i] = st->glo_res[i];  /* routing.cpp:5479-5483 at year end */
   return 0;
 }
@@ -366,7 +368,7 @@ void wgo_destroy(wgo_model *m) {
                   &m->glo_wetl, &m->res, &m->river, &m->red_ll, &m->red_lw, &m->red_gl, &m->red_gw,
                   &m->red_res, &m->red_river, &m->river_frac_next, &m->river_frac_change,
                   &m->fswb_next, &m->laf, &m->laf_prev, &m->laf_next, &m->k_release,
-                  &m->river_inflow, &m->discharge, &m->st_sum};
+                  &m->river_inflow, &m->discharge, &m->st_sum, &m->pot_acc};
   for (size_t i = 0; i < sizeof(d) / sizeof(d[0]); i++) free(*d[i]);
   free(m->smax); free(m->lai_max); free(m->contcell); free(m->routing_cell);
   free(m->lai_days); free(m->lai_status); free(m->st_cell); free(m->st_slot);
@@ -811,6 +813,16 @@ static double new_river_velocity(double slope, double bottom, double rough, doub
     return v;
 }
 
+/* potCellRunoff's open-water term, routing.cpp:2047-2051: G_lakeBalance
+ * (daily.cpp:773) in mm over the global lake, reservoir and wetland area. */
+static double pot_open_water(const wgo_model *m, int32_t i) {
+  const wg_static_fields *st = m->st;
+  const double lakeBalance = (m->ow_prec[i] - m->ow_pet[i]) * m->cfa[i];
+  return ((lakeBalance / 1000000.0) *
+          (((double)st->lake_area[i] + (double)st->reservoir_area[i]) +
+           ((st->cell_area[i] / 100.0) * (st->loc_lake[i] + st->loc_wetland[i] + st->glo_wetland[i]))));
+}
+
 /* One cell of the serial routing loop, routing.cpp:1838-4517 (parity profile). */
 static void route_cell(wgo_model *m, int32_t i, int day, int month) {
   const wg_static_fields *st = m->st;
@@ -863,6 +875,8 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
     localGWRunoffIntoRiver = (1. - fswb_catchment) * gw_runoff;
     double localGWRunoff = fswb_catchment * gw_runoff;
     localRunoff = (fswb_catchment * dailyLocalSurfaceRunoff) + localGWRunoff;
+    /* routing.cpp:2045-2053 */
+    m->pot_acc[i] += localRunoff + localRunoffIntoRiver + localGWRunoffIntoRiver + pot_open_water(m, i);
   }
   if (ldd < 0) {
     /* routing.cpp:2112-2159 inland sinks */
@@ -881,6 +895,8 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
     else
       dailyLocalSurfaceRunoff = m->surf_runoff[i] * cellArea / 1000000. * m->laf[i] / 100.;
     localRunoff = dailyLocalSurfaceRunoff + gw_runoff;
+    /* routing.cpp:2180-2187 */
+    m->pot_acc[i] += localRunoff + pot_open_water(m, i);
   }
 
   /* routing.cpp:2308-2313 */
@@ -1136,6 +1152,8 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
       if (fabs(m->gw[i]) <= MIN_STOR_VOL) m->gw[i] = 0.;
     }
     localGWRunoffIntoRiver = gw_runoff;
+    /* routing.cpp:3372-3379 */
+    m->pot_acc[i] += localRunoff + localRunoffIntoRiver + localGWRunoffIntoRiver + pot_open_water(m, i);
   }
 
   /* routing.cpp:3402-3413 */
@@ -1391,6 +1409,14 @@ void wgo_station_discharge(const wgo_model *m, double *sum) {
 
 void wgo_reset_station_discharge(wgo_model *m) {
   for (int32_t b = 0; b < m->n_st; b++) m->st_sum[b] = 0.;
+}
+
+void wgo_pot_cell_runoff(const wgo_model *m, double *pot) {
+  for (int32_t i = 0; i < m->n; i++) pot[i] = m->pot_acc[i];
+}
+
+void wgo_reset_pot_cell_runoff(wgo_model *m) {
+  for (int32_t i = 0; i < m->n; i++) m->pot_acc[i] = 0.;
 }
 
 /* ------------------------------------------------------------------ */

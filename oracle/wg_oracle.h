@@ -81,6 +81,10 @@ int wgo_get_day_outputs(const wgo_model *m, wg_day_outputs *out);
 void wgo_set_stations(wgo_model *m, int32_t n_st, const int32_t *cells);
 void wgo_station_discharge(const wgo_model *m, double *sum);
 void wgo_reset_station_discharge(wgo_model *m);
+/* G_potCellRunoff (routing.cpp:1962-1970, 2045-2053, 2180-2187, 3372-3379), km3
+ * summed since the model was made, the last reset or the last annual init */
+void wgo_pot_cell_runoff(const wgo_model *m, double *pot);
+void wgo_reset_pot_cell_runoff(wgo_model *m);
 /* Timing options of the CPU baseline: the reference's O(N x stations) station
  * search instead of the index map, and the O(N) riparian scan per lake/reservoir
  * cell of aggrNUsGloLakResOpt = 1.  Neither changes a result. */

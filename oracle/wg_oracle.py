@@ -62,6 +62,10 @@ def load(variant="wg"):
     lib.wgo_set_reference_scans.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.wgo_set_reference_scans.restype = None
     lib.wgo_reset_station_discharge.restype = None
+    lib.wgo_pot_cell_runoff.argtypes = [C.c_void_p, P_d]
+    lib.wgo_pot_cell_runoff.restype = None
+    lib.wgo_reset_pot_cell_runoff.argtypes = [C.c_void_p]
+    lib.wgo_reset_pot_cell_runoff.restype = None
     lib.wgo_perturb_forcing.argtypes = [C.c_uint64, i32, i32, C.c_double, C.c_double, P_f, P_f, P_f, P_f]
     lib.wgo_perturb_forcing.restype = None
     lib.wgo_month_of_day.argtypes = [C.c_int]
@@ -341,3 +345,11 @@ class Oracle:
 
     def reset_station_discharge(self):
         self._lib.wgo_reset_station_discharge(self._m)
+
+    def pot_cell_runoff(self):
+        out = np.zeros(self.n)
+        self._lib.wgo_pot_cell_runoff(self._m, _abi.ptr(out, _abi.P_d))
+        return out
+
+    def reset_pot_cell_runoff(self):
+        self._lib.wgo_reset_pot_cell_runoff(self._m)

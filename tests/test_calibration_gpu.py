@@ -18,20 +18,25 @@ class OracleRunner:
         self.forcing_years = forcing_years
         self.initial = self.o.get_state()
 
-    def run(self, gamma, cfa, n_years):
+    def run(self, gamma, cfa, n_years, cell_cfa=None, pot=False):
         b = self.basins
         from watergap2_amd import PARAM_INDEX
-        cell_cfa = b.params[PARAM_INDEX["P_CFA"]] if cfa is None else b.per_basin(cfa)
+        if cell_cfa is None:
+            cell_cfa = b.params[PARAM_INDEX["P_CFA"]] if cfa is None else b.per_basin(cfa)
         self.o.set_params(gamma=b.per_basin(gamma), cfa=cell_cfa)
         self.o.set_state(self.initial)
         self.o.reset_station_discharge()
+        self.o.reset_pot_cell_runoff()
         out = np.zeros((b.n_basins, n_years))
+        pots = np.zeros((n_years, b.n), np.float32)
         for y in range(n_years):
             k = y % self.forcing_years
             self.o.run(self.f[365 * k:365 * (k + 1)], 1)
             out[:, y] = self.o.station_discharge()
             self.o.reset_station_discharge()
-        return out
+            pots[y] = self.o.pot_cell_runoff().astype(np.float32)
+            self.o.reset_pot_cell_runoff()
+        return (out, pots) if pot else out
 
 
 def test_candidates_match_oracle(wg, oracle):
@@ -117,3 +122,39 @@ def test_nested_stations_search(wg, oracle):
     assert (res["status"] >= 1).all()
     assert all(within_1pc(final[b], measured[b]) for b in range(nb)), res
     assert res["tier"].max() >= 1  # the outlet waits for the stations above it
+
+
+def test_cell_correction_grid_after_a_search_at_the_limit(wg, oracle):
+    """Targets three times the discharge at the lowest gamma: the search stops at
+    the lower limit with a cell correction factor (status 3), and the correction
+    grid (createCorrectionGrid) is built from the final run's annual potential cell
+    runoff.  The GPU's annual potential runoff equals the oracle's, the grid equals
+    the float restatement on the oracle's grids, and running with the grid moves
+    the discharge towards the targets."""
+    from watergap2_amd.calibration import BasinRunner, correction_grids, make_basins, plain_search
+    print("case: correction grid, 3 basins, 3 years", flush=True)
+    bs = make_basins(3, seed=21, min_cells=30, max_cells=150)
+    years = 3
+    r = BasinRunner(bs, n_members=1, forcing_years=3)
+    low = r.run(np.full((1, bs.n_basins), 0.1), n_years=years)[0]
+    measured = (3.0 * low).astype(np.float32)
+    res = plain_search(r, measured, n_years=years)
+    assert (res["status"] == 3).all(), res["status"]
+    assert np.all(res["gamma"] == np.float32(0.1))
+    corr, todo = correction_grids(r, res, n_years=years)
+    assert list(todo) == [0, 1, 2]
+    q, pot = r.run(res["gamma"].astype(np.float64)[None, :], n_years=years, pot=True)
+    ref = OracleRunner(oracle, bs, forcing_years=3)
+    qo, pot_o = ref.run(res["gamma"].astype(np.float64), None, years, pot=True)
+    assert np.array_equal(q[0], qo)
+    assert np.array_equal(pot[0], pot_o)
+    expect = r.base_cfa.astype(np.float64).copy()
+    basin_no = np.repeat(np.arange(1, bs.n_basins + 1), np.diff(bs.offsets))
+    for i in range(bs.n_basins):
+        expect, _ = co.correction_grid(basin_no, i + 1, pot_o, res["sim_avg"][i], res["measured_adapt"][i],
+                                       expect)
+    assert np.array_equal(corr, expect)
+    assert np.all((corr >= 0.5) & (corr <= 1.5)) and np.any(corr > 1.0)
+    q2 = r.run(res["gamma"].astype(np.float64)[None, :], n_years=years, cell_cfa=corr[None, :])[0]
+    assert np.all(q2.sum(axis=1) > q[0].sum(axis=1))
+    assert np.all(q2.sum(axis=1) < measured.sum(axis=1))

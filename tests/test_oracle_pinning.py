@@ -708,6 +708,65 @@ def test_threshold_elevation_rule(wg):
 
 
 # ---------------------------------------------------------------------------
+# potential cell runoff (routing.cpp:1962-1970, 2045-2053, 2180-2187, 3372-3379)
+# ---------------------------------------------------------------------------
+
+def check_pot_cell_runoff(wg, variant="wg"):
+    """The cell's potential runoff is the volume it generates: its surface runoff
+    on the land fraction, plus the groundwater it discharges (recharge on the land
+    fraction minus the storage change, whichever groundwater equation applied), plus
+    the open-water balance (P - PET) * CFA over its lake, reservoir and wetland area.
+    Summed over 40 days this holds in every humid cell, every inland sink and every
+    (semi-)arid cell without surface water bodies (their groundwater recharge from
+    surface water does not show in the day's outputs), with CFA drawn per cell."""
+    w = wg.World(wg.global_spec(seed=2, n_land=3000, n_outlets=30, n_sinks=15))
+    arid = np.flatnonzero((w.aindex == 1) & (w.ldd >= 0))
+    dry = arid[::2]
+    for name in ("loc_lake", "loc_wetland", "glo_wetland", "lake_area", "reservoir_area", "glo_lake",
+                 "glo_res", "stor_cap"):
+        getattr(w, name)[dry] = 0
+    cfa = w.params[PI(wg, "P_CFA")]
+    cfa[:] = np.random.default_rng(0).uniform(0.7, 1.3, w.n)
+    o = wg_oracle.Oracle(w, variant)
+    n_days = 40
+    f = w.forcing(seed=1, day0=0, n_days=n_days)
+    gw0 = o.get_state()["groundwater"].copy()
+    area = w.cell_area
+    ow_area = (w.lake_area + w.reservoir_area) + (area / 100.0) * (w.loc_lake + w.loc_wetland + w.glo_wetland)
+    surf = np.zeros(w.n)
+    recharge = np.zeros(w.n)
+    open_water = np.zeros(w.n)
+    laf_pos = np.ones(w.n, bool)
+    for d in range(n_days):
+        laf = o.get_state()["laf"].copy()
+        o.run(np.ascontiguousarray(f[d:d + 1]), d + 1)
+        out = o.day_outputs()
+        laf_pos &= laf > 0
+        surf += out["surface_runoff"] * area / 1e6 * laf / 100.0
+        recharge += out["gw_recharge"] * area * (laf / 100.0) / 1e6
+        open_water += (out["prec"] - out["open_water_pet"]) * cfa / 1e6 * ow_area
+    pot = o.pot_cell_runoff()
+    expect = surf + recharge - (o.get_state()["groundwater"] - gw0) + open_water
+    scale = np.abs(surf) + np.abs(recharge) + np.abs(open_water)
+    swb = ow_area > 0
+    kinds = {"humid": (w.aindex == 0) & (w.ldd >= 0), "inland sink": w.ldd < 0,
+             "arid without surface water": (w.aindex == 1) & (w.ldd >= 0) & ~swb}
+    for kind, sel in kinds.items():
+        sel = sel & laf_pos
+        assert sel.sum() >= 10, kind
+        err = np.abs(pot[sel] - expect[sel]) / scale[sel]
+        assert err.max() < 1e-12, (kind, float(err.max()))
+    # the open-water term matters in the humid cells, the groundwater term in the arid ones
+    assert np.median(np.abs(open_water[kinds["humid"]]) / scale[kinds["humid"]]) > 0.1
+    sel = kinds["arid without surface water"]
+    assert np.median(np.abs(recharge[sel]) / scale[sel]) > 0.01
+
+
+def test_pot_cell_runoff_is_the_generated_volume(wg):
+    check_pot_cell_runoff(wg)
+
+
+# ---------------------------------------------------------------------------
 # planted mistakes: every check above fails on the mutant it is written against
 # ---------------------------------------------------------------------------
 
@@ -766,6 +825,14 @@ MUTANTS = [
     ("evergreen rule dropped", [("      if (lct <= 2) *status = 0;\n", "\n")], check_lai),
     ("full-cover reset one day short", [("        *days = 30 + init_days;", "        *days = 29 + init_days;")],
      check_lai),
+    ("potential runoff of humid cells without the direct runoff",
+     [("/* routing.cpp:2045-2053 */\n    m->pot_acc[i] += localRunoff + localRunoffIntoRiver +",
+       "/* routing.cpp:2045-2053 */\n    m->pot_acc[i] += localRunoff +")], check_pot_cell_runoff),
+    ("potential runoff of arid cells without the groundwater runoff",
+     [("/* routing.cpp:3372-3379 */\n    m->pot_acc[i] += localRunoff + localRunoffIntoRiver + localGWRunoffIntoRiver +",
+       "/* routing.cpp:3372-3379 */\n    m->pot_acc[i] += localRunoff + localRunoffIntoRiver +")], check_pot_cell_runoff),
+    ("open-water balance without CFA",
+     [("(m->ow_prec[i] - m->ow_pet[i]) * m->cfa[i];", "(m->ow_prec[i] - m->ow_pet[i]);")], check_pot_cell_runoff),
     ("threshold elevation ignored",
      [("          temp_elev = dailyTempC - ((thresh_elev - elev[0]) * T_GRADNT);", "          ;")],
      check_threshold_elevation),

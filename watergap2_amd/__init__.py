@@ -333,6 +333,18 @@ class WaterBalanceModel:
     def reset_station_discharge(self):
         self._c(self._lib.wg_reset_station_discharge(self._h))
 
+    def pot_cell_runoff_output(self, enable=True):
+        """Keep G_potCellRunoff (wg_set_pot_cell_runoff): enabling when it is on
+        restarts the sums at 0."""
+        self._c(self._lib.wg_set_pot_cell_runoff(self._h, 1 if enable else 0))
+
+    def pot_cell_runoff(self, member=0):
+        """G_potCellRunoff of one member (km3 per cell, reference cell order),
+        summed since the output was enabled or the last annual init."""
+        out = np.zeros(self.n)
+        self._c(self._lib.wg_get_pot_cell_runoff(self._h, int(member), _abi.ptr(out, _abi.P_d)))
+        return out
+
     def synchronize(self):
         self._c(self._lib.wg_synchronize(self._h))
 

@@ -184,7 +184,7 @@ class GammaSearch(C.Structure):
                 ("cell_corr_factor_ind", C.c_int32), ("cell_corr_factor", C.c_float),
                 ("gamma_upper_limit", C.c_float), ("gamma_lower_limit", C.c_float),
                 ("last_nash_sutcliffe", C.c_float), ("last_sum_diff", C.c_float),
-                ("measured_avg_adapt", C.c_float)]
+                ("measured_avg_adapt", C.c_float), ("sim_runoff_avg", C.c_float)]
 
 
 class RoutingInputs(C.Structure):
@@ -253,6 +253,8 @@ def _declare(lib):
     lib.wg_libm_eval.argtypes = [i32, i32, P_d, P_d, P_d]
     lib.wg_get_station_discharge.argtypes = [C.c_void_p, P_d, P_d, i32]
     lib.wg_reset_station_discharge.argtypes = [C.c_void_p]
+    lib.wg_set_pot_cell_runoff.argtypes = [C.c_void_p, i32]
+    lib.wg_get_pot_cell_runoff.argtypes = [C.c_void_p, i32, P_d]
     lib.wg_station_daily_window.argtypes = [C.c_void_p, P_i32, P_i32]
     lib.wg_get_day_outputs.argtypes = [C.c_void_p, i32, C.POINTER(DayOutputs)]
     lib.wg_synchronize.argtypes = [C.c_void_p]
@@ -271,6 +273,7 @@ def _declare(lib):
     lib.wg_station_corr_factor.restype = C.c_float
     lib.wg_calib_basins.argtypes = [i32, P_i32, P_i32, i32, P_i32, P_i32, P_i16, P_i32]
     lib.wg_upstream_stations.argtypes = [i32, P_i32, i32, P_i32, i32, P_i32, P_i32, i32]
+    lib.wg_cell_corr_factor_grid.argtypes = [i32, P_i16, i32, i32, P_f, C.c_float, C.c_float, P_f, P_d]
     lib.wg_annual_init.argtypes = [C.c_void_p, C.POINTER(StaticFields), P_i32, i32]
     lib.wg_reservoir_year.argtypes = [i32, i32, P_d, P_d, P_i32, P_i8, P_d, P_d]
     lib.wg_mlp_create.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
@@ -313,5 +316,6 @@ ABI_SYMBOLS = [
     "wg_gamma_search_init", "wg_gamma_start", "wg_find_new_gamma",
     "wg_mlp_create", "wg_mlp_destroy", "wg_mlp_last_error", "wg_mlp_n_params", "wg_mlp_get_params",
     "wg_station_corr_factor", "wg_calib_basins", "wg_upstream_stations", "wg_annual_init", "wg_reservoir_year",
+    "wg_set_pot_cell_runoff", "wg_get_pot_cell_runoff", "wg_cell_corr_factor_grid",
     "wg_mlp_set_params", "wg_mlp_forward", "wg_mlp_forward_noalloc", "wg_mlp_gradient", "wg_mlp_train",
 ]

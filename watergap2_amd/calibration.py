@@ -189,14 +189,19 @@ class BasinRunner:
         self.base_gamma = basins.params[PARAM_INDEX["P_GAMRUN_C"]].copy()
         self.base_cfa = basins.params[PARAM_INDEX["P_CFA"]].copy()
 
-    def run(self, gamma, cfa=None, n_years=30):
-        """gamma, cfa: [n_members][n_basins].  Returns annual discharge
-        [n_members][n_basins][n_years] (km3/yr, float64)."""
+    def run(self, gamma, cfa=None, n_years=30, cell_cfa=None, pot=False):
+        """gamma, cfa: [n_members][n_basins]; cell_cfa: [n_members][N] cell
+        correction factors in place of cfa (a correction grid).  Returns annual
+        discharge [n_members][n_basins][n_years] (km3/yr, float64); with pot, also
+        each year's potential cell runoff [n_members][n_years][N] as the reference
+        writes it (G_POT_CELL_RUNOFF_<year>.UNF0, float)."""
         b = self.basins
         M = self.n_members
         gamma = np.asarray(gamma, dtype=np.float64).reshape(M, b.n_basins)
         g = np.stack([b.per_basin(gamma[m]) for m in range(M)])
-        if cfa is None:
+        if cell_cfa is not None:
+            c = np.asarray(cell_cfa, dtype=np.float64).reshape(M, b.n)
+        elif cfa is None:
             c = np.tile(self.base_cfa, (M, 1))
         else:
             cfa = np.asarray(cfa, dtype=np.float64).reshape(M, b.n_basins)
@@ -204,12 +209,21 @@ class BasinRunner:
         self.model.set_member_params(gamma=g, cfa=c)
         self.model.set_state(self.initial, member=-1)
         self.model.reset_station_discharge()
+        self.model.pot_cell_runoff_output(pot)
         out = np.zeros((M, b.n_basins, n_years))
+        pot_years = np.zeros((M, n_years, b.n), dtype=np.float32) if pot else None
         for y in range(n_years):
             self.model.step_days((y % self.forcing_years) * 365, 1, 365)
             tot, _ = self.model.station_discharge()
             out[:, :, y] = tot
             self.model.reset_station_discharge()
+            if pot:
+                for m in range(M):
+                    pot_years[m, y] = self.model.pot_cell_runoff(m).astype(np.float32)
+                self.model.pot_cell_runoff_output(True)  # the next year starts at 0 (routing.cpp:1009)
+        if pot:
+            self.model.pot_cell_runoff_output(False)
+            return out, pot_years
         return out
 
     def close(self):
@@ -261,8 +275,49 @@ def plain_search(runner, measured, n_years=30, max_iter=40, gamma_from_file=None
                 trajectories[i].append(float(g_new))
     status = np.array([s.calib_status for s in states], dtype=np.int32)
     cfa = np.array([s.cell_corr_factor for s in states], dtype=np.float32)
+    sim_avg = np.array([s.sim_runoff_avg for s in states], dtype=np.float32)
+    meas_adapt = np.array([s.measured_avg_adapt for s in states], dtype=np.float32)
     return {"gamma": final, "status": status, "cfa": cfa, "runs": runs_needed, "model_runs": n_runs,
-            "unfinished": int(active.sum()), "trajectories": trajectories}
+            "unfinished": int(active.sum()), "trajectories": trajectories,
+            "sim_avg": sim_avg, "measured_adapt": meas_adapt}
+
+
+def correction_grid(calib_basin, station_number, annual_pot, simulated, measured, corr):
+    """Cell correction factors of one calibration basin (wg_cell_corr_factor_grid,
+    createCorrectionGrid, calibration.cpp:730-801).  annual_pot: [n_years][N] float;
+    corr: the current factors [N], returned updated for the basin's cells, with
+    the mean potential runoff [N]."""
+    lib = _abi.load_library()
+    basin = np.ascontiguousarray(calib_basin, dtype=np.int16)
+    annual = np.ascontiguousarray(annual_pot, dtype=np.float32)
+    out = np.array(corr, dtype=np.float64)
+    mean = np.zeros(annual.shape[1], np.float32)
+    rc = lib.wg_cell_corr_factor_grid(annual.shape[1], _abi.ptr(basin, _abi.P_i16), int(station_number),
+                                      annual.shape[0], _abi.ptr(annual, _abi.P_f), float(simulated), float(measured),
+                                      _abi.ptr(mean, _abi.P_f), _abi.ptr(out, _abi.P_d))
+    if rc != 0:
+        raise RuntimeError(f"wg_cell_corr_factor_grid failed: {_abi.ERRORS.get(rc, rc)}")
+    return out, mean
+
+
+def correction_grids(runner, result, n_years=30):
+    """The cell correction grid after a plain search: for the basins whose search
+    ended with a cell correction factor (status 3, calibration.cpp:625-628), one
+    run of the final gamma keeps the annual potential cell runoff (the files the
+    reference's last run wrote), and each such basin's cells get
+    createCorrectionGrid's factors.  Returns the per-cell factors [N] (the runner's
+    CFA elsewhere) and the basins corrected."""
+    b = runner.basins
+    corr = runner.base_cfa.astype(np.float64).copy()
+    todo = np.flatnonzero(result["status"] == 3)
+    if len(todo) == 0:
+        return corr, todo
+    _, pot = runner.run(np.asarray(result["gamma"], np.float64)[None, :], n_years=n_years, pot=True)
+    calib_basin = np.repeat(np.arange(1, b.n_basins + 1, dtype=np.int16), np.diff(b.offsets))
+    for i in todo:
+        corr, _ = correction_grid(calib_basin, i + 1, pot[0], result["sim_avg"][i], result["measured_adapt"][i],
+                                  corr)
+    return corr, todo
 
 
 def within_1pc(sim, measured):

@@ -131,6 +131,16 @@ int dev_alloc(wg_handle *h, T **out, size_t count) {
   return WG_OK;
 }
 
+// Frees one buffer that dev_alloc allocated.
+void release_buf(wg_handle *h, void *ptr) {
+  for (auto it = h->bufs.begin(); it != h->bufs.end(); ++it)
+    if (it->p == ptr) {
+      (void)hipFree(it->p);
+      h->bufs.erase(it);
+      return;
+    }
+}
+
 template <class T>
 int upload(wg_handle *h, T **out, const std::vector<T> &host) {
   int rc = dev_alloc(h, out, host.size());
@@ -1170,14 +1180,7 @@ extern "C" int wg_set_stations(wg_handle *h, int32_t n_st, const int32_t *cells)
     if (slot[h->inv[cells[b]]] < 0) slot[h->inv[cells[b]]] = b;
   }
   // the new accumulators first: on failure the handle keeps its stations
-  auto release = [&](void *ptr) {
-    for (auto it = h->bufs.begin(); it != h->bufs.end(); ++it)
-      if (it->p == ptr) {
-        (void)hipFree(it->p);
-        h->bufs.erase(it);
-        return;
-      }
-  };
+  auto release = [&](void *ptr) { release_buf(h, ptr); };
   double *st_sum = nullptr, *st_daily = nullptr;
   int rc = dev_alloc(h, &st_sum, size_t(h->m) * std::max(n_st, 1));
   if (!rc) rc = dev_alloc(h, &st_daily, size_t(h->m) * WG_DAILY_SLOTS * std::max(n_st, 1));
@@ -1254,6 +1257,10 @@ int enqueue_block(wg_handle *h, int days, hipEvent_t *evs) {
                          size_t(h->max_group_cells) * sizeof(double), h->stream, h->S, h->X, h->T,
                          h->d_level_off, h->d_group_first, h->d_clock, k, slot);
     HIP_CHECK(h, hipGetLastError());
+    if (h->X.pot_acc) {
+      hipLaunchKernelGGL(wg_pot_runoff_kernel, dim3((h->n + 255) / 256, M), dim3(256), 0, h->stream, h->S, h->X);
+      HIP_CHECK(h, hipGetLastError());
+    }
     if (h->n_stations > 0 && (slot == WG_DAILY_SLOTS - 1 || k == days - 1)) {
       // fold the daily station series into the running sums, in day order
       const int nsm = h->n_stations * M;
@@ -1662,7 +1669,57 @@ extern "C" int wg_annual_init(wg_handle *h, const wg_static_fields *st, const in
     rc = to_device(h, h->d_wave_items[t], items);
   }
   if (rc) return rc;
+  // the potential cell runoff restarts with the year (routing.cpp:1009)
+  if (h->X.pot_acc) {
+    HIP_CHECK(h, hipMemsetAsync(h->X.pot_acc, 0, size_t(h->m) * n * sizeof(double), h->stream));
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  }
   h->glo_res_prev.assign(st->glo_res, st->glo_res + n);
+  return WG_OK;
+}
+
+extern "C" int wg_set_pot_cell_runoff(wg_handle *h, int32_t enable) {
+  if (!h) return WG_E_ARG;
+  HIP_CHECK(h, hipSetDevice(h->device));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  const size_t mn = size_t(h->m) * h->n;
+  if (enable && h->X.pot_acc) {
+    HIP_CHECK(h, hipMemsetAsync(h->X.pot_acc, 0, mn * sizeof(double), h->stream));
+    HIP_CHECK(h, hipStreamSynchronize(h->stream));
+    return WG_OK;
+  }
+  if (enable) {
+    double *buf[4] = {nullptr, nullptr, nullptr, nullptr};
+    int rc = WG_OK;
+    for (int i = 0; i < 4 && !rc; i++) rc = dev_alloc(h, &buf[i], mn);
+    if (rc) {
+      for (double *b : buf)
+        if (b) release_buf(h, b);
+      return rc;
+    }
+    // the captured days hold the state's pointers
+    clear_graphs(h);
+    h->X.pot_acc = buf[0];
+    h->X.pot_base = buf[1];
+    h->X.pot_ow = buf[2];
+    h->X.pot_lgw = buf[3];
+  } else if (h->X.pot_acc) {
+    clear_graphs(h);
+    for (double *b : {h->X.pot_acc, h->X.pot_base, h->X.pot_ow, h->X.pot_lgw}) release_buf(h, b);
+    h->X.pot_acc = h->X.pot_base = h->X.pot_ow = h->X.pot_lgw = nullptr;
+  }
+  return WG_OK;
+}
+
+extern "C" int wg_get_pot_cell_runoff(wg_handle *h, int32_t member, double *pot) {
+  if (!h || !pot || member < 0 || member >= h->m) return WG_E_ARG;
+  if (!h->X.pot_acc) return fail(h, WG_E_ARG, "the potential cell runoff output is off (wg_set_pot_cell_runoff)");
+  HIP_CHECK(h, hipSetDevice(h->device));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  std::vector<double> buf(h->n);
+  HIP_CHECK(h, hipMemcpy(buf.data(), h->X.pot_acc + size_t(member) * h->n, h->n * sizeof(double),
+                         hipMemcpyDeviceToHost));
+  for (int32_t i = 0; i < h->n; i++) pot[i] = buf[h->inv[i]];
   return WG_OK;
 }
 

@@ -4,8 +4,11 @@
 // function-static state moved into a per-basin struct, so that many basins can be
 // searched side by side.  Arithmetic is in float as in the reference (sums,
 // averages, bracket midpoints); every decision and update is kept.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <set>
+#include <vector>
 
 #include "../../include/watergap2_amd.h"
 
@@ -23,6 +26,10 @@ extern "C" void wg_gamma_search_init(wg_gamma_search *s) {
   s->cell_corr_factor = 1.f;
   s->gamma_upper_limit = 5.f;
   s->gamma_lower_limit = 0.1f;
+  s->last_nash_sutcliffe = 0.f;
+  s->last_sum_diff = 0.f;
+  s->measured_avg_adapt = 0.f;
+  s->sim_runoff_avg = 0.f;
 }
 
 // calibration.cpp:196-205: start value of gamma from the parameter file
@@ -195,13 +202,12 @@ extern "C" float wg_find_new_gamma(wg_gamma_search *s, float gamma_old, int32_t 
   s->last_nash_sutcliffe = NashSutclCoeff;
   s->last_sum_diff = sumOfDifferences;
   s->measured_avg_adapt = measuredRunoffAvgAdapt;
+  s->sim_runoff_avg = simRunoffSum / numberOfYears;  // createCorrectionGrid's argument (calibration.cpp:627)
   return gamma;
 }
 
 // ---- calibration basins and upstream stations ----
 
-#include <set>
-#include <vector>
 
 namespace {
 
@@ -335,6 +341,33 @@ extern "C" int wg_upstream_stations(int32_t n, const int32_t *inflow9, int32_t n
 
 // calibGammaClass::writeCorrFactors (calibration.cpp:656-728): the station correction
 // factor after a search, in float as the reference computes it.
+// calibGammaClass::createCorrectionGrid (calibration.cpp:730-801), in float as the
+// reference computes it: the mean of the annual grids (read back from their float
+// files), the basin's sum of absolute means, and per basin cell
+// 1 - sign(mean) * (simulated - measured) / sum, limited to 0.5 .. 1.5.
+extern "C" int wg_cell_corr_factor_grid(int32_t n_cells, const int16_t *calib_basin, int32_t station_number,
+                                        int32_t n_years, const float *annual_pot, float simulated, float measured,
+                                        float *mean_pot, double *corr) {
+  if (n_cells <= 0 || n_years <= 0 || n_years > 32767 || !calib_basin || !annual_pot || !corr) return WG_E_ARG;
+  const short numberOfYears = short(n_years);
+  std::vector<float> mean(size_t(n_cells), 0.0f);
+  for (int32_t y = 0; y < n_years; y++)
+    for (int32_t i = 0; i < n_cells; i++) mean[i] += annual_pot[size_t(y) * n_cells + i];
+  for (int32_t i = 0; i < n_cells; i++) mean[i] /= numberOfYears;
+  float sumOfAbsValuesForBasin = 0;
+  for (int32_t i = 0; i < n_cells; i++)
+    if (station_number == calib_basin[i]) sumOfAbsValuesForBasin += std::fabs(mean[i]);
+  for (int32_t i = 0; i < n_cells; i++) {
+    if (station_number != calib_basin[i]) continue;
+    const int sg = mean[i] > 0 ? 1 : (mean[i] == 0 ? 0 : -1);  // sign(), calibration.cpp:32-43
+    corr[i] = 1 - ((sg * (simulated - measured)) / sumOfAbsValuesForBasin);
+    if (corr[i] > 1.5) corr[i] = 1.5;
+    if (corr[i] < 0.5) corr[i] = 0.5;
+  }
+  if (mean_pot) std::copy(mean.begin(), mean.end(), mean_pot);
+  return WG_OK;
+}
+
 extern "C" float wg_station_corr_factor(wg_gamma_search *s, float gamma, int32_t n_years, const float *measured,
                                         const float *simulated) {
   if (!s || n_years <= 0 || !measured || !simulated) return -99.f;

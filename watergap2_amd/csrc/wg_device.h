@@ -120,6 +120,13 @@ struct WgState {
   WgD2 *crec;                   // [M][WG_DEV_NREC / 2][N] routing records
   // diagnostics of the last day
   double *d_surf, *d_recharge, *d_ow_pet, *d_aet, *d_prec;
+  // potential cell runoff (routing.cpp:1962-1970, 2045-2053, 2180-2187, 3372-3379),
+  // allocated only when that output is on: the running sum since the last reset,
+  // and the day's terms of each cell that the chain kernels complete
+  double *pot_acc;              // [M][N] sum of the daily potential cell runoff (km3)
+  double *pot_base;             // [M][N] local runoff terms of the day
+  double *pot_ow;               // [M][N] open-water balance term of the day
+  double *pot_lgw;              // [M][N] arid groundwater runoff of the day
   // stations
   double *st_sum;               // [M][n_stations]
   double *st_daily;             // [M][WG_DAILY_SLOTS][n_stations]

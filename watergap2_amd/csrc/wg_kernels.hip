@@ -117,6 +117,7 @@ __device__ inline void store_chain(const WgState &X, int64_t mp, int flags, cons
   }
   if (flags & CFL_GWET) X.glo_wetl[mp] = o.glo_wetl;
   if (flags & CFL_ARID) X.gw[mp] = o.gw;
+  if ((flags & CFL_ARID) && X.pot_lgw) X.pot_lgw[mp] = o.lgw;
   X.river[mp] = o.river;
   X.tv[mp] = o.tv;
 }
@@ -1009,6 +1010,15 @@ __global__ __launch_bounds__(256, kSnowChunk == 5 ? 3 : 1) void wg_cell_kernel(W
     }
   }
 
+  if (X.pot_base) {
+    // terms of the potential cell runoff: the local runoff terms (routing.cpp:2045-2047
+    // humid, 2180 inland sinks; (semi-)arid cells add their groundwater runoff in the
+    // chain, 3372-3373) and the open-water precipitation minus evaporation of
+    // G_lakeBalance (daily.cpp:773), which wg_pot_runoff_kernel completes
+    X.pot_base[mp] = (ldd < 0) ? localRunoff : (arid ? localRunoff + lrir : (localRunoff + lrir) + lgwrir);
+    X.pot_ow[mp] = ow_prec - ow_pet;
+  }
+
   double inflow = localRunoff, outflow = 0.;
   const double k_sw = ci.s_k_sw;
   const double evarexp = ci.s_evarexp;
@@ -1424,6 +1434,7 @@ __device__ __forceinline__ void chain_cell(const WgStatic &S, const WgState &X, 
   store_or_drop(X.k_release + base, n, run && (q.flags & CFL_RES), p, o.k_release);
   store_or_drop(X.glo_wetl + base, n, run && (q.flags & CFL_GWET), p, o.glo_wetl);
   store_or_drop(X.gw + base, n, run && (q.flags & CFL_ARID), p, o.gw);
+  if (X.pot_lgw) store_or_drop(X.pot_lgw + base, n, run && (q.flags & CFL_ARID), p, o.lgw);
   store_or_drop(X.river + base, n, run, p, o.river);
   store_or_drop(X.tv + base, n, run, p, o.tv);
   if (run) s_tv[p - gs] = o.tv;
@@ -1564,7 +1575,7 @@ __device__ __forceinline__ void wave_batch(const WgStatic &S, const WgState &X, 
   }
   const int p = b0 + lane;
   const bool have = p < bend;
-  ChainOut o = {0., 0., 0., 0., 0., 0., 0.};
+  ChainOut o = {0., 0., 0., 0., 0., 0., 0., 0.};
   bool run = false;
   if (b0 < bend) {
     const int last = min(63, bend - 1 - b0);
@@ -1606,6 +1617,7 @@ __device__ __forceinline__ void wave_batch(const WgStatic &S, const WgState &X, 
   }
   store_or_drop(X.glo_wetl + base, n, run && (q.flags & CFL_GWET), p, o.glo_wetl);
   store_or_drop(X.gw + base, n, run && (q.flags & CFL_ARID), p, o.gw);
+  if (X.pot_lgw) store_or_drop(X.pot_lgw + base, n, run && (q.flags & CFL_ARID), p, o.lgw);
   store_or_drop(X.river + base, n, run, p, o.river);
   store_or_drop(X.tv + base, n, run, p, o.tv);
   const double qq = (run && (q.flags & CFL_LDD)) ? o.tv : 0.;
@@ -1844,6 +1856,25 @@ __global__ void wg_station_reduce(WgState X, int32_t n_stations, int32_t n_membe
   }
   for (; d < n_slots; d++) s += q[int64_t(d) * n_stations];
   X.st_sum[i] = s;
+}
+
+// Adds the day's potential cell runoff of every continental cell to its running
+// sum, G_potCellRunoff[n] += potCellRunoff (routing.cpp:1970, 2053, 2187, 3379):
+// the local runoff terms, then the arid groundwater step's runoff where that step
+// ran, then the open-water balance, in the reference's order of addition.
+__global__ void wg_pot_runoff_kernel(WgStatic S, WgState X) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= S.n) return;
+  const int f = S.cflags[p];
+  if (!(f & CFL_CONT)) return;
+  const int64_t mp = int64_t(blockIdx.y) * S.n + p;
+  const double base = X.pot_base[mp];
+  const double lake_balance = X.pot_ow[mp] * X.cfa[mp];  // (dailyPrec - dailyOpenWaterPET) * CFA
+  const double ow = (lake_balance / 1000000.0) *
+                    ((S.lake_area[p] + S.reservoir_area[p]) +
+                     ((S.cell_area[p] / 100.0) * ((S.loc_lake[p] + S.loc_wetland[p]) + S.glo_wetland[p])));
+  const double pot = (f & CFL_ARID) ? (base + X.pot_lgw[mp]) + ow : base + ow;
+  X.pot_acc[mp] += pot;
 }
 
 // Sets the day clock read by the kernels of a captured block of days; launched

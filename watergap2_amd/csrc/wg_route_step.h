@@ -102,8 +102,10 @@ struct ChainRec {
   int flags;
 };
 
+// lgw: G_localGWRunoffIntoRiver after the step (the arid groundwater step's
+// runoff in (semi-)arid cells), a term of the potential cell runoff
 struct ChainOut {
-  double glo_lake, res, k_release, glo_wetl, gw, river, tv;
+  double glo_lake, res, k_release, glo_wetl, gw, river, tv, lgw;
 };
 
 struct RiverOut {
@@ -309,6 +311,7 @@ WG_RS_FN ChainOut chain_step(const ChainRec &c, double inflow_up, const DayInfo 
   const double st = r.st, tv = r.tv;
   o.river = st;
   o.tv = tv;
+  o.lgw = lgwrir;
   return o;
 }
 
@@ -346,6 +349,7 @@ WG_RS_FN ChainOut step_fast(const ChainRec &c, const StepPre &pre, double inflow
   o.res = 0.;
   o.k_release = 0.;
   o.gw = pre.gw;
+  o.lgw = pre.lgwrir;
   double inflow = v[CF_INFLOW];
   inflow += inflow_up;  // routing.cpp:2623
   slow = (c.flags & CFL_RES) != 0;
