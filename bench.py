@@ -401,8 +401,9 @@ def day_loop_leg(wg, world, n_years, warmup_years, profile_days, autotune):
 
 
 def deep_run(wg, args):
-    """The day loop on a global grid of the survey's routing depth: the same
-    67,420-cell mask with 60 ocean outlets, 351 routing levels."""
+    """The day loop on a deep routing network (about 350 levels, as continental
+    river systems at 0.5 degrees reach): the same 67,420-cell mask with 60 ocean
+    outlets, 352 routing levels."""
     world = wg.World(wg.global_spec(seed=1, n_outlets=60))
     leg = day_loop_leg(wg, world, args.deep_years, 1, 30, autotune=not args.no_autotune)
     n = world.n

@@ -97,3 +97,18 @@ def test_search_hands_over_the_simulated_mean(wg):
         assert f32(s.measured_avg_adapt) == st.measured_avg_adapt
         ok = meas > -1
         assert abs(float(s.sim_runoff_avg) - float(sim[ok].mean())) <= 1e-5 * float(sim[ok].mean())
+
+
+def test_bad_arguments_are_refused(wg):
+    lib = wg._abi.load_library()
+    P_f, P_d, P_i16 = wg._abi.P_f, wg._abi.P_d, wg._abi.P_i16
+    basin = np.ones(4, np.int16)
+    annual = np.ones((2, 4), np.float32)
+    corr = np.ones(4)
+    ptr = wg._abi.ptr
+    args = [4, ptr(basin, P_i16), 1, 2, ptr(annual, P_f), 1.0, 1.0, None, ptr(corr, P_d)]
+    assert lib.wg_cell_corr_factor_grid(*args) == 0
+    for k, bad in ((0, 0), (3, 0), (1, None), (4, None), (8, None)):
+        a = list(args)
+        a[k] = bad
+        assert lib.wg_cell_corr_factor_grid(*a) == -1, k  # WG_E_ARG
