@@ -3,6 +3,8 @@
 * Global grid (67,420 cells, seed 1): one year against the oracle on every cell,
   and ten years against the oracle on whole basins cut out of it (basins are
   independent, so the cut-out cells must follow the same trajectories).
+* Deep network (the same mask with 60 ocean outlets, 352 routing levels, the
+  bench's deep leg): 60 days against the oracle on every cell.
 * Ensemble: 256 perturbed members of the global grid for a year; three members
   against oracle runs of their inputs for 30 days, a batched member with the
   perturbation off against the one-member run of its parameters, and the year's
@@ -88,6 +90,25 @@ def test_global_grid_ten_years_on_whole_basins(wg, oracle, global_world):
     assert_close(got, ref, label="year 10:")
     assert g["started"] == 1
     m.close()
+
+
+def test_deep_network_60_days(wg, oracle):
+    w = wg.World(wg.global_spec(seed=1, n_outlets=60))
+    print(f"case: deep network, {w.n} cells, {w.max_level + 1} levels, 60 days against the oracle", flush=True)
+    assert w.max_level + 1 >= 300
+    n_days = 60
+    f = w.forcing(seed=3, day0=0, n_days=n_days)
+    m = wg.WaterBalanceModel(w)
+    m.set_stations(w.basin_outlet)
+    m.load_forcing(f)
+    m.step_days(0, 100, n_days)
+    o = oracle_for(oracle, w)
+    o.run(f, 100)
+    assert_close(m.get_state(), o.get_state(), label="deep:")
+    assert_close(m.day_outputs(), o.day_outputs(), label="deep day:")
+    tot, _ = m.station_discharge()
+    assert_close({"discharge": tot[0]}, {"discharge": o.station_discharge()}, label="deep station:")
+    assert m.schedule_info()["max_depth"] == w.max_level + 1
 
 
 def test_ensemble_256_members_one_year(wg, oracle, global_world):
