@@ -188,11 +188,12 @@ GAMMA_LIMITS = (0.1, 5.0)  # the search range of calibGammaClass (calibration.cp
 
 
 def bracket_proposals(points, target, n=3):
-    """Three gamma values aimed at `target` (log runoff) from verified (gamma, log
-    runoff) points: interpolated in log gamma between the closest points on either
-    side of the target (runoff falls as gamma rises), at fractions w and w +- a
-    quarter of the bracket; with points on one side only, steps beyond the
-    outermost point by factors of 1.5, 2 and 3, within the search range."""
+    """n (up to 5) gamma values aimed at `target` (log runoff) from verified (gamma,
+    log runoff) points: interpolated in log gamma between the closest points on
+    either side of the target (runoff falls as gamma rises), at fractions w, w +- a
+    quarter and w +- an eighth of the bracket; with points on one side only, steps
+    beyond the outermost point by factors of 1.5, 2, 3, 1.25 and 4, within the
+    search range."""
     g = np.array([p_[0] for p_ in points], dtype=np.float64)
     y = np.array([p_[1] for p_ in points], dtype=np.float64)
     lo_g, hi_g = np.log(GAMMA_LIMITS[0]), np.log(GAMMA_LIMITS[1])
@@ -203,19 +204,19 @@ def bracket_proposals(points, target, n=3):
         la, lb = np.log(g[ia]), np.log(g[ib])
         if la < lb:
             w = (y[ia] - target) / max(y[ia] - y[ib], 1e-12)
-            ws = np.clip(np.array([w, w - 0.25, w + 0.25]), 0.02, 0.98)
+            ws = np.clip(np.array([w, w - 0.25, w + 0.25, w - 0.125, w + 0.125]), 0.02, 0.98)
             return np.exp(la + ws * (lb - la))[:n]
     if above.all():              # every point has too much runoff: larger gamma
         base = np.log(g.max())
-        steps = np.log([1.5, 2.0, 3.0])
+        steps = np.log([1.5, 2.0, 3.0, 1.25, 4.0])
     else:                        # too little runoff everywhere: smaller gamma
         base = np.log(g.min())
-        steps = -np.log([1.5, 2.0, 3.0])
+        steps = -np.log([1.5, 2.0, 3.0, 1.25, 4.0])
     return np.exp(np.clip(base + steps, lo_g, hi_g))[:n]
 
 
 def calibrate_with_surrogate(model, runner, feats, measured, spread=0.006, rounds=3, bracket_rounds=2,
-                             first_members=1, first_spread=0.0, log=None):
+                             first_members=1, first_spread=0.0, round_members=3, bracket_members=3, log=None):
     """Surrogate-guided calibration of every basin of the runner's set.
 
     Round 1: the network's proposal for each basin, verified by one model run of
@@ -224,11 +225,11 @@ def calibrate_with_surrogate(model, runner, feats, measured, spread=0.006, round
     that many members of the one run.  Rounds 2..`rounds`, for the basins still missing the 1 % criterion:
     the network's curve corrected through the verified points (shifted by the
     miss after one point; shifted and scaled through the two closest points after
-    two or more), aimed at the target and at the target +- `spread` (log runoff),
-    three proposals per basin verified as three members of one run of just those
-    basins.  Then up to `bracket_rounds` rounds that interpolate between the
-    verified points on either side of the target (runoff falls as gamma rises),
-    three points per basin.  Then the plain search (the reference's findNewGamma)
+    two or more), aimed at the target and at the target +- k `spread` (log runoff,
+    k = 1, 2, ...), `round_members` proposals per basin verified as that many
+    members of one run of just those basins.  Then up to `bracket_rounds` rounds
+    that interpolate between the verified points on either side of the target
+    (runoff falls as gamma rises), `bracket_members` points per basin.  Then the plain search (the reference's findNewGamma)
     for what still misses.
 
     Returns per-basin round (1..rounds + bracket_rounds, or one more = plain
@@ -280,7 +281,9 @@ def calibrate_with_surrogate(model, runner, feats, measured, spread=0.006, round
     rnd[hit] = 1
     if log:
         log(f"surrogate round 1: {int(hit.sum())}/{nb} within 1 %")
-    shifts = (0.0, -spread, spread)
+    shifts = [0.0]
+    for k in range(1, round_members):
+        shifts.append(((k + 1) // 2) * spread * (-1.0 if k % 2 else 1.0))
     for r in range(2, rounds + 1):
         miss = np.flatnonzero(~hit)
         if not len(miss):
@@ -320,7 +323,7 @@ def calibrate_with_surrogate(model, runner, feats, measured, spread=0.006, round
         miss = np.flatnonzero(~hit)
         if not len(miss):
             break
-        props = np.stack([bracket_proposals(points[b], target[b]) for b in miss], axis=1)
+        props = np.stack([bracket_proposals(points[b], target[b], n=bracket_members) for b in miss], axis=1)
         sr = sub_runner(miss, props.shape[0])
         q2 = sr.run(props, n_years=n_years)
         sr.close()
