@@ -281,8 +281,10 @@ def surrogate_run(args):
     log(f"plain search: {res['model_runs']} runs in {plain_s:.1f}s, {plain_ok} within 1 %")
 
     t1 = time.perf_counter()
+    rounds, round_members, bracket_rounds, bracket_members = (int(v) for v in args.sur_rounds.split(","))
     sres = sg.calibrate_with_surrogate(model, runner, feats, measured, first_members=args.sur_first_members,
-                                       first_spread=spread, log=log)
+                                       first_spread=spread, rounds=rounds, round_members=round_members,
+                                       bracket_rounds=bracket_rounds, bracket_members=bracket_members, log=log)
     sur_s = time.perf_counter() - t1
     sur_ok = int(sres["within_1pc"].sum())
     runner.close()
@@ -299,7 +301,7 @@ def surrogate_run(args):
         "config": {"workload": f"{bs.n_basins} unseen synthetic basins (log-uniform 50-5,000 cells), {bs.n} "
                                f"cells, {years} years, targets from a hidden gamma U(0.1, 5)",
                    "basins": bs.n_basins, "cells": bs.n, "years": years,
-                   "first_round_proposals_per_basin": args.sur_first_members},
+                   "first_round_proposals_per_basin": args.sur_first_members, "later_rounds": args.sur_rounds},
         "plain": {"seconds": round(plain_s, 2), "model_runs": res["model_runs"], "within_1pc": plain_ok,
                   "runs_per_basin_mean": round(float(res["runs"].mean()), 2)},
         "surrogate": {"seconds": round(sur_s, 2), "first_round_hits": by_round[0],
@@ -455,6 +457,9 @@ def main():
                     help="trained surrogate (python -m watergap2_amd.train_surrogate)")
     ap.add_argument("--sur-first-members", type=int, default=1,
                     help="proposals per basin verified in the surrogate's first run")
+    ap.add_argument("--sur-rounds", default="3,3,2,3",
+                    help="surrogate's later rounds: correction rounds, proposals per basin in each, "
+                         "bracket rounds, points per basin in each")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch configuration (profiling runs)")
     ap.add_argument("--ensemble-members", type=int, default=256,
