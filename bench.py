@@ -382,8 +382,9 @@ def day_loop_leg(wg, world, n_years, warmup_years, profile_days, autotune):
     model.set_stations(world.basin_outlet)
     model.load_forcing(forcing)
     info = model.schedule_info()
+    info["partitions"] = model.partitions()
     if autotune:
-        tuned, tuned_ms = model.autotune(n_days=5)
+        tuned, tuned_ms = model.autotune(n_days=30)
     else:
         tuned, tuned_ms = (256, 512, 5, 3), 0.0
     for s in range(warmup_years):
@@ -616,7 +617,9 @@ def main():
                    "launch": {"cell_tile": tuned[0], "chain_workgroup": tuned[1], "snow_chunk": tuned[2],
                               "chain_kernel": ["level walk, 1 level ahead", "dataflow walk",
                                                "level walk, 2 levels ahead",
-                                               "wave walk in 64-cell batches"][tuned[3]]}},
+                                               "wave walk in 64-cell batches"][tuned[3]],
+                              "partitions": [{"cells": c, "deepest_group_levels": d}
+                                             for c, d in info["partitions"]]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "deep": deep,

@@ -478,8 +478,9 @@ int wg_last_timing(wg_handle *h, double *ms, int64_t *n_launches);
  * (cell-kernel tile 128 or 256 cells, chain kernel mode 0, 1 or 2 with a
  * workgroup of 64, 128 or 256 threads or mode 3 with 256 or 512, snow chunk 5 or
  * 10 sub-cell pairs)
- * on n_days days of the loaded forcing, each from the same saved state, keeps the
- * fastest and restores the state.  Needs device memory for one copy of the
+ * on n_days days of the loaded forcing, each from the same saved state and run
+ * as wg_step_days runs them (graphs, partitions), keeps the fastest and restores
+ * the state.  Needs device memory for one copy of the
  * handle's buffers.  config receives [cell threads, chain threads, snow chunk,
  * chain mode]; ms_per_day the best day time.  Results are identical under every
  * configuration. */
@@ -497,6 +498,14 @@ int wg_set_launch_config(wg_handle *h, int32_t cell_threads, int32_t chain_threa
 /* Schedule facts: [0] routing groups, [1] deepest group (levels), [2] level
  * slots, [3] cells routed by the cell kernel (no upstream cell). */
 int wg_schedule_info(wg_handle *h, int32_t *info);
+/* No counterpart in the reference (its routing is one serial loop).  The routing
+ * groups of the day loop fall into one or two partitions: with two, the deepest
+ * groups (at least 0.8 times the deepest, at most 30 % of the cells) run their days
+ * as a graph on a stream of their own beside the others', which changes no result.
+ * Returns the number of partitions; cells[k] and depth[k] (may be NULL) receive the
+ * cells and deepest group (levels) of the first max_parts of them.  The environment
+ * variable WG_PARTITIONS=1, read by wg_create, keeps one. */
+int wg_partitions(wg_handle *h, int32_t max_parts, int32_t *cells, int32_t *depth);
 
 /* Diagnostic with no reference counterpart: evaluates the library's FP64 math on
  * the device, fn 0 exp(x), 1 pow(x, y), 2 sqrt(x), 3 log(x), 4 sin(2 pi x),

@@ -292,6 +292,15 @@ class WaterBalanceModel:
         return {"groups": info[0], "max_depth": info[1], "level_slots": info[2],
                 "level0_cells": info[3]}
 
+    def partitions(self):
+        """[(cells, deepest group levels)] of the day loop's partitions (wg_partitions)."""
+        cells = np.zeros(2, np.int32)
+        depth = np.zeros(2, np.int32)
+        n = self._lib.wg_partitions(self._h, 2, _abi.ptr(cells, _abi.P_i32), _abi.ptr(depth, _abi.P_i32))
+        if n < 0:
+            self._c(n)
+        return [(int(cells[k]), int(depth[k])) for k in range(min(n, 2))]
+
     def get_state(self, member=0):
         st = new_state(self.n)
         s = state_struct(st)

@@ -250,6 +250,7 @@ def _declare(lib):
     lib.wg_step_days.argtypes = [C.c_void_p, i32, i32, i32]
     lib.wg_step_days_profiled.argtypes = [C.c_void_p, i32, i32, i32]
     lib.wg_schedule_info.argtypes = [C.c_void_p, P_i32]
+    lib.wg_partitions.argtypes = [C.c_void_p, i32, P_i32, P_i32]
     lib.wg_libm_eval.argtypes = [i32, i32, P_d, P_d, P_d]
     lib.wg_get_station_discharge.argtypes = [C.c_void_p, P_d, P_d, i32]
     lib.wg_reset_station_discharge.argtypes = [C.c_void_p]
@@ -316,6 +317,6 @@ ABI_SYMBOLS = [
     "wg_gamma_search_init", "wg_gamma_start", "wg_find_new_gamma",
     "wg_mlp_create", "wg_mlp_destroy", "wg_mlp_last_error", "wg_mlp_n_params", "wg_mlp_get_params",
     "wg_station_corr_factor", "wg_calib_basins", "wg_upstream_stations", "wg_annual_init", "wg_reservoir_year",
-    "wg_set_pot_cell_runoff", "wg_get_pot_cell_runoff", "wg_cell_corr_factor_grid",
+    "wg_set_pot_cell_runoff", "wg_get_pot_cell_runoff", "wg_cell_corr_factor_grid", "wg_partitions",
     "wg_mlp_set_params", "wg_mlp_forward", "wg_mlp_forward_noalloc", "wg_mlp_gradient", "wg_mlp_train",
 ]

@@ -34,6 +34,21 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+// One partition of the routing groups.  With two partitions the deepest groups
+// form the first; each partition's days run as a graph of its own on a stream of
+// its own, so that the long walk of the deep groups overlaps the other groups'
+// days (basins are independent, so the results do not change).  A partition's
+// cells, tiles and chain blocks are contiguous ranges of the whole grid's.
+struct WgPart {
+  std::vector<int32_t> bin_start, bin_end;  // device cell range of each XCD bin
+  int32_t cell_begin = 0, cell_end = 0;     // device cell range
+  int32_t tile_off = 0, n_tiles = 0;        // range of the whole grid's tile table
+  int32_t group_block_off = 0, n_group_blocks = 0;  // range of its chain block table
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  std::map<int, hipGraphExec_t> graphs;     // keyed by block length
+};
+
 }  // namespace
 
 struct wg_handle {
@@ -78,8 +93,9 @@ struct wg_handle {
                                   // 2 level walk, loads two levels ahead; 3 wave walk in batches
   int32_t max_group_cells = 0;
   int n_xcd = 8;
-  std::vector<int32_t> bin_start, bin_end;   // device cell range of each XCD bin
-  std::vector<int32_t> tile_start, tile_end, block_group;
+  std::vector<WgPart> parts;                 // at least one; two run concurrently
+  hipEvent_t ev_fork = nullptr;
+  std::vector<int32_t> tile_start, tile_end, block_group;  // whole grid: the partitions' tables in turn
   int32_t tile_capacity = 0;      // entries of the device tile tables
   std::vector<WgChainLink> clink;
   std::vector<int32_t> wave_wide;    // wide levels of each group (host copy of d_wave_wide)
@@ -303,21 +319,29 @@ std::vector<double> reservoir_prov_rel(const wg_static_fields *st, int32_t n) {
   return out;
 }
 
-// Cell-kernel tiles of block_threads_cell cells: block 8j + x covers tile j of XCD bin x.
+// Cell-kernel tiles of block_threads_cell cells: within each partition, block
+// 8j + x covers tile j of XCD bin x; the whole grid's table holds the partitions'
+// tables in turn (each a multiple of 8 blocks long, so placement is kept).
 void build_tiles(wg_handle *h) {
   const int nx = h->n_xcd, T = h->block_threads_cell;
-  int32_t max_tiles = 1;
-  for (int x = 0; x < nx; x++) max_tiles = std::max(max_tiles, (h->bin_end[x] - h->bin_start[x] + T - 1) / T);
-  h->tile_start.assign(size_t(nx) * max_tiles, -1);
-  h->tile_end.assign(size_t(nx) * max_tiles, -1);
-  for (int32_t j = 0; j < max_tiles; j++)
-    for (int x = 0; x < nx; x++) {
-      const int32_t a = h->bin_start[x] + T * j;
-      if (a < h->bin_end[x]) {
-        h->tile_start[size_t(j) * nx + x] = a;
-        h->tile_end[size_t(j) * nx + x] = std::min(a + T, h->bin_end[x]);
+  h->tile_start.clear();
+  h->tile_end.clear();
+  for (WgPart &P : h->parts) {
+    int32_t max_tiles = 1;
+    for (int x = 0; x < nx; x++) max_tiles = std::max(max_tiles, (P.bin_end[x] - P.bin_start[x] + T - 1) / T);
+    P.tile_off = int32_t(h->tile_start.size());
+    P.n_tiles = nx * max_tiles;
+    h->tile_start.resize(size_t(P.tile_off) + P.n_tiles, -1);
+    h->tile_end.resize(size_t(P.tile_off) + P.n_tiles, -1);
+    for (int32_t j = 0; j < max_tiles; j++)
+      for (int x = 0; x < nx; x++) {
+        const int32_t a = P.bin_start[x] + T * j;
+        if (a < P.bin_end[x]) {
+          h->tile_start[size_t(P.tile_off) + size_t(j) * nx + x] = a;
+          h->tile_end[size_t(P.tile_off) + size_t(j) * nx + x] = std::min(a + T, P.bin_end[x]);
+        }
       }
-    }
+  }
 }
 
 // Builds the level schedule and the device order.
@@ -394,23 +418,52 @@ int build_schedule(wg_handle *h, const wg_static_fields *st, int group_cells) {
   for (int32_t gg = 0; gg < n_groups0; gg++) gsort[gg] = gg;
   std::stable_sort(gsort.begin(), gsort.end(), [&](int32_t a, int32_t b) { return g_cells[a] > g_cells[b]; });
   const int nx = h->n_xcd;
-  std::vector<int64_t> bin_load(nx, 0);
-  std::vector<std::vector<int32_t>> bin_groups(nx);
-  for (int32_t gg : gsort) {
-    int best = 0;
-    for (int x = 1; x < nx; x++)
-      if (bin_load[x] < bin_load[best]) best = x;
-    bin_groups[best].push_back(gg);
-    bin_load[best] += g_cells[gg];
-  }
-  std::vector<int32_t> new_gid(n_groups0), bin_of_new;
-  std::vector<int32_t> depth_new;
-  for (int x = 0; x < nx; x++)
-    for (int32_t gg : bin_groups[x]) {
-      new_gid[gg] = int32_t(depth_new.size());
-      depth_new.push_back(g_depth[gg]);
-      bin_of_new.push_back(x);
+  // Partitions: the groups at least 0.8 times as deep as the deepest form the
+  // first when they hold at most 30 % of the cells (and not every group); their
+  // days then run beside the others' (WG_PARTITIONS=1 keeps one partition).
+  std::vector<int32_t> part_of(n_groups0, 0);
+  int n_parts = 1;
+  {
+    int32_t max_d = 0;
+    for (int32_t d : g_depth) max_d = std::max(max_d, d);
+    const int32_t deep = (4 * max_d + 4) / 5;
+    int64_t deep_cells = 0;
+    int32_t n_deep = 0;
+    for (int32_t gg = 0; gg < n_groups0; gg++)
+      if (g_depth[gg] >= deep) {
+        n_deep++;
+        deep_cells += g_cells[gg];
+      }
+    const char *env = getenv("WG_PARTITIONS");
+    const bool allow = !(env && atoi(env) == 1);
+    if (allow && n_deep > 0 && n_deep < n_groups0 && deep_cells * 10 <= int64_t(n) * 3) {
+      n_parts = 2;
+      for (int32_t gg = 0; gg < n_groups0; gg++) part_of[gg] = g_depth[gg] >= deep ? 0 : 1;
     }
+  }
+  // XCD bins of each partition, [part][xcd], each partition balanced on its own
+  // (balancing the two together measured slower: it crowds the other partition's
+  // cells onto six XCDs while the deep groups' cells sit on two)
+  std::vector<std::vector<std::vector<int32_t>>> bin_groups(n_parts, std::vector<std::vector<int32_t>>(nx));
+  for (int pp = 0; pp < n_parts; pp++) {
+    std::vector<int64_t> bin_load(nx, 0);
+    for (int32_t gg : gsort) {
+      if (part_of[gg] != pp) continue;
+      int best = 0;
+      for (int x = 1; x < nx; x++)
+        if (bin_load[x] < bin_load[best]) best = x;
+      bin_groups[pp][best].push_back(gg);
+      bin_load[best] += g_cells[gg];
+    }
+  }
+  std::vector<int32_t> new_gid(n_groups0);
+  std::vector<int32_t> depth_new;
+  for (int pp = 0; pp < n_parts; pp++)
+    for (int x = 0; x < nx; x++)
+      for (int32_t gg : bin_groups[pp][x]) {
+        new_gid[gg] = int32_t(depth_new.size());
+        depth_new.push_back(g_depth[gg]);
+      }
   h->n_groups = n_groups0;
   h->max_depth = 0;
   for (int32_t d : depth_new) h->max_depth = std::max(h->max_depth, d);
@@ -445,27 +498,35 @@ int build_schedule(wg_handle *h, const wg_static_fields *st, int group_cells) {
     acc += slot_count[s];
   }
   h->level_off[n_slots] = acc;
-  // block tables: block 8j + x runs on XCD x
-  std::vector<int32_t> bin_start(nx, 0), bin_end(nx, 0);
+  // block tables: within each partition, block 8j + x runs on XCD x
+  h->parts.assign(n_parts, WgPart{});
+  h->block_group.clear();
   {
     int32_t gg = 0;
-    for (int x = 0; x < nx; x++) {
-      bin_start[x] = h->level_off[h->group_first_level[gg]];
-      gg += int32_t(bin_groups[x].size());
-      bin_end[x] = h->level_off[h->group_first_level[gg]];
+    for (int pp = 0; pp < n_parts; pp++) {
+      WgPart &P = h->parts[pp];
+      P.bin_start.assign(nx, 0);
+      P.bin_end.assign(nx, 0);
+      const int32_t g0 = gg;
+      int32_t max_groups = 1;
+      for (int x = 0; x < nx; x++) {
+        P.bin_start[x] = h->level_off[h->group_first_level[gg]];
+        gg += int32_t(bin_groups[pp][x].size());
+        P.bin_end[x] = h->level_off[h->group_first_level[gg]];
+        max_groups = std::max(max_groups, int32_t(bin_groups[pp][x].size()));
+      }
+      P.cell_begin = h->level_off[h->group_first_level[g0]];
+      P.cell_end = h->level_off[h->group_first_level[gg]];
+      P.group_block_off = int32_t(h->block_group.size());
+      P.n_group_blocks = nx * max_groups;
+      h->block_group.resize(size_t(P.group_block_off) + P.n_group_blocks, -1);
+      int32_t g1 = g0;
+      for (int x = 0; x < nx; x++)
+        for (size_t j = 0; j < bin_groups[pp][x].size(); j++)
+          h->block_group[size_t(P.group_block_off) + j * nx + x] = g1++;
     }
   }
-  h->bin_start = bin_start;
-  h->bin_end = bin_end;
-  int32_t max_groups = 1;
-  for (int x = 0; x < nx; x++) max_groups = std::max(max_groups, int32_t(bin_groups[x].size()));
   build_tiles(h);
-  h->block_group.assign(size_t(nx) * max_groups, -1);
-  {
-    int32_t gg = 0;
-    for (int x = 0; x < nx; x++)
-      for (size_t j = 0; j < bin_groups[x].size(); j++) h->block_group[j * nx + x] = gg++;
-  }
   return WG_OK;
 }
 
@@ -887,9 +948,12 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
     // tile tables sized for the smallest cell block the tuner tries (64 threads)
     int32_t *dt;
     {
-      int32_t cap = 1;
-      for (int x = 0; x < H->n_xcd; x++) cap = std::max(cap, (H->bin_end[x] - H->bin_start[x] + 63) / 64);
-      H->tile_capacity = cap * H->n_xcd;
+      H->tile_capacity = 0;
+      for (const WgPart &P : H->parts) {
+        int32_t cap = 1;
+        for (int x = 0; x < H->n_xcd; x++) cap = std::max(cap, (P.bin_end[x] - P.bin_start[x] + 63) / 64);
+        H->tile_capacity += cap * H->n_xcd;
+      }
     }
     r_ = dev_alloc(H, &dt, size_t(H->tile_capacity));
     if (r_) { *out = h.release(); return r_; }
@@ -973,6 +1037,14 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
   if (rc) { *out = h.release(); return rc; }
   e = hipEventCreate(&H->ev0);
   if (e == hipSuccess) e = hipEventCreate(&H->ev1);
+  if (e == hipSuccess && H->parts.size() > 1) {
+    // the partitions' streams (default priority) and their fork/join events
+    e = hipEventCreateWithFlags(&H->ev_fork, hipEventDisableTiming);
+    for (WgPart &P : H->parts) {
+      if (e == hipSuccess) e = hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&P.done, hipEventDisableTiming);
+    }
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(H->stream);
   if (e != hipSuccess) {
     *out = h.release();
@@ -986,6 +1058,13 @@ extern "C" int wg_destroy(wg_handle *h) {
   if (!h) return WG_E_ARG;
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (WgPart &P : h->parts) {
+    if (P.stream) (void)hipStreamSynchronize(P.stream);
+    for (auto &kv : P.graphs) (void)hipGraphExecDestroy(kv.second);
+    if (P.done) (void)hipEventDestroy(P.done);
+    if (P.stream) (void)hipStreamDestroy(P.stream);
+  }
+  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
   for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   for (auto &b : h->bufs) (void)hipFree(b.p);
   if (h->d_forcing) (void)hipFree(h->d_forcing);
@@ -1127,6 +1206,10 @@ namespace {
 void clear_graphs(wg_handle *h) {
   for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   h->graphs.clear();
+  for (WgPart &P : h->parts) {
+    for (auto &kv : P.graphs) (void)hipGraphExecDestroy(kv.second);
+    P.graphs.clear();
+  }
 }
 }  // namespace
 
@@ -1212,88 +1295,109 @@ extern "C" int wg_set_stations(wg_handle *h, int32_t n_st, const int32_t *cells)
 namespace {
 
 // Launches the cell kernel with the handle's snow chunk.
-void launch_cell(wg_handle *h, dim3 grid, int k, int do_post, int do_vertical, int write_diag, int slot) {
+void launch_cell(wg_handle *h, const WgStatic &S, hipStream_t stream, dim3 grid, int k, int do_post,
+                 int do_vertical, int write_diag, int slot) {
   if (h->snow_chunk == 10)
-    hipLaunchKernelGGL(wg_cell_kernel<10>, grid, dim3(h->block_threads_cell), 0, h->stream, h->S, h->X, h->T,
+    hipLaunchKernelGGL(wg_cell_kernel<10>, grid, dim3(h->block_threads_cell), 0, stream, S, h->X, h->T,
                        h->d_forcing, h->n_forcing_days, h->d_clock, k, do_post, do_vertical, write_diag, slot);
   else
-    hipLaunchKernelGGL(wg_cell_kernel<5>, grid, dim3(h->block_threads_cell), 0, h->stream, h->S, h->X, h->T,
+    hipLaunchKernelGGL(wg_cell_kernel<5>, grid, dim3(h->block_threads_cell), 0, stream, S, h->X, h->T,
                        h->d_forcing, h->n_forcing_days, h->d_clock, k, do_post, do_vertical, write_diag, slot);
 }
 
-// Enqueues one block of `days` days on the stream (captured or eager).
-int enqueue_block(wg_handle *h, int days, hipEvent_t *evs) {
+// Enqueues one block of `days` days (captured or eager): of the whole grid on the
+// handle's stream (part -1), or of one partition on its stream, without the
+// station sums, which the caller folds once every partition is done.
+int enqueue_block(wg_handle *h, int days, hipEvent_t *evs, int part = -1) {
   const int32_t M = h->m;
-  const dim3 cgrid(uint32_t(h->tile_start.size()), M);
-  const dim3 chgrid(uint32_t(h->block_group.size()), M);
+  WgStatic S = h->S;
+  hipStream_t stream = h->stream;
+  uint32_t n_tiles = uint32_t(h->tile_start.size()), n_blocks = uint32_t(h->block_group.size());
+  int32_t c0 = 0, c1 = h->n;
+  if (part >= 0) {
+    const WgPart &P = h->parts[part];
+    S.tile_start += P.tile_off;
+    S.tile_end += P.tile_off;
+    S.block_group += P.group_block_off;
+    stream = P.stream;
+    n_tiles = uint32_t(P.n_tiles);
+    n_blocks = uint32_t(P.n_group_blocks);
+    c0 = P.cell_begin;
+    c1 = P.cell_end;
+  }
+  const dim3 cgrid(n_tiles, M);
+  const dim3 chgrid(n_blocks, M);
   int ev = 0;
   for (int k = 0; k < days; k++) {
     const int slot = k % WG_DAILY_SLOTS;
-    if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], h->stream));
-    launch_cell(h, cgrid, k, k > 0 ? 1 : 0, 1, k == days - 1 ? 1 : 0, slot);
+    if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], stream));
+    launch_cell(h, S, stream, cgrid, k, k > 0 ? 1 : 0, 1, k == days - 1 ? 1 : 0, slot);
     HIP_CHECK(h, hipGetLastError());
-    if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], h->stream));
+    if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], stream));
     if (h->chain_mode == 3) {
       // the wave walk runs 8 wavefronts with a 512-thread configuration, else 4
       if (h->block_threads_chain >= 512)
         hipLaunchKernelGGL(wg_chain_wave_kernel<8>, chgrid, dim3(512), wave_lds_bytes(h),
-                           h->stream, h->S, h->X, h->T, h->d_level_off, h->d_group_first, h->d_wave_wide,
+                           stream, S, h->X, h->T, h->d_level_off, h->d_group_first, h->d_wave_wide,
                            h->d_wave_item_off[1], h->d_wave_items[1], h->d_clock, k, slot);
       else
         hipLaunchKernelGGL(wg_chain_wave_kernel<4>, chgrid, dim3(256), wave_lds_bytes(h),
-                           h->stream, h->S, h->X, h->T, h->d_level_off, h->d_group_first, h->d_wave_wide,
+                           stream, S, h->X, h->T, h->d_level_off, h->d_group_first, h->d_wave_wide,
                            h->d_wave_item_off[0], h->d_wave_items[0], h->d_clock, k, slot);
     }
     else if (h->chain_mode == 1)
       hipLaunchKernelGGL(wg_chain_flow_kernel, chgrid, dim3(h->block_threads_chain),
-                         size_t(h->max_group_cells) * sizeof(double), h->stream, h->S, h->X, h->T,
+                         size_t(h->max_group_cells) * sizeof(double), stream, S, h->X, h->T,
                          h->d_level_off, h->d_group_first, h->d_clock, k, slot);
     else if (h->chain_mode == 2)
       hipLaunchKernelGGL(wg_chain_kernel<2>, chgrid, dim3(h->block_threads_chain),
-                         size_t(h->max_group_cells) * sizeof(double), h->stream, h->S, h->X, h->T,
+                         size_t(h->max_group_cells) * sizeof(double), stream, S, h->X, h->T,
                          h->d_level_off, h->d_group_first, h->d_clock, k, slot);
     else
       hipLaunchKernelGGL(wg_chain_kernel<1>, chgrid, dim3(h->block_threads_chain),
-                         size_t(h->max_group_cells) * sizeof(double), h->stream, h->S, h->X, h->T,
+                         size_t(h->max_group_cells) * sizeof(double), stream, S, h->X, h->T,
                          h->d_level_off, h->d_group_first, h->d_clock, k, slot);
     HIP_CHECK(h, hipGetLastError());
     if (h->X.pot_acc) {
-      hipLaunchKernelGGL(wg_pot_runoff_kernel, dim3((h->n + 255) / 256, M), dim3(256), 0, h->stream, h->S, h->X);
+      hipLaunchKernelGGL(wg_pot_runoff_kernel, dim3((c1 - c0 + 255) / 256, M), dim3(256), 0, stream, S, h->X, c0,
+                         c1);
       HIP_CHECK(h, hipGetLastError());
     }
-    if (h->n_stations > 0 && (slot == WG_DAILY_SLOTS - 1 || k == days - 1)) {
+    if (part < 0 && h->n_stations > 0 && (slot == WG_DAILY_SLOTS - 1 || k == days - 1)) {
       // fold the daily station series into the running sums, in day order
       const int nsm = h->n_stations * M;
-      hipLaunchKernelGGL(wg_station_reduce, dim3((nsm + 255) / 256), dim3(256), 0, h->stream, h->X,
+      hipLaunchKernelGGL(wg_station_reduce, dim3((nsm + 255) / 256), dim3(256), 0, stream, h->X,
                          h->n_stations, M, 0, slot + 1);
       HIP_CHECK(h, hipGetLastError());
     }
   }
-  if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], h->stream));
+  if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], stream));
   // closing pass: finish the last day so that the state is complete
-  launch_cell(h, cgrid, 0, 1, 0, 0, 0);
+  launch_cell(h, S, stream, cgrid, 0, 1, 0, 0, 0);
   HIP_CHECK(h, hipGetLastError());
-  if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], h->stream));
+  if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], stream));
   return WG_OK;
 }
 
-int get_graph(wg_handle *h, int days, hipGraphExec_t *out) {
-  auto it = h->graphs.find(days);
-  if (it != h->graphs.end()) {
+int get_graph(wg_handle *h, int days, hipGraphExec_t *out, int part = -1) {
+  std::map<int, hipGraphExec_t> &graphs = part < 0 ? h->graphs : h->parts[part].graphs;
+  hipStream_t stream = part < 0 ? h->stream : h->parts[part].stream;
+  auto it = graphs.find(days);
+  if (it != graphs.end()) {
     *out = it->second;
     return WG_OK;
   }
   hipGraph_t graph;
-  HIP_CHECK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_block(h, days, nullptr);
-  hipError_t e = hipStreamEndCapture(h->stream, &graph);
+  HIP_CHECK(h, hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_block(h, days, nullptr, part);
+  hipError_t e = hipStreamEndCapture(stream, &graph);
   if (rc) return rc;
   if (e != hipSuccess) return fail(h, WG_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
   hipGraphExec_t exec;
   e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return fail(h, WG_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-  h->graphs[days] = exec;
+  graphs[days] = exec;
   *out = exec;
   return WG_OK;
 }
@@ -1337,14 +1441,38 @@ extern "C" int wg_step_days(wg_handle *h, int32_t forcing_day0, int32_t first_do
   int doy = first_doy;
   int remaining = n_days;
   int64_t launches = 0;
+  const int n_parts = int(h->parts.size());
   while (remaining > 0) {
     const int k = std::min(block, remaining);
-    hipGraphExec_t g;
-    rc = get_graph(h, k, &g);
-    if (rc) return rc;
     hipLaunchKernelGGL(wg_clock_set, dim3(1), dim3(64), 0, h->stream, h->d_clock, day, doy, h->started ? 1 : 0);
     HIP_CHECK(h, hipGetLastError());
-    HIP_CHECK(h, hipGraphLaunch(g, h->stream));
+    if (n_parts < 2) {
+      hipGraphExec_t g;
+      rc = get_graph(h, k, &g);
+      if (rc) return rc;
+      HIP_CHECK(h, hipGraphLaunch(g, h->stream));
+    } else {
+      // every partition's block on its own stream after the clock, then the
+      // station sums once all are done (the sums of a block are folded in day order)
+      for (int pp = 0; pp < n_parts; pp++) {
+        hipGraphExec_t g;
+        rc = get_graph(h, k, &g, pp);
+        if (rc) return rc;
+      }
+      HIP_CHECK(h, hipEventRecord(h->ev_fork, h->stream));
+      for (WgPart &P : h->parts) {
+        HIP_CHECK(h, hipStreamWaitEvent(P.stream, h->ev_fork, 0));
+        HIP_CHECK(h, hipGraphLaunch(P.graphs[k], P.stream));
+        HIP_CHECK(h, hipEventRecord(P.done, P.stream));
+        HIP_CHECK(h, hipStreamWaitEvent(h->stream, P.done, 0));
+      }
+      if (h->n_stations > 0) {
+        const int nsm = h->n_stations * h->m;
+        hipLaunchKernelGGL(wg_station_reduce, dim3((nsm + 255) / 256), dim3(256), 0, h->stream, h->X,
+                           h->n_stations, h->m, 0, (k - 1) % WG_DAILY_SLOTS + 1);
+        HIP_CHECK(h, hipGetLastError());
+      }
+    }
     h->started = true;
     day += k;
     doy = (doy - 1 + k) % 365 + 1;
@@ -1420,8 +1548,9 @@ int apply_config(wg_handle *h, int cell_threads, int chain_threads, int chunk, i
 // Tries cell-kernel tiles of 128 and 256 cells, the chain modes with their
 // workgroups (level walks 64, 128 or 256 threads, wave walk 4 or 8 wavefronts)
 // and snow chunks of 5 and 10 sub-cell pairs on n_days days of the loaded
-// forcing, each from the same saved state, and keeps the fastest.  Results do not
-// depend on the choice (every configuration is bit-identical).
+// forcing, each from the same saved state and run as wg_step_days runs them, and
+// keeps the fastest.  Results do not depend on the choice (every configuration is
+// bit-identical).
 extern "C" int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double *ms_per_day) {
   if (!h || n_days <= 0) return WG_E_ARG;
   if (n_days > h->n_forcing_days) return fail(h, WG_E_ARG, "autotune needs %d loaded forcing days", n_days);
@@ -1462,11 +1591,13 @@ extern "C" int wg_autotune(wg_handle *h, int32_t n_days, int32_t *config, double
         if (md == 3 ? (ch != 256 && ch != 512) : ch > 256) continue;
         rc = apply_config(h, c, ch, k, md);
         double ms = 0;
-        for (int rep = 0; rep < 2 && !rc; rep++) {  // the first pass loads the kernels
+        // the first pass loads the kernels and captures the graphs of the path that
+        // wg_step_days runs (with its partitions); the second is timed
+        for (int rep = 0; rep < 2 && !rc; rep++) {
           if (copy_all(false) != hipSuccess) rc = fail(h, WG_E_HIP, "autotune restore");
           if (!rc) rc = upload_tiles(h);
           h->started = started;
-          if (!rc) rc = wg_step_days_profiled(h, 0, 1, n_days);
+          if (!rc) rc = wg_step_days(h, 0, 1, n_days);
           ms = h->last_ms[2];
         }
         if (!rc && ms < best) {
@@ -1782,6 +1913,26 @@ extern "C" int wg_schedule_info(wg_handle *h, int32_t *info) {
   }
   info[3] = lvl0;
   return WG_OK;
+}
+
+extern "C" int wg_partitions(wg_handle *h, int32_t max_parts, int32_t *cells, int32_t *depth) {
+  if (!h || max_parts < 0 || (max_parts > 0 && !cells)) return WG_E_ARG;
+  const int32_t np = int32_t(h->parts.size());
+  for (int32_t pp = 0; pp < np && pp < max_parts; pp++) {
+    const WgPart &P = h->parts[pp];
+    cells[pp] = P.cell_end - P.cell_begin;
+    if (depth) {
+      // the deepest group whose cells lie in the partition
+      int32_t d = 0;
+      for (int32_t g = 0; g < h->n_groups; g++) {
+        const int32_t a = h->level_off[h->group_first_level[g]];
+        if (a >= P.cell_begin && a < P.cell_end)
+          d = std::max(d, h->group_first_level[g + 1] - h->group_first_level[g]);
+      }
+      depth[pp] = d;
+    }
+  }
+  return np;
 }
 
 extern "C" int wg_libm_eval(int32_t fn, int32_t n, const double *x, const double *y, double *out) {

@@ -1858,13 +1858,14 @@ __global__ void wg_station_reduce(WgState X, int32_t n_stations, int32_t n_membe
   X.st_sum[i] = s;
 }
 
-// Adds the day's potential cell runoff of every continental cell to its running
-// sum, G_potCellRunoff[n] += potCellRunoff (routing.cpp:1970, 2053, 2187, 3379):
-// the local runoff terms, then the arid groundwater step's runoff where that step
-// ran, then the open-water balance, in the reference's order of addition.
-__global__ void wg_pot_runoff_kernel(WgStatic S, WgState X) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= S.n) return;
+// Adds the day's potential cell runoff of every continental cell of [p0, p1)
+// (device order) to its running sum, G_potCellRunoff[n] += potCellRunoff
+// (routing.cpp:1970, 2053, 2187, 3379): the local runoff terms, then the arid
+// groundwater step's runoff where that step ran, then the open-water balance, in
+// the reference's order of addition.
+__global__ void wg_pot_runoff_kernel(WgStatic S, WgState X, int32_t p0, int32_t p1) {
+  const int p = p0 + int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (p >= p1) return;
   const int f = S.cflags[p];
   if (!(f & CFL_CONT)) return;
   const int64_t mp = int64_t(blockIdx.y) * S.n + p;
