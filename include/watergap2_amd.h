@@ -499,9 +499,10 @@ int wg_set_launch_config(wg_handle *h, int32_t cell_threads, int32_t chain_threa
  * slots, [3] cells routed by the cell kernel (no upstream cell). */
 int wg_schedule_info(wg_handle *h, int32_t *info);
 /* No counterpart in the reference (its routing is one serial loop).  The routing
- * groups of the day loop fall into one or two partitions: with two, the deepest
- * groups (at least 0.8 times the deepest, at most 30 % of the cells) run their days
- * as a graph on a stream of their own beside the others', which changes no result.
+ * groups of the day loop fall into one or two partitions: with two (handles of at
+ * most 8 members), the deepest groups (at least 0.8 times the deepest, at most 30 %
+ * of the cells) run their days as a graph on a stream of their own beside the
+ * others', which changes no result.
  * Returns the number of partitions; cells[k] and depth[k] (may be NULL) receive the
  * cells and deepest group (levels) of the first max_parts of them.  The environment
  * variable WG_PARTITIONS=1, read by wg_create, keeps one. */

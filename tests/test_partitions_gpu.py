@@ -63,3 +63,12 @@ def test_partitions_change_no_result(wg, global_world):
         same(one, run(wg, w, f, True, cfg), f"launch {cfg}")
     print("case: two partitions, eager path", flush=True)
     same(one, run(wg, w, f, True, profiled=True), "eager")
+
+
+def test_many_members_keep_one_partition(wg, global_world):
+    """A day of many members is bandwidth-bound: no partitions above 8 members."""
+    print("case: partitions by member count", flush=True)
+    for M, parts in ((1, 2), (8, 2), (9, 1), (64, 1)):
+        m = wg.WaterBalanceModel(global_world, n_members=M)
+        assert len(m.partitions()) == parts, M
+        m.close()

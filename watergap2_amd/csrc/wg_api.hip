@@ -420,7 +420,12 @@ int build_schedule(wg_handle *h, const wg_static_fields *st, int group_cells) {
   const int nx = h->n_xcd;
   // Partitions: the groups at least 0.8 times as deep as the deepest form the
   // first when they hold at most 30 % of the cells (and not every group); their
-  // days then run beside the others' (WG_PARTITIONS=1 keeps one partition).
+  // days then run beside the others' (WG_PARTITIONS=1 keeps one partition).  Only
+  // for up to 8 members: a day of few members is latency-bound, and overlapping
+  // the deep groups' walk pays; with many it is bandwidth-bound, and the deep
+  // groups' cells, on few XCDs, would load those more than the rest (measured
+  // 24 % slower at 256 members).
+  constexpr int32_t kPartitionMaxMembers = 8;
   std::vector<int32_t> part_of(n_groups0, 0);
   int n_parts = 1;
   {
@@ -436,7 +441,8 @@ int build_schedule(wg_handle *h, const wg_static_fields *st, int group_cells) {
       }
     const char *env = getenv("WG_PARTITIONS");
     const bool allow = !(env && atoi(env) == 1);
-    if (allow && n_deep > 0 && n_deep < n_groups0 && deep_cells * 10 <= int64_t(n) * 3) {
+    if (allow && h->m <= kPartitionMaxMembers && n_deep > 0 && n_deep < n_groups0 &&
+        deep_cells * 10 <= int64_t(n) * 3) {
       n_parts = 2;
       for (int32_t gg = 0; gg < n_groups0; gg++) part_of[gg] = g_depth[gg] >= deep ? 0 : 1;
     }
