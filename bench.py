@@ -6,16 +6,19 @@ FP64 storages, 100 elevation sub-cells per cell.  One "step" is one simulated
 year (365 days: calcNewDay for every cell, routing in upstream-to-downstream
 order, land-area update) run as hipGraph-captured days; the default times 10
 steps = 10 years.  Forcing for all timed days is resident in HBM before the
-timed region starts.
+timed region starts.  The library runs the deepest routing groups as a second
+partition beside the others (config.launch.partitions; WG_PARTITIONS=1 in the
+environment keeps one, for comparison).
 
 Besides the JSON line's value, the run reports
   * roofline: algorithmic bytes per launch of the dominant kernel divided by its
     average duration from HIP events on the model's stream (eager launches of
-    the same kernels over a profiling window after the timed region), against
+    the same kernels over the whole grid, one after the other, in a profiling
+    window after the timed region), against
     the 8 TB/s HBM peak, with the HBM traffic per launch when a rocprof counter
     summary is present under profiles/;
-  * deep: the same measurement on a global grid of 351 routing levels (60 ocean
-    outlets), where the routing chain is four times as deep;
+  * deep: the same measurement on a global grid of 352 routing levels (60 ocean
+    outlets), where the routing chain is three times as deep;
   * ensemble: 256 perturbed members of the global grid for one year;
   * calibration: basins calibrated per hour at the 1 % criterion with the
     reference's gamma search (120 synthetic basins, 30 years), and 64 candidates
