@@ -64,3 +64,34 @@ def test_restart_through_files_continues_the_run(wg, oracle, tmp_path):
     a.run(f[60:], 61)
     b.run(f[60:], 61)
     assert_close(b.get_state(), a.get_state())
+
+
+def test_malformed_state_files_are_refused(wg, tmp_path):
+    """A missing file, a truncated table (fewer rows than cells), a row with fewer
+    numbers than columns, an id outside 1..N and a non-numeric row all give
+    WG_E_INPUT; header lines (starting with a letter) are skipped, and a table
+    ends at its first empty line, as the reference's reader does."""
+    import pytest
+    v = np.arange(30, dtype=np.float64).reshape(3, 10) / 7.0
+    p = tmp_path / "s.txt"
+    statefiles.write_wghm_state(str(p), v)
+    good = p.read_text().split("\n")
+
+    def refused(text):
+        q = tmp_path / "bad.txt"
+        q.write_text(text)
+        with pytest.raises(wg.WaterGapError) as e:
+            statefiles.read_wghm_state(str(q), 3)
+        assert e.value.code == -5
+
+    with pytest.raises(wg.WaterGapError):
+        statefiles.read_wghm_state(str(tmp_path / "missing.txt"), 3)
+    refused("\n".join(good[:4]) + "\n")                              # two of three rows
+    refused("\n".join(good[:4] + [" ".join(good[4].split()[:6])]) + "\n")  # short row
+    refused("\n".join(good[:4] + ["     9" + good[4][6:]]) + "\n")  # id 9 of 3 cells
+    refused("\n".join(good[:4] + ["     3 1.0 x"]) + "\n")              # not a number
+    refused("\n".join(good[:3] + [""] + good[3:]))                     # empty line ends the table early
+    # extra header lines anywhere before the rows are skipped
+    q = tmp_path / "hdr.txt"
+    q.write_text("\n".join(good[:2] + ["Comment line"] + good[2:]))
+    assert np.array_equal(statefiles.read_wghm_state(str(q), 3), v)
