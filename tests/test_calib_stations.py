@@ -122,3 +122,35 @@ def test_station_correction_factor(wg):
         a = station_corr_factor(s, g, m, sm)
         b = co.station_corr_factor(st, g, m, sm)
         assert a == float(b) and s.calib_status == st.calib_status
+
+
+def test_station_edge_cases(wg):
+    """An empty station list gives no basin and an all-zero grid; a station list of
+    only skipped cells likewise; a basin list with no station upstream gives empty
+    lists; bad arguments are refused (WG_E_ARG)."""
+    from watergap2_amd.calibration import calib_basins, upstream_stations
+    from watergap2_amd import _abi
+    grid, used, overlap, nb = calib_basins(INFLOW, OUTFLOW, [])
+    assert nb == 0 and not grid.any()
+    grid, used, overlap, nb = calib_basins(INFLOW, OUTFLOW, [1, 0, -3])
+    assert nb == 0 and not grid.any() and used.tolist() == [0, 0, 0] and overlap.tolist() == [-1, -1, -1]
+    for all_st in (0, 1):
+        assert upstream_stations(INFLOW, [4], all_st) == [[]]
+        assert upstream_stations(INFLOW, [], all_st) == []
+    lib = _abi.load_library()
+    inflow = np.ascontiguousarray(INFLOW, dtype=np.int32)
+    outflow = np.ascontiguousarray(OUTFLOW, dtype=np.int32)
+    grid = np.zeros(6, np.int16)
+    P32, P16 = _abi.P_i32, _abi.P_i16
+    assert lib.wg_calib_basins(0, _abi.ptr(inflow, P32), _abi.ptr(outflow, P32), 0, None, None,
+                               _abi.ptr(grid, P16), None) == -1
+    assert lib.wg_calib_basins(6, _abi.ptr(inflow, P32), _abi.ptr(outflow, P32), 2, None, None,
+                               _abi.ptr(grid, P16), None) == -1
+    assert lib.wg_calib_basins(6, _abi.ptr(inflow, P32), _abi.ptr(outflow, P32), 40000,
+                               _abi.ptr(np.zeros(40000, np.int32), P32), None, _abi.ptr(grid, P16), None) == -1
+    # more list entries than room for them (station 3 lies above station 4)
+    lst = np.zeros(1, np.int32)
+    cells = np.array([4, 3], np.int32)
+    rc = lib.wg_upstream_stations(6, _abi.ptr(inflow, P32), 2, _abi.ptr(cells, P32), 1,
+                                  _abi.ptr(np.zeros(3, np.int32), P32), _abi.ptr(lst, P32), 0)
+    assert rc == -1
