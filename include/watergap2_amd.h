@@ -134,7 +134,7 @@ typedef struct wg_options_profile {
   int32_t pet_opt;              /* 0: Priestley-Taylor */
   int32_t use_kc;               /* 0 */
   int32_t river_velo_opt;       /* 2: storage-based Manning velocity */
-  int32_t subtract_use;         /* 0 */
+  int32_t subtract_use;         /* 0; 2 or 3 (water use, wg_set_water_use) with aggr_nus_glolakres_opt 0 */
   int32_t res_opt;              /* 1 */
   int32_t statcorr_opt;         /* 0 */
   int32_t arid_area_opt;        /* 1 */
@@ -454,6 +454,57 @@ int wg_reset_station_discharge(wg_handle *h);
 int wg_set_pot_cell_runoff(wg_handle *h, int32_t enable);
 /* The sums since the last restart for one member, pot[N] in reference cell order. */
 int wg_get_pot_cell_runoff(wg_handle *h, int32_t member, double *pot);
+
+/*
+ * Water use (subtract_use 2 or 3 with use_alloc = 0 and aggrNUsGloLakResOpt = 0):
+ * net abstractions from surface water (NUs) and groundwater (NUg) taken from the
+ * cell's own storages, in the reference's order: the groundwater abstraction,
+ * adapted by updateNetAbstractionGW (routing.cpp:5503-5572), in the groundwater
+ * step (1929-2137, 3325-3330); the surface-water use from the global lake and
+ * reservoir (2678-3171), the river (3466-3512) and last the local lake
+ * (3590-3622); the unsatisfied use kept for the next day (3898-3919) and the
+ * year-end processing (annualWaterUsePostProcessing, 5246-5306) after day 365.
+ * The irrigation reservoirs' release takes the month's use of the cell and its
+ * downstream cells (2970-2996).
+ *
+ * Per-cell state of one member, [WG_USE_NFIELDS][N] in reference cell order
+ * (km3; routing.h:204-318).
+ */
+enum {
+  WG_USE_TOTAL_UNSAT = 0,       /* G_totalUnsatisfiedUse */
+  WG_USE_PREV_TOTAL_UNSAT,      /* G_PrevTotalUnstatisfiedUse */
+  WG_USE_DAILY_REMAINING,       /* G_dailyRemainingUse */
+  WG_USE_UNSAT_IRRIG,           /* G_unsatisfiedNAsFromIrrig */
+  WG_USE_UNSAT_OTHER,           /* G_unsatisfiedNAsFromOtherSectors */
+  WG_USE_REDUCED_RETURN,        /* G_reducedReturnFlow */
+  WG_USE_WITHDRAWAL_IRRIG,      /* G_withdrawalIrrigFromSwb (km3/day) */
+  WG_USE_CONSUMPTIVE_IRRIG,     /* G_consumptiveUseIrrigFromSwb (km3/day) */
+  WG_USE_UNSAT_PREV_YEAR,       /* G_UnsatisfiedUsePrevYear */
+  WG_USE_UNSAT_IRRIG_PREV_YEAR, /* G_unsatisfiedNAsFromIrrigPrevYear */
+  WG_USE_UNSAT_OTHER_PREV_YEAR, /* G_unsatisfiedNAsFromOtherSectorsPrevYear */
+  WG_USE_REDUCED_RETURN_PREV_YEAR, /* G_reducedReturnFlowPrevYear */
+  WG_USE_NFIELDS
+};
+
+/* Turns water use on for a handle made with subtract_use 2 or 3 (WG_E_STATE
+ * otherwise).  Tables of n_years model years, [n_years][N][12] in reference cell
+ * order as the .12.UNF0 files hold them, in m3 per month: net use from surface
+ * water (G_NETUSE_SW_m3_<year> or G_ACTUAL_NAS_<year>) and from groundwater,
+ * irrigation withdrawal and consumptive use from surface water
+ * (G_IRRIG_WITHDRAWAL_USE_SW_m3_<year>, G_IRRIG_CONS_USE_SW_m3_<year>).  Year y
+ * covers the forcing days year0_day + 365 y .. + 364; wg_step_days refuses a day
+ * outside the tables.  frac_return_gw [N] is G_FRACTRETURNGW_IRRIG.  delayed is
+ * delayedUseSatisfaction (0 or 1).  Daily values as dailyNUInit
+ * (routing.cpp:909-972): times M_NETABSSW / M_NETABSGW, over 1e9 x days in the
+ * month.  st: the statics of wg_create or of the last wg_annual_init (the
+ * irrigation reservoirs' release walks their downstream cells); after a
+ * wg_annual_init the tables must be set again (wg_step_days: WG_E_STATE).  The
+ * state starts at 0; wg_set_water_use_state sets it. */
+int wg_set_water_use(wg_handle *h, const wg_static_fields *st, int32_t n_years, int32_t year0_day, const double *nus_m3,
+                     const double *nug_m3, const double *withdrawal_irrig_m3, const double *consumptive_irrig_m3,
+                     const double *frac_return_gw, int32_t delayed);
+int wg_get_water_use_state(wg_handle *h, int32_t member, double *state);
+int wg_set_water_use_state(wg_handle *h, int32_t member, const double *state);
 
 /* Diagnostics of the last simulated day for one member. */
 int wg_get_day_outputs(wg_handle *h, int32_t member, wg_day_outputs *out);

@@ -72,6 +72,11 @@ struct wgo_model {
   double *river_frac_next, *river_frac_change, *fswb_next, *laf, *laf_prev, *laf_next, *k_release;
   double *river_inflow, *discharge;
   double *pot_acc;         /* G_potCellRunoff (routing.h:286), km3 since the last annual init */
+  /* water use (subtract_use 2/3, use_alloc 0, aggrNUsGloLakResOpt 0) */
+  int use_on, use_delayed;
+  double *use_nus, *use_nug, *use_wu, *use_cu; /* [N][12] daily values of the year (dailyNUInit) */
+  double *use_frgi;        /* G_fractreturngw_irrig */
+  double *use;             /* [WG_USE_NFIELDS][N] state */
   int32_t started;
   /* stations */
   int32_t n_st;
@@ -368,7 +373,8 @@ void wgo_destroy(wgo_model *m) {
                   &m->glo_wetl, &m->res, &m->river, &m->red_ll, &m->red_lw, &m->red_gl, &m->red_gw,
                   &m->red_res, &m->red_river, &m->river_frac_next, &m->river_frac_change,
                   &m->fswb_next, &m->laf, &m->laf_prev, &m->laf_next, &m->k_release,
-                  &m->river_inflow, &m->discharge, &m->st_sum, &m->pot_acc};
+                  &m->river_inflow, &m->discharge, &m->st_sum, &m->pot_acc, &m->use_nus, &m->use_nug,
+                  &m->use_wu, &m->use_cu, &m->use_frgi, &m->use};
   for (size_t i = 0; i < sizeof(d) / sizeof(d[0]); i++) free(*d[i]);
   free(m->smax); free(m->lai_max); free(m->contcell); free(m->routing_cell);
   free(m->lai_days); free(m->lai_status); free(m->st_cell); free(m->st_slot);
@@ -824,6 +830,107 @@ static double pot_open_water(const wgo_model *m, int32_t i) {
 }
 
 /* One cell of the serial routing loop, routing.cpp:1838-4517 (parity profile). */
+#define USE(f) (m->use[(size_t)(f) * (size_t)m->n + (size_t)i])
+
+/* updateNetAbstractionGW, routing.cpp:5503-5572: the day's net abstraction from
+ * groundwater, with the return flows of irrigation from surface water reduced
+ * when yesterday's surface-water use went unsatisfied (G_dailyRemainingUse > 0)
+ * or given back when a delayed use was satisfied (< 0). */
+static double update_net_abstraction_gw(wgo_model *m, int32_t i, double nug) {
+  double NAgnew = 0., returnflowChange = 0., eff = 0., frgi = 0., factor = 0., WUsiNew = 0.;
+  double dailyRemainingUseFromIrrig = 0., reintroducedRatio = 0.;
+  const double wu = USE(WG_USE_WITHDRAWAL_IRRIG), cu = USE(WG_USE_CONSUMPTIVE_IRRIG);
+  if (USE(WG_USE_DAILY_REMAINING) > 1.e-12) {
+    if (wu > 0.) {
+      eff = cu / wu;
+      frgi = m->use_frgi[i];
+      factor = 1 - (1 - frgi) * (1 - eff);
+      WUsiNew = 1 / factor * (wu * factor - USE(WG_USE_DAILY_REMAINING));
+      if (WUsiNew < 0.) {
+        WUsiNew = 0.;
+        USE(WG_USE_UNSAT_IRRIG) += wu * factor;
+        USE(WG_USE_UNSAT_OTHER) += USE(WG_USE_DAILY_REMAINING) - (wu * factor);
+      } else {
+        USE(WG_USE_UNSAT_IRRIG) += USE(WG_USE_DAILY_REMAINING);
+      }
+      returnflowChange = (frgi * (1 - eff) * (WUsiNew - wu));
+      USE(WG_USE_REDUCED_RETURN) += returnflowChange;
+      NAgnew = nug - returnflowChange;
+      USE(WG_USE_DAILY_REMAINING) = 0.;
+      return NAgnew;
+    }
+    USE(WG_USE_UNSAT_OTHER) += USE(WG_USE_DAILY_REMAINING);
+    return nug;
+  } else if (USE(WG_USE_DAILY_REMAINING) < -1.e-12) {
+    dailyRemainingUseFromIrrig = USE(WG_USE_DAILY_REMAINING) + USE(WG_USE_UNSAT_OTHER);
+    if (dailyRemainingUseFromIrrig < 0.) {
+      USE(WG_USE_UNSAT_OTHER) = 0.;
+    } else {
+      USE(WG_USE_UNSAT_OTHER) += USE(WG_USE_DAILY_REMAINING);
+      USE(WG_USE_DAILY_REMAINING) = 0.;
+      return nug;
+    }
+    if (USE(WG_USE_UNSAT_IRRIG) == 0.) {
+      USE(WG_USE_DAILY_REMAINING) = 0.;
+      return nug;
+    }
+    reintroducedRatio = (dailyRemainingUseFromIrrig / USE(WG_USE_UNSAT_IRRIG));
+    if (reintroducedRatio < -1.) {
+      reintroducedRatio = -1.;
+      dailyRemainingUseFromIrrig = USE(WG_USE_UNSAT_IRRIG) * -1.;
+    }
+    returnflowChange = (reintroducedRatio * USE(WG_USE_REDUCED_RETURN));
+    USE(WG_USE_UNSAT_IRRIG) += dailyRemainingUseFromIrrig;
+    USE(WG_USE_REDUCED_RETURN) += returnflowChange;
+    NAgnew = nug - returnflowChange;
+    USE(WG_USE_DAILY_REMAINING) = 0.;
+    return NAgnew;
+  }
+  return nug;
+}
+
+/* G_dailydailyNUg of the day (routing.cpp:7440-7441), adapted when yesterday left
+ * a remaining use (routing.cpp:1990-1994 and the other groundwater branches) */
+static double net_abstraction_gw(wgo_model *m, int32_t i, int month) {
+  double nug = m->use_nug[(size_t)i * 12 + month];
+  if (USE(WG_USE_DAILY_REMAINING) != 0.) nug = update_net_abstraction_gw(m, i, nug);
+  return nug;
+}
+
+/* routing.cpp:3590-3622, 3898-3919: what the river left of the use is taken from
+ * the local lake (down to -maxStorage), whose area reduction factor is then
+ * recomputed; the unsatisfied use is kept for the next day; the month's
+ * irrigation withdrawal and consumptive use are kept for updateNetAbstractionGW. */
+static void use_day_end(wgo_model *m, int32_t i, double remainingUse, int month) {
+  const wg_static_fields *st = m->st;
+  if ((st->loc_lake[i] > 0.) && (remainingUse > 0.)) {
+    const double evarexp = st->params[(size_t)WG_M_EVAREDEX * m->n + i];
+    const double maxStorage = ((st->loc_lake[i]) / 100.) * st->cell_area[i] * m->lake_depth[i];
+    if (m->loc_lake[i] > (-1.) * maxStorage) {
+      if (remainingUse < (maxStorage + m->loc_lake[i])) {
+        m->loc_lake[i] -= remainingUse;
+        remainingUse = 0.;
+      } else {
+        remainingUse -= (maxStorage + m->loc_lake[i]);
+        m->loc_lake[i] = (-1.) * maxStorage;
+      }
+    }
+    double r = 1. - POW(fabs(m->loc_lake[i] - maxStorage) / (2. * maxStorage), (evarexp * EVAPO_RED_EXP));
+    if (r < 0.) r = 0.;
+    if (r > 1.) r = 1.;
+    m->red_ll[i] = r;
+  }
+  if (m->use_delayed == 1) {
+    USE(WG_USE_DAILY_REMAINING) = remainingUse - USE(WG_USE_PREV_TOTAL_UNSAT);
+    USE(WG_USE_TOTAL_UNSAT) = remainingUse;
+  } else {
+    USE(WG_USE_TOTAL_UNSAT) += remainingUse;
+    USE(WG_USE_DAILY_REMAINING) = remainingUse;
+  }
+  USE(WG_USE_WITHDRAWAL_IRRIG) = m->use_wu[(size_t)i * 12 + month];
+  USE(WG_USE_CONSUMPTIVE_IRRIG) = m->use_cu[(size_t)i * 12 + month];
+}
+
 static void route_cell(wgo_model *m, int32_t i, int day, int month) {
   const wg_static_fields *st = m->st;
   const int32_t n = m->n;
@@ -864,6 +971,8 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
     /* routing.cpp:1968-2039 humid cells */
     gw_prev = m->gw[i];
     netGWin = (double)m->gw_recharge[i] * cellArea * ((double)m->laf[i] / 100.) / 1000000.;
+    /* routing.cpp:1990-1996 */
+    if (m->use_on) netGWin -= net_abstraction_gw(m, i, month);
     m->gw[i] = gw_prev * EXP(-1. * k_gw) + (1. / k_gw) * netGWin * (1. - EXP(-1. * k_gw));
     if (fabs(m->gw[i]) <= MIN_STOR_VOL) m->gw[i] = 0.;
     gw_runoff = gw_prev - m->gw[i] + netGWin;
@@ -882,6 +991,8 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
     /* routing.cpp:2112-2159 inland sinks */
     gw_prev = m->gw[i];
     netGWin = m->gw_recharge[i] * cellArea * (m->laf[i] / 100.) / 1000000.;
+    /* routing.cpp:2131-2137 */
+    if (m->use_on) netGWin -= net_abstraction_gw(m, i, month);
     m->gw[i] = gw_prev * EXP(-1. * k_gw) + (1. / k_gw) * netGWin * (1. - EXP(-1. * k_gw));
     if (fabs(m->gw[i]) <= MIN_STOR_VOL) m->gw[i] = 0.;
     gw_runoff = gw_prev - m->gw[i] + netGWin;
@@ -898,6 +1009,20 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
     /* routing.cpp:2180-2187 */
     m->pot_acc[i] += localRunoff + pot_open_water(m, i);
   }
+
+  /* routing.cpp:2193-2296 (use_alloc = 0, aggrNUsGloLakResOpt = 0): the day's
+   * desired surface-water use; remainingUse is 0 without water use */
+  double remainingUse = 0.;
+  if (m->use_on) {
+    double totalDesiredUse = m->use_nus[(size_t)i * 12 + month];
+    if (m->use_delayed == 1) {
+      totalDesiredUse += USE(WG_USE_TOTAL_UNSAT);
+      USE(WG_USE_PREV_TOTAL_UNSAT) = USE(WG_USE_TOTAL_UNSAT);
+      USE(WG_USE_TOTAL_UNSAT) = 0;
+    }
+    remainingUse = totalDesiredUse;
+  }
+  double remainingUseGloLake = 0., remainingUseRes = 0.;
 
   /* routing.cpp:2308-2313 */
   inflow = localRunoff;
@@ -989,7 +1114,7 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
   /* routing.cpp:2623-2624: the only read of upstream water */
   inflow += m->river_inflow[i];
 
-  /* routing.cpp:2630-2804 global lake (remainingUse = 0 without water use) */
+  /* routing.cpp:2630-2804 global lake */
   if (st->lake_area[i] > 0.) {
     double prev = m->glo_lake[i], rf = m->red_gl[i], evap, use, useMax;
     maxStorage = ((double)st->lake_area[i]) * m->lake_depth[i];
@@ -1000,12 +1125,23 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
       gwr_glolak = KSWBGW * rf * ((double)st->lake_area[i] / (cellArea * (cf / 100.)));
     else
       gwr_glolak = 0.;
-    use = evap * ((double)st->lake_area[i] / 1000000.) + gwr_glolak * cellArea * (cf / 100.) / 1000000. + 0.;
+    /* routing.cpp:2680-2689: half the use when the cell also has a reservoir */
+    if (st->reservoir_area[i] > 0.)
+      remainingUseGloLake = 0.5 * remainingUse;
+    else
+      remainingUseGloLake = remainingUse;
+    use = evap * ((double)st->lake_area[i] / 1000000.) + gwr_glolak * cellArea * (cf / 100.) / 1000000. +
+          remainingUseGloLake;
     useMax = totalInflow + maxStorage + prev;
     if (use > useMax) {
       m->glo_lake[i] = (-1.) * maxStorage;
       outflow = 0.;
       gwr_glolak *= useMax / use;
+      /* routing.cpp:2710-2716 */
+      if (remainingUseGloLake > 0.)
+        remainingUseGloLake -= remainingUseGloLake * useMax / use;
+      else
+        remainingUseGloLake = 0.;
     } else {
       m->glo_lake[i] = prev * EXP(-1. * k_sw) + (1. / k_sw) * (totalInflow - use) * (1. - EXP(-1. * k_sw));
       outflow = totalInflow + prev - m->glo_lake[i] - use;
@@ -1017,6 +1153,7 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
         outflow = 0.;
         m->glo_lake[i] = prev + totalInflow - use;
       }
+      remainingUseGloLake = 0.; /* routing.cpp:2752 */
     }
     if (fabs(m->glo_lake[i]) <= MIN_STOR_VOL) m->glo_lake[i] = 0.;
     inflow = outflow;
@@ -1047,6 +1184,28 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
     } else {
       m->res[i] = prev + totalInflow - PETgwr;
     }
+    /* routing.cpp:2869-2874, 2922-2945: the use down to 10 % of the capacity;
+     * a negative use (return flow) is added */
+    if (st->lake_area[i] > 0.)
+      remainingUseRes = 0.5 * remainingUse + remainingUseGloLake;
+    else
+      remainingUseRes = remainingUse;
+    if (m->use_on) {
+      if (remainingUseRes < 0.) {
+        m->res[i] -= remainingUseRes;
+        remainingUseRes = 0.;
+      } else {
+        if (m->res[i] > (0.1 * cap)) {
+          if (remainingUseRes < (m->res[i] - (cap * 0.1))) {
+            m->res[i] -= remainingUseRes;
+            remainingUseRes = 0.;
+          } else {
+            remainingUseRes -= (m->res[i] - (cap * 0.1));
+            m->res[i] = cap * 0.1;
+          }
+        }
+      }
+    }
     if (fabs(m->res[i]) <= MIN_STOR_VOL) m->res[i] = 0.;
     if (month == st->start_month[i] - 1) {
       if (day == first_day_in_month_r[month]) {
@@ -1057,10 +1216,25 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
       }
     }
     if ((st->res_type[i] + 0) == 1) {
-      /* irrigation reservoir; without water use the daily net use of the cell and
-       * of its downstream cells is 0 (routing.cpp:2972-2983) */
+      /* irrigation reservoir: the daily net use of the cell and of up to five
+       * downstream cells (routing.cpp:2972-2983; 0 without water use) */
       const double mean_demand = m->mean_demand[i];
-      double monthlyUse = 0.0 * days_in_month[month];
+      double dailyUse = 0.0;
+      if (m->use_on) {
+        dailyUse = m->use_nus[(size_t)i * 12 + month];
+        short k = 0;
+        int32_t downstreamCell = st->downstream[i];
+        while (k < 5 && downstreamCell > 0) {
+          /* G_reservoir_area[downstreamCell], the cell after the downstream cell (routing.cpp:2981) */
+          const double area_next = downstreamCell < n ? st->reservoir_area[downstreamCell] : 0.;
+          if (!(area_next <= 0)) break;
+          const double coeff = st->alloc_coeff ? st->alloc_coeff[(size_t)i * 5 + k] : 1.0;
+          k++;
+          dailyUse += m->use_nus[(size_t)(downstreamCell - 1) * 12 + month] * coeff;
+          downstreamCell = st->downstream[downstreamCell - 1];
+        }
+      }
+      double monthlyUse = dailyUse * days_in_month[month];
       monthlyUse = monthlyUse * 1000000000. / (days_in_month[month] * 86400.);
       if (mean_demand >= 0.5 * qmean)
         prov_rel = qmean / 2. * (1. + monthlyUse / mean_demand);
@@ -1104,6 +1278,14 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
     m->aggr_matches += hits;
   }
 
+  /* routing.cpp:3162-3171 (aggrNUsGloLakResOpt = 0) */
+  if (st->reservoir_area[i] > 0. || st->lake_area[i] > 0.) {
+    if (st->reservoir_area[i] > 0.)
+      remainingUse = remainingUseRes;
+    else
+      remainingUse = remainingUseGloLake;
+  }
+
   /* routing.cpp:3178-3297 global wetland */
   if (st->glo_wetland[i] > 0) {
     double prev = m->glo_wetl[i], rf = m->red_gw[i], evap, PETgwr, PETgwrMax;
@@ -1143,6 +1325,8 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
     gw_prev = m->gw[i];
     netGWin = gwr_swb * cellArea * (cf / 100.) / 1000000. +
               (double)m->gw_recharge[i] * cellArea * ((double)m->laf[i] / 100.) / 1000000.;
+    /* routing.cpp:3324-3330 */
+    if (m->use_on) netGWin -= net_abstraction_gw(m, i, month);
     m->gw[i] = gw_prev * EXP(-1. * k_gw) + (1. / k_gw) * netGWin * (1. - EXP(-1. * k_gw));
     if (fabs(m->gw[i]) <= MIN_STOR_VOL) m->gw[i] = 0.;
     gw_runoff = gw_prev - m->gw[i] + netGWin;
@@ -1173,18 +1357,24 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
   double riverEvapo = ((1.0 - cfa) * (ow_prec) + (cfa * ow_pet)) * riverAreaFrac / 100. * cellArea / 1000000.;
   double riverPrecip = ow_prec * riverAreaFrac / 100. * cellArea / 1000000.;
   riverInflow += riverPrecip;
-  double evapRemUse = 0. + riverEvapo;
+  double evapRemUse = remainingUse + riverEvapo;
   double evapRemUseMax = riverInflow + (K * prev * EXP(-1. * K)) / (1. - EXP(-1. * K));
   double tv;
   if (evapRemUse > evapRemUseMax) {
     m->river[i] = 0.;
     tv = riverInflow + prev - evapRemUseMax;
     if (tv < 0.) tv = 0.;
+    /* routing.cpp:3489-3492 */
+    if (remainingUse > 0.)
+      remainingUse -= remainingUse * evapRemUseMax / evapRemUse;
+    else
+      remainingUse = 0.;
   } else {
     m->river[i] = prev * EXP(-1. * K) + (1. / K) * (riverInflow - evapRemUse) * (1. - EXP(-1. * K));
     if (fabs(m->river[i]) <= MIN_STOR_VOL) m->river[i] = 0.;
     tv = riverInflow + prev - m->river[i] - evapRemUse;
     if (tv < 0.) tv = 0.;
+    remainingUse = 0.; /* routing.cpp:3512 */
   }
   /* routing.cpp:3546-3586 river width and area fraction for the next day */
   {
@@ -1203,6 +1393,7 @@ static void route_cell(wgo_model *m, int32_t i, int day, int month) {
       m->river_frac_next[i] = 0.;
     m->river_frac_change[i] = m->river_frac_next[i] - riverAreaFrac;
   }
+  if (m->use_on) use_day_end(m, i, remainingUse, month);
   m->discharge[i] = tv;
   /* routing.cpp:3955-3958 the only write to another cell */
   if ((st->downstream[i] - 1) >= 0) m->river_inflow[st->downstream[i] - 1] += tv;
@@ -1417,6 +1608,93 @@ void wgo_pot_cell_runoff(const wgo_model *m, double *pot) {
 
 void wgo_reset_pot_cell_runoff(wgo_model *m) {
   for (int32_t i = 0; i < m->n; i++) m->pot_acc[i] = 0.;
+}
+
+/* ------------------------------------------------------------------ */
+/* water use                                                            */
+/* ------------------------------------------------------------------ */
+
+int wgo_set_water_use(wgo_model *m, int delayed, const double *frac_return_gw) {
+  const size_t n = (size_t)m->n;
+  if (!m->use) {
+    m->use = (double *)calloc(n * WG_USE_NFIELDS, sizeof(double));
+    m->use_nus = (double *)calloc(n * 12, sizeof(double));
+    m->use_nug = (double *)calloc(n * 12, sizeof(double));
+    m->use_wu = (double *)calloc(n * 12, sizeof(double));
+    m->use_cu = (double *)calloc(n * 12, sizeof(double));
+    m->use_frgi = (double *)calloc(n, sizeof(double));
+    if (!m->use || !m->use_nus || !m->use_nug || !m->use_wu || !m->use_cu || !m->use_frgi) return -4;
+  }
+  memcpy(m->use_frgi, frac_return_gw, n * sizeof(double));
+  m->use_on = 1;
+  m->use_delayed = delayed;
+  return 0;
+}
+
+/* dailyNUInit, routing.cpp:909-972: the year's [N][12] tables in m3 per month;
+ * NUs and NUg times their multipliers, then over 1e9 x days of the month; the
+ * irrigation withdrawal and consumptive use over the same (routing.cpp:3918-3919) */
+void wgo_water_use_year(wgo_model *m, const double *nus_m3, const double *nug_m3, const double *wu_m3,
+                        const double *cu_m3) {
+  const int32_t n = m->n;
+  const double *par = m->st->params;
+  for (int32_t i = 0; i < n; i++) {
+    for (int month = 0; month < 12; month++) {
+      const size_t k = (size_t)i * 12 + month;
+      double nus = par[(size_t)WG_M_NETABSSW * n + i] * nus_m3[k];
+      double nug = par[(size_t)WG_M_NETABSGW * n + i] * nug_m3[k];
+      m->use_nus[k] = nus / (1000000000. * (double)days_in_month[month]);
+      m->use_nug[k] = nug / (1000000000. * (double)days_in_month[month]);
+      m->use_wu[k] = wu_m3[k] / (1000000000. * (double)days_in_month[month]);
+      m->use_cu[k] = cu_m3[k] / (1000000000. * (double)days_in_month[month]);
+    }
+  }
+}
+
+/* annualWaterUsePostProcessing, routing.cpp:5257-5296 */
+void wgo_water_use_year_end(wgo_model *m) {
+  if (!m->use_on) return;
+  for (int32_t i = 0; i < m->n; i++) {
+    if (USE(WG_USE_TOTAL_UNSAT) < USE(WG_USE_UNSAT_PREV_YEAR)) {
+      USE(WG_USE_UNSAT_PREV_YEAR) = USE(WG_USE_TOTAL_UNSAT);
+      USE(WG_USE_UNSAT_IRRIG_PREV_YEAR) = USE(WG_USE_UNSAT_IRRIG);
+      USE(WG_USE_UNSAT_OTHER_PREV_YEAR) = USE(WG_USE_UNSAT_OTHER);
+      USE(WG_USE_REDUCED_RETURN_PREV_YEAR) = USE(WG_USE_REDUCED_RETURN);
+      USE(WG_USE_UNSAT_IRRIG) = 0.;
+      USE(WG_USE_UNSAT_OTHER) = 0.;
+      USE(WG_USE_REDUCED_RETURN) = 0.;
+      USE(WG_USE_TOTAL_UNSAT) = 0.;
+    } else {
+      USE(WG_USE_TOTAL_UNSAT) -= USE(WG_USE_UNSAT_PREV_YEAR);
+      USE(WG_USE_UNSAT_IRRIG) -= USE(WG_USE_UNSAT_IRRIG_PREV_YEAR);
+      USE(WG_USE_UNSAT_OTHER) -= USE(WG_USE_UNSAT_OTHER_PREV_YEAR);
+      USE(WG_USE_REDUCED_RETURN) -= USE(WG_USE_REDUCED_RETURN_PREV_YEAR);
+      if ((USE(WG_USE_UNSAT_IRRIG) + USE(WG_USE_UNSAT_OTHER)) < 0) {
+        USE(WG_USE_UNSAT_IRRIG) = 0;
+        USE(WG_USE_UNSAT_OTHER) = 0;
+        USE(WG_USE_REDUCED_RETURN) = 0;
+      } else if (USE(WG_USE_UNSAT_OTHER) < 0) {
+        USE(WG_USE_UNSAT_IRRIG) += USE(WG_USE_UNSAT_OTHER);
+        USE(WG_USE_UNSAT_OTHER) = 0;
+      } else if (USE(WG_USE_UNSAT_IRRIG) < 0) {
+        USE(WG_USE_UNSAT_OTHER) += USE(WG_USE_UNSAT_IRRIG);
+        USE(WG_USE_UNSAT_IRRIG) = 0;
+        USE(WG_USE_REDUCED_RETURN) = 0;
+      }
+      if (USE(WG_USE_REDUCED_RETURN) > 0) {
+        USE(WG_USE_REDUCED_RETURN) = 0;
+        USE(WG_USE_UNSAT_IRRIG) = 0;
+      }
+    }
+  }
+}
+
+void wgo_get_water_use_state(const wgo_model *m, double *out) {
+  if (m->use) memcpy(out, m->use, (size_t)m->n * WG_USE_NFIELDS * sizeof(double));
+}
+
+void wgo_set_water_use_state(wgo_model *m, const double *in) {
+  if (m->use) memcpy(m->use, in, (size_t)m->n * WG_USE_NFIELDS * sizeof(double));
 }
 
 /* ------------------------------------------------------------------ */

@@ -85,6 +85,17 @@ void wgo_reset_station_discharge(wgo_model *m);
  * summed since the model was made, the last reset or the last annual init */
 void wgo_pot_cell_runoff(const wgo_model *m, double *pot);
 void wgo_reset_pot_cell_runoff(wgo_model *m);
+/* Water use (subtract_use 2/3, use_alloc 0, aggrNUsGloLakResOpt 0; routing.cpp:884-972,
+ * 1929-2137, 2193-2296, 2678-3171, 3324-3330, 3466-3512, 3590-3919, 5246-5306, 5503-5572).
+ * wgo_set_water_use turns it on (state [WG_USE_NFIELDS][N] at 0 when first made);
+ * wgo_water_use_year sets the year's [N][12] tables (m3 per month); the caller
+ * runs wgo_water_use_year_end after day 365 of each year, as integrateWGHM.cpp:940-945. */
+int wgo_set_water_use(wgo_model *m, int delayed, const double *frac_return_gw);
+void wgo_water_use_year(wgo_model *m, const double *nus_m3, const double *nug_m3, const double *wu_m3,
+                        const double *cu_m3);
+void wgo_water_use_year_end(wgo_model *m);
+void wgo_get_water_use_state(const wgo_model *m, double *out);
+void wgo_set_water_use_state(wgo_model *m, const double *in);
 /* Timing options of the CPU baseline: the reference's O(N x stations) station
  * search instead of the index map, and the O(N) riparian scan per lake/reservoir
  * cell of aggrNUsGloLakResOpt = 1.  Neither changes a result. */

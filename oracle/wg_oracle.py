@@ -66,6 +66,16 @@ def load(variant="wg"):
     lib.wgo_pot_cell_runoff.restype = None
     lib.wgo_reset_pot_cell_runoff.argtypes = [C.c_void_p]
     lib.wgo_reset_pot_cell_runoff.restype = None
+    lib.wgo_set_water_use.argtypes = [C.c_void_p, C.c_int, P_d]
+    lib.wgo_set_water_use.restype = C.c_int
+    lib.wgo_water_use_year.argtypes = [C.c_void_p, P_d, P_d, P_d, P_d]
+    lib.wgo_water_use_year.restype = None
+    lib.wgo_water_use_year_end.argtypes = [C.c_void_p]
+    lib.wgo_water_use_year_end.restype = None
+    lib.wgo_get_water_use_state.argtypes = [C.c_void_p, P_d]
+    lib.wgo_get_water_use_state.restype = None
+    lib.wgo_set_water_use_state.argtypes = [C.c_void_p, P_d]
+    lib.wgo_set_water_use_state.restype = None
     lib.wgo_perturb_forcing.argtypes = [C.c_uint64, i32, i32, C.c_double, C.c_double, P_f, P_f, P_f, P_f]
     lib.wgo_perturb_forcing.restype = None
     lib.wgo_month_of_day.argtypes = [C.c_int]
@@ -353,3 +363,27 @@ class Oracle:
 
     def reset_pot_cell_runoff(self):
         self._lib.wgo_reset_pot_cell_runoff(self._m)
+
+    # water use (subtract_use 2/3, use_alloc 0, aggrNUsGloLakResOpt 0)
+    def set_water_use(self, frac_return_gw, delayed=0):
+        self._use_frgi = np.ascontiguousarray(frac_return_gw, dtype=np.float64)
+        rc = self._lib.wgo_set_water_use(self._m, int(delayed), _abi.ptr(self._use_frgi, _abi.P_d))
+        if rc:
+            raise MemoryError("wgo_set_water_use")
+
+    def water_use_year(self, nus_m3, nug_m3, wu_m3, cu_m3):
+        """The year's [N][12] tables in m3 per month (dailyNUInit)."""
+        a = [np.ascontiguousarray(x, dtype=np.float64).reshape(self.n, 12) for x in (nus_m3, nug_m3, wu_m3, cu_m3)]
+        self._lib.wgo_water_use_year(self._m, *[_abi.ptr(x, _abi.P_d) for x in a])
+
+    def water_use_year_end(self):
+        self._lib.wgo_water_use_year_end(self._m)
+
+    def water_use_state(self):
+        out = np.zeros((_abi.WG_USE_NFIELDS, self.n))
+        self._lib.wgo_get_water_use_state(self._m, _abi.ptr(out, _abi.P_d))
+        return out
+
+    def set_water_use_state(self, state):
+        a = np.ascontiguousarray(state, dtype=np.float64).reshape(_abi.WG_USE_NFIELDS, self.n)
+        self._lib.wgo_set_water_use_state(self._m, _abi.ptr(a, _abi.P_d))

@@ -354,6 +354,36 @@ class WaterBalanceModel:
         self._c(self._lib.wg_get_pot_cell_runoff(self._h, int(member), _abi.ptr(out, _abi.P_d)))
         return out
 
+    def set_water_use(self, nus_m3, nug_m3, withdrawal_irrig_m3, consumptive_irrig_m3, frac_return_gw,
+                      year0_day=0, delayed=0, statics=None):
+        """Water use (wg_set_water_use, a handle made with subtract_use 2 or 3):
+        tables [n_years][N][12] in m3 per month, model year y starting at forcing
+        day year0_day + 365 y; statics: the World of the last annual init (default:
+        the model's own)."""
+        tabs = [np.ascontiguousarray(t, dtype=np.float64) for t in
+                (nus_m3, nug_m3, withdrawal_irrig_m3, consumptive_irrig_m3)]
+        n_years = tabs[0].size // (12 * self.n)
+        for t in tabs:
+            if t.size != n_years * 12 * self.n or n_years == 0:
+                raise ValueError("water-use tables must be [n_years][N][12]")
+        fr = np.ascontiguousarray(frac_return_gw, dtype=np.float64)
+        st = self._static if statics is None else (statics if isinstance(statics, _abi.StaticFields)
+                                                  else World.static_fields(statics))
+        self._use_static = st
+        self._c(self._lib.wg_set_water_use(self._h, C.byref(st), int(n_years), int(year0_day),
+                                           *[_abi.ptr(t, _abi.P_d) for t in tabs], _abi.ptr(fr, _abi.P_d),
+                                           int(delayed)))
+
+    def water_use_state(self, member=0):
+        """[WG_USE_NFIELDS][N] water-use state of one member (fields _abi.USE_FIELDS)."""
+        out = np.zeros((_abi.WG_USE_NFIELDS, self.n))
+        self._c(self._lib.wg_get_water_use_state(self._h, int(member), _abi.ptr(out, _abi.P_d)))
+        return out
+
+    def set_water_use_state(self, state, member=0):
+        a = np.ascontiguousarray(state, dtype=np.float64).reshape(_abi.WG_USE_NFIELDS, self.n)
+        self._c(self._lib.wg_set_water_use_state(self._h, int(member), _abi.ptr(a, _abi.P_d)))
+
     def synchronize(self):
         self._c(self._lib.wg_synchronize(self._h))
 

@@ -11,6 +11,11 @@ import numpy as np
 NSUB = 100
 NELEV = 101
 NLCT = 18
+# water-use state fields, [WG_USE_NFIELDS][N] (include/watergap2_amd.h)
+USE_FIELDS = ["total_unsat", "prev_total_unsat", "daily_remaining", "unsat_irrig", "unsat_other",
+              "reduced_return", "withdrawal_irrig", "consumptive_irrig", "unsat_prev_year",
+              "unsat_irrig_prev_year", "unsat_other_prev_year", "reduced_return_prev_year"]
+WG_USE_NFIELDS = len(USE_FIELDS)
 
 PARAMS = [
     "P_GAMRUN_C", "P_CFA", "P_CFS", "M_ROOT_D", "M_RIVRGH_C", "P_LAK_D", "P_WET_D",
@@ -256,6 +261,9 @@ def _declare(lib):
     lib.wg_reset_station_discharge.argtypes = [C.c_void_p]
     lib.wg_set_pot_cell_runoff.argtypes = [C.c_void_p, i32]
     lib.wg_get_pot_cell_runoff.argtypes = [C.c_void_p, i32, P_d]
+    lib.wg_set_water_use.argtypes = [C.c_void_p, C.POINTER(StaticFields), i32, i32, P_d, P_d, P_d, P_d, P_d, i32]
+    lib.wg_get_water_use_state.argtypes = [C.c_void_p, i32, P_d]
+    lib.wg_set_water_use_state.argtypes = [C.c_void_p, i32, P_d]
     lib.wg_station_daily_window.argtypes = [C.c_void_p, P_i32, P_i32]
     lib.wg_get_day_outputs.argtypes = [C.c_void_p, i32, C.POINTER(DayOutputs)]
     lib.wg_synchronize.argtypes = [C.c_void_p]
@@ -318,5 +326,6 @@ ABI_SYMBOLS = [
     "wg_mlp_create", "wg_mlp_destroy", "wg_mlp_last_error", "wg_mlp_n_params", "wg_mlp_get_params",
     "wg_station_corr_factor", "wg_calib_basins", "wg_upstream_stations", "wg_annual_init", "wg_reservoir_year",
     "wg_set_pot_cell_runoff", "wg_get_pot_cell_runoff", "wg_cell_corr_factor_grid", "wg_partitions",
+    "wg_set_water_use", "wg_get_water_use_state", "wg_set_water_use_state",
     "wg_mlp_set_params", "wg_mlp_forward", "wg_mlp_forward_noalloc", "wg_mlp_gradient", "wg_mlp_train",
 ]

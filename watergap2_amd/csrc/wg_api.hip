@@ -94,6 +94,10 @@ struct wg_handle {
   int32_t max_group_cells = 0;
   int n_xcd = 8;
   std::vector<WgPart> parts;                 // at least one; two run concurrently
+  // water use: the year tables' generation (wg_set_water_use), which must follow
+  // the last wg_annual_init (the irrigation reservoirs' release depends on both)
+  int32_t statics_gen = 0, use_gen = -1;
+  int32_t subtract_use = 0;
   hipEvent_t ev_fork = nullptr;
   std::vector<int32_t> tile_start, tile_end, block_group;  // whole grid: the partitions' tables in turn
   int32_t tile_capacity = 0;      // entries of the device tile tables
@@ -201,7 +205,8 @@ std::vector<T> replicate(const std::vector<T> &v, int m) {
 
 bool options_supported(const wg_options_profile *o) {
   return o->time_series == 1 && o->cloud == 1 && o->intercept == 1 && o->calc_albedo == 0 &&
-         o->pet_opt == 0 && o->use_kc == 0 && o->river_velo_opt == 2 && o->subtract_use == 0 &&
+         o->pet_opt == 0 && o->use_kc == 0 && o->river_velo_opt == 2 &&
+         (o->subtract_use == 0 || ((o->subtract_use == 2 || o->subtract_use == 3) && o->aggr_nus_glolakres_opt == 0)) &&
          o->res_opt == 1 && o->statcorr_opt == 0 && o->arid_area_opt == 1 && o->fractional_routing_opt == 1 &&
          o->river_evapo_opt == 1 && (o->aggr_nus_glolakres_opt == 0 || o->aggr_nus_glolakres_opt == 1);
 }
@@ -272,9 +277,10 @@ namespace {
 size_t wave_lds_bytes(const wg_handle *h) { return (size_t(h->max_group_cells) * 9 + 15) / 16 * 16; }
 
 // Provisional release (m3/s) of each reservoir cell for the release rule of
-// routing.cpp:2972-2999 without water use (the daily net use of the cell and of
-// its downstream cells is 0, so the month's use term is +0 and the value is the
-// same every day):
+// routing.cpp:2972-2999 in a month of `days` days whose daily net surface-water
+// use (km3/day, reference cell order) is daily_nus; without water use (NULL) the
+// use of the cell and of its downstream cells is 0, the month's use term +0 and
+// the value the same every day:
 //  - irrigation reservoirs (type 1): from the mean demand of the cell and of up to
 //    five downstream cells, routing.cpp:360-403: the cell's mean net surface-water
 //    use plus each downstream cell's weighted by its allocation coefficient,
@@ -286,7 +292,8 @@ size_t wave_lds_bytes(const wg_handle *h) { return (size_t(h->max_group_cells) *
 //    input the demand is 0.  Allocation coefficients default to 1, the value
 //    rout_prepare.cpp:906-909 gives cells that no reservoir allocates to.
 //  - other reservoirs (type 2): the mean outflow.
-std::vector<double> reservoir_prov_rel(const wg_static_fields *st, int32_t n) {
+std::vector<double> reservoir_prov_rel(const wg_static_fields *st, int32_t n, const double *daily_nus = nullptr,
+                                       int days = 31) {
   std::vector<double> out(n, 0.);
   for (int32_t i = 0; i < n; i++) {
     if (!(st->reservoir_area[i] > 0.)) continue;
@@ -307,10 +314,21 @@ std::vector<double> reservoir_prov_rel(const wg_static_fields *st, int32_t n) {
       }
     }
     mean_demand = mean_demand / 31536000.;
-    // routing.cpp:2972-2996 with dailyUse = 0
-    const double days = 31.;
-    double monthly_use = 0.0 * days;
-    monthly_use = monthly_use * 1000000000. / (days * 86400.);
+    // routing.cpp:2972-2996: the cell's daily use and that of up to five
+    // downstream cells, walked as for the mean demand
+    double daily_use = 0.0;
+    if (daily_nus) {
+      daily_use = daily_nus[i];
+      int k = 0;
+      int32_t dc = st->downstream[i];
+      while (k < 5 && dc > 0 && (dc < n ? st->reservoir_area[dc] : 0.) <= 0) {
+        daily_use += daily_nus[dc - 1] * (st->alloc_coeff ? st->alloc_coeff[size_t(i) * 5 + k] : 1.0);
+        k++;
+        dc = st->downstream[dc - 1];
+      }
+    }
+    double monthly_use = daily_use * double(days);
+    monthly_use = monthly_use * 1000000000. / (double(days) * 86400.);
     if (mean_demand >= 0.5 * qmean)
       out[i] = qmean / 2. * (1. + monthly_use / mean_demand);
     else
@@ -593,6 +611,7 @@ extern "C" int wg_create(int32_t n, const wg_static_fields *st, const wg_options
   h->n = n;
   h->m = n_members;
   if (!options_supported(opt)) return WG_E_ARG;
+  h->subtract_use = opt->subtract_use;
   for (int32_t i = 0; i < n; i++) {
     if (st->aindex[i] != 0 && st->aindex[i] != 1) return WG_E_INPUT;  // daily.cpp:345-347
     if (st->land_cover[i] < 1 || st->land_cover[i] > WG_NLCT) return WG_E_INPUT;
@@ -1378,8 +1397,8 @@ int enqueue_block(wg_handle *h, int days, hipEvent_t *evs, int part = -1) {
     }
   }
   if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], stream));
-  // closing pass: finish the last day so that the state is complete
-  launch_cell(h, S, stream, cgrid, 0, 1, 0, 0, 0);
+  // closing pass: finish the last day (day `days` - 1) so that the state is complete
+  launch_cell(h, S, stream, cgrid, days, 1, 0, 0, 0);
   HIP_CHECK(h, hipGetLastError());
   if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], stream));
   return WG_OK;
@@ -1415,6 +1434,14 @@ int check_step_args(wg_handle *h, int32_t forcing_day0, int32_t first_doy, int32
   if (int64_t(forcing_day0) + n_days > h->n_forcing_days)
     return fail(h, WG_E_ARG, "forcing days %d..%d exceed the %lld loaded days", forcing_day0,
                 forcing_day0 + n_days - 1, (long long)h->n_forcing_days);
+  if (h->subtract_use != 0) {
+    if (!h->S.use_on) return fail(h, WG_E_STATE, "water use is on but no tables are set (wg_set_water_use)");
+    if (h->use_gen != h->statics_gen)
+      return fail(h, WG_E_STATE, "wg_annual_init changed the statics: set the water-use tables again");
+    const int64_t first = int64_t(forcing_day0) - h->S.use_day0, last = first + n_days - 1;
+    if (first < 0 || last >= int64_t(h->S.use_years) * 365)
+      return fail(h, WG_E_ARG, "days %d..%d are outside the water-use tables", forcing_day0, forcing_day0 + n_days - 1);
+  }
   return WG_OK;
 }
 
@@ -1791,6 +1818,7 @@ extern "C" int wg_annual_init(wg_handle *h, const wg_static_fields *st, const in
   if (!rc) rc = to_device(h, h->S.stor_cap, permute(h, st->stor_cap));
   if (!rc) rc = to_device(h, h->S.res_prov_rel, permute(h, reservoir_prov_rel(st, n).data()));
   if (rc) return rc;
+  h->statics_gen++;
   std::vector<int8_t> cflags(n);
   for (int32_t p = 0; p < n; p++) {
     const int32_t i = h->perm[p];
@@ -1844,6 +1872,128 @@ extern "C" int wg_set_pot_cell_runoff(wg_handle *h, int32_t enable) {
     clear_graphs(h);
     for (double *b : {h->X.pot_acc, h->X.pot_base, h->X.pot_ow, h->X.pot_lgw}) release_buf(h, b);
     h->X.pot_acc = h->X.pot_base = h->X.pot_ow = h->X.pot_lgw = nullptr;
+  }
+  return WG_OK;
+}
+
+extern "C" int wg_set_water_use(wg_handle *h, const wg_static_fields *st, int32_t n_years, int32_t year0_day,
+                                const double *nus_m3, const double *nug_m3, const double *withdrawal_irrig_m3,
+                                const double *consumptive_irrig_m3, const double *frac_return_gw, int32_t delayed) {
+  if (!h || !st || n_years <= 0 || year0_day < -365 * 1000 || !nus_m3 || !nug_m3 || !withdrawal_irrig_m3 ||
+      !consumptive_irrig_m3 || !frac_return_gw || (delayed != 0 && delayed != 1) || !st->reservoir_area ||
+      !st->res_type || !st->mean_outflow || !st->downstream || !st->params)
+    return WG_E_ARG;
+  if (h->subtract_use == 0) return fail(h, WG_E_STATE, "the handle was made with subtract_use 0");
+  const int32_t n = h->n, M = h->m;
+  const size_t tab = size_t(n_years) * 12 * n;
+  static const int kDays[12] = {31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31};
+  // dailyNUInit (routing.cpp:909-972): NUs and NUg times their multipliers, then
+  // over 1e9 x days of the month; the irrigation withdrawal and consumptive use
+  // over the same (routing.cpp:3918-3919).  Reference order [y][month][n] first.
+  std::vector<double> nus(tab), nug(tab), wu(tab), cu(tab), prov(tab);
+  const double *mult_s = st->params + size_t(WG_M_NETABSSW) * n, *mult_g = st->params + size_t(WG_M_NETABSGW) * n;
+  for (int32_t y = 0; y < n_years; y++)
+    for (int mo = 0; mo < 12; mo++) {
+      const double div = 1000000000. * double(kDays[mo]);
+      double *a = &nus[(size_t(y) * 12 + mo) * n], *b = &nug[(size_t(y) * 12 + mo) * n];
+      double *c = &wu[(size_t(y) * 12 + mo) * n], *d = &cu[(size_t(y) * 12 + mo) * n];
+      for (int32_t i = 0; i < n; i++) {
+        const size_t k = (size_t(y) * n + i) * 12 + mo;
+        const double vs = mult_s[i] * nus_m3[k], vg = mult_g[i] * nug_m3[k];
+        a[i] = vs / div;
+        b[i] = vg / div;
+        c[i] = withdrawal_irrig_m3[k] / div;
+        d[i] = consumptive_irrig_m3[k] / div;
+      }
+      const std::vector<double> pr = reservoir_prov_rel(st, n, a, kDays[mo]);
+      std::copy(pr.begin(), pr.end(), &prov[(size_t(y) * 12 + mo) * n]);
+    }
+  // to the device order, table by table
+  auto dev_order = [&](std::vector<double> &v) {
+    std::vector<double> t(static_cast<size_t>(n));
+    for (size_t r = 0; r < size_t(n_years) * 12; r++) {
+      double *row = &v[r * n];
+      for (int32_t p = 0; p < n; p++) t[p] = row[h->perm[p]];
+      std::copy(t.begin(), t.end(), row);
+    }
+  };
+  for (std::vector<double> *v : {&nus, &nug, &wu, &cu, &prov}) dev_order(*v);
+  HIP_CHECK(h, hipSetDevice(h->device));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  double *d_tab[5] = {nullptr, nullptr, nullptr, nullptr, nullptr}, *d_frgi = nullptr, *d_use = nullptr;
+  const bool fresh = h->X.use == nullptr;
+  int rc = WG_OK;
+  for (int i = 0; i < 5 && !rc; i++) rc = dev_alloc(h, &d_tab[i], tab);
+  if (!rc) rc = dev_alloc(h, &d_frgi, size_t(n));
+  if (!rc && fresh) rc = dev_alloc(h, &d_use, size_t(WG_USE_NFIELDS + 1) * M * n);
+  if (rc) {
+    for (double *b : d_tab)
+      if (b) release_buf(h, b);
+    if (d_frgi) release_buf(h, d_frgi);
+    return rc;
+  }
+  const std::vector<double> *src[5] = {&nus, &nug, &wu, &cu, &prov};
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < 5 && e == hipSuccess; i++)
+    e = hipMemcpy(d_tab[i], src[i]->data(), tab * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    const std::vector<double> fr = permute(h, frac_return_gw);
+    e = hipMemcpy(d_frgi, fr.data(), size_t(n) * sizeof(double), hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) {
+    for (double *b : d_tab) release_buf(h, b);
+    release_buf(h, d_frgi);
+    if (d_use) release_buf(h, d_use);
+    return fail(h, WG_E_HIP, "water-use tables: %s", hipGetErrorString(e));
+  }
+  // the captured days hold the statics' and the state's pointers
+  clear_graphs(h);
+  for (const double *b : {h->S.use_nus, h->S.use_nug, h->S.use_wu, h->S.use_cu, h->S.use_provrel, h->S.use_frgi})
+    if (b) release_buf(h, const_cast<double *>(b));
+  h->S.use_nus = d_tab[0];
+  h->S.use_nug = d_tab[1];
+  h->S.use_wu = d_tab[2];
+  h->S.use_cu = d_tab[3];
+  h->S.use_provrel = d_tab[4];
+  h->S.use_frgi = d_frgi;
+  if (fresh) {
+    h->X.use = d_use;
+    h->X.use_rem = d_use + size_t(WG_USE_NFIELDS) * M * n;
+  }
+  h->S.use_stride = int64_t(M) * n;
+  h->S.use_years = n_years;
+  h->S.use_day0 = year0_day;
+  h->S.use_delayed = delayed;
+  h->S.use_on = 1;
+  h->use_gen = h->statics_gen;
+  return WG_OK;
+}
+
+extern "C" int wg_get_water_use_state(wg_handle *h, int32_t member, double *state) {
+  if (!h || !state || member < 0 || member >= h->m) return WG_E_ARG;
+  if (!h->X.use) return fail(h, WG_E_STATE, "water use is not on");
+  const int32_t n = h->n;
+  std::vector<double> t(size_t(WG_USE_NFIELDS) * n);
+  HIP_CHECK(h, hipSetDevice(h->device));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  for (int f = 0; f < WG_USE_NFIELDS; f++)
+    HIP_CHECK(h, hipMemcpy(&t[size_t(f) * n], h->X.use + (size_t(f) * h->m + member) * n, size_t(n) * sizeof(double),
+                           hipMemcpyDeviceToHost));
+  for (int f = 0; f < WG_USE_NFIELDS; f++)
+    for (int32_t p = 0; p < n; p++) state[size_t(f) * n + h->perm[p]] = t[size_t(f) * n + p];
+  return WG_OK;
+}
+
+extern "C" int wg_set_water_use_state(wg_handle *h, int32_t member, const double *state) {
+  if (!h || !state || member < 0 || member >= h->m) return WG_E_ARG;
+  if (!h->X.use) return fail(h, WG_E_STATE, "water use is not on");
+  const int32_t n = h->n;
+  HIP_CHECK(h, hipSetDevice(h->device));
+  HIP_CHECK(h, hipStreamSynchronize(h->stream));
+  for (int f = 0; f < WG_USE_NFIELDS; f++) {
+    const std::vector<double> v = permute(h, state + size_t(f) * n);
+    HIP_CHECK(h, hipMemcpy(h->X.use + (size_t(f) * h->m + member) * n, v.data(), size_t(n) * sizeof(double),
+                           hipMemcpyHostToDevice));
   }
   return WG_OK;
 }

@@ -48,7 +48,7 @@ struct alignas(16) WgChainStatic {
 // Stored as pairs of fields, [M][WG_DEV_NREC / 2][N] x 16 bytes, so that the lanes
 // of a wavefront, which hold consecutive cells, read and write whole cache lines
 // with 16-byte accesses.
-#define WG_DEV_NREC 38
+#define WG_DEV_NREC 40
 
 // Device layout of the routing links, [2][N] x 16 bytes: (flags, station,
 // nu | nu_level << 16, level), then four words of two upstream indices (up[2k] in
@@ -102,6 +102,15 @@ struct WgStatic {
   const int32_t *up_off, *up_idx;   // CSR of upstream cells sorted by routing order
   const int32_t *station;           // station slot of each cell, -1 = none
   int32_t n_stations;
+  // water use (use_on 0: off).  Daily values by model year and month,
+  // [use_years][12][N]: net use from surface water and from groundwater, the
+  // irrigation withdrawal and consumptive use from surface water, and the
+  // provisional release of irrigation reservoirs (m3/s); G_fractreturngw_irrig [N].
+  // Model year y covers the forcing days use_day0 + 365 y .. + 364.
+  int32_t use_on, use_delayed, use_years;
+  int64_t use_day0;
+  int64_t use_stride;           // M * N, between the fields of WgState::use
+  const double *use_nus, *use_nug, *use_wu, *use_cu, *use_provrel, *use_frgi;
 };
 
 // Per-member prognostic state and day buffers ([member][N] unless noted).
@@ -127,6 +136,11 @@ struct WgState {
   double *pot_base;             // [M][N] local runoff terms of the day
   double *pot_ow;               // [M][N] open-water balance term of the day
   double *pot_lgw;              // [M][N] arid groundwater runoff of the day
+  // water-use state [WG_USE_NFIELDS + 1][M][N] (field order of the C ABI's
+  // WG_USE_*), the last field the use the river left on the day just routed,
+  // which the next cell-kernel pass takes from the local lake
+  double *use;
+  double *use_rem;              // the last field of use (NULL: water use off)
   // stations
   double *st_sum;               // [M][n_stations]
   double *st_daily;             // [M][WG_DAILY_SLOTS][n_stations]

@@ -120,6 +120,102 @@ __device__ inline void store_chain(const WgState &X, int64_t mp, int flags, cons
   if ((flags & CFL_ARID) && X.pot_lgw) X.pot_lgw[mp] = o.lgw;
   X.river[mp] = o.river;
   X.tv[mp] = o.tv;
+  if (X.use_rem) X.use_rem[mp] = o.rem;
+}
+
+// updateNetAbstractionGW, routing.cpp:5503-5572: the day's net abstraction from
+// groundwater, with the return flows of irrigation from surface water reduced
+// when the last day's surface-water use went unsatisfied (G_dailyRemainingUse > 0)
+// or given back when a delayed use was satisfied (< 0).  U points at the cell's
+// first water-use field, us is the stride between fields.
+__device__ inline double update_net_abstraction_gw(double *U, int64_t us, double frgi, double nug) {
+  double &dru = U[WG_USE_DAILY_REMAINING * us], &ui = U[WG_USE_UNSAT_IRRIG * us];
+  double &uo = U[WG_USE_UNSAT_OTHER * us], &rrf = U[WG_USE_REDUCED_RETURN * us];
+  const double wu = U[WG_USE_WITHDRAWAL_IRRIG * us], cu = U[WG_USE_CONSUMPTIVE_IRRIG * us];
+  if (dru > 1.e-12) {
+    if (wu > 0.) {
+      const double eff = cu / wu;
+      const double factor = 1 - (1 - frgi) * (1 - eff);
+      double WUsiNew = 1 / factor * (wu * factor - dru);
+      if (WUsiNew < 0.) {
+        WUsiNew = 0.;
+        ui += wu * factor;
+        uo += dru - (wu * factor);
+      } else {
+        ui += dru;
+      }
+      const double returnflowChange = (frgi * (1 - eff) * (WUsiNew - wu));
+      rrf += returnflowChange;
+      dru = 0.;
+      return nug - returnflowChange;
+    }
+    uo += dru;
+    return nug;
+  }
+  if (dru < -1.e-12) {
+    double fromIrrig = dru + uo;
+    if (fromIrrig < 0.) {
+      uo = 0.;
+    } else {
+      uo += dru;
+      dru = 0.;
+      return nug;
+    }
+    if (ui == 0.) {
+      dru = 0.;
+      return nug;
+    }
+    double ratio = (fromIrrig / ui);
+    if (ratio < -1.) {
+      ratio = -1.;
+      fromIrrig = ui * -1.;
+    }
+    const double returnflowChange = (ratio * rrf);
+    ui += fromIrrig;
+    rrf += returnflowChange;
+    dru = 0.;
+    return nug - returnflowChange;
+  }
+  return nug;
+}
+
+// annualWaterUsePostProcessing, routing.cpp:5257-5296, for one cell
+__device__ inline void water_use_year_end(double *U, int64_t us) {
+  double &tu = U[WG_USE_TOTAL_UNSAT * us], &ui = U[WG_USE_UNSAT_IRRIG * us];
+  double &uo = U[WG_USE_UNSAT_OTHER * us], &rrf = U[WG_USE_REDUCED_RETURN * us];
+  double &tu_py = U[WG_USE_UNSAT_PREV_YEAR * us], &ui_py = U[WG_USE_UNSAT_IRRIG_PREV_YEAR * us];
+  double &uo_py = U[WG_USE_UNSAT_OTHER_PREV_YEAR * us], &rrf_py = U[WG_USE_REDUCED_RETURN_PREV_YEAR * us];
+  if (tu < tu_py) {
+    tu_py = tu;
+    ui_py = ui;
+    uo_py = uo;
+    rrf_py = rrf;
+    ui = 0.;
+    uo = 0.;
+    rrf = 0.;
+    tu = 0.;
+  } else {
+    tu -= tu_py;
+    ui -= ui_py;
+    uo -= uo_py;
+    rrf -= rrf_py;
+    if ((ui + uo) < 0) {
+      ui = 0;
+      uo = 0;
+      rrf = 0;
+    } else if (uo < 0) {
+      ui += uo;
+      uo = 0;
+    } else if (ui < 0) {
+      uo += ui;
+      ui = 0;
+      rrf = 0;
+    }
+    if (rrf > 0) {
+      rrf = 0;
+      ui = 0;
+    }
+  }
 }
 
 // routing.cpp:7309-7341 getNewRiverVelocity
@@ -211,7 +307,14 @@ struct PostDay {
   double laf, laf_prev, red_ll, red_lw, red_gw, red_gl, red_res, river_frac_next;
 };
 
-__device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p, PostDay &out) {
+// With water use, the finished day's end of the routing loop comes first
+// (routing.cpp:3590-3622, 3898-3919): the use the river left is taken from the
+// local lake, whose area reduction factor is recomputed, the unsatisfied use is
+// kept and the month's irrigation withdrawal and consumptive use are recorded
+// (fin_ym: model year x 12 + month of the finished day); after day 365 the
+// year-end processing follows (fin_year_end).
+__device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p, PostDay &out, int fin_ym,
+                         bool fin_year_end) {
   // Every input is loaded before the first store, so that the loads are in flight
   // together (the arrays cover all cells, so the loads of unused fields are in bounds).
   const double evarexp = S.evarexp[p];
@@ -231,6 +334,43 @@ __device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p,
   double red_river = X.red_river[mp];
   const double fswb_old = X.fswb_next[mp];
   const double laf = X.laf[mp];
+  if (S.use_on && contcell) {
+    const int64_t us = S.use_stride;
+    double *U = X.use + mp;
+    double rem = X.use_rem[mp];
+    if ((loc_lake > 0.) && (rem > 0.)) {
+      const double maxStorage = ((loc_lake) / 100.) * cellArea * lake_depth;
+      double ls = X.loc_lake[mp];
+      if (ls > (-1.) * maxStorage) {
+        if (rem < (maxStorage + ls)) {
+          ls -= rem;
+          rem = 0.;
+        } else {
+          rem -= (maxStorage + ls);
+          ls = (-1.) * maxStorage;
+        }
+      }
+      double r = 1. - wg_pow(fabs(ls - maxStorage) / (2. * maxStorage), (evarexp * kEvapoRedExp));
+      if (r < 0.) r = 0.;
+      if (r > 1.) r = 1.;
+      red_ll = r;
+      X.loc_lake[mp] = ls;
+    }
+    if (S.use_delayed == 1) {
+      const double tu = U[WG_USE_TOTAL_UNSAT * us];
+      U[WG_USE_PREV_TOTAL_UNSAT * us] = tu;
+      U[WG_USE_DAILY_REMAINING * us] = rem - tu;
+      U[WG_USE_TOTAL_UNSAT * us] = rem;
+    } else {
+      U[WG_USE_TOTAL_UNSAT * us] += rem;
+      U[WG_USE_DAILY_REMAINING * us] = rem;
+    }
+    const int64_t t = int64_t(fin_ym) * S.n + p;
+    U[WG_USE_WITHDRAWAL_IRRIG * us] = S.use_wu[t];
+    U[WG_USE_CONSUMPTIVE_IRRIG * us] = S.use_cu[t];
+    X.use_rem[mp] = 0.;
+    if (fin_year_end) water_use_year_end(U, us);
+  }
 
   double red_gl = 0., red_res = 0.;
   if (lake_area > 0.) {
@@ -603,7 +743,16 @@ __global__ __launch_bounds__(256, kSnowChunk == 5 ? 3 : 1) void wg_cell_kernel(W
   // the fractions post_day writes, which the rest of the kernel reads
   PostDay pd;
   if (do_post) {
-    post_day(S, X, mp, p, pd);
+    // the finished day is day k - 1 of the block (the closing pass runs with k = days)
+    int fin_ym = 0;
+    bool fin_year_end = false;
+    if (S.use_on) {
+      const int64_t fin = clock->day0 + k - 1;
+      const int fin_doy = int((clock->doy0 - 1 + k - 1) % 365) + 1;
+      fin_ym = int((fin - S.use_day0) / 365) * 12 + day_info(fin_doy).month;
+      fin_year_end = fin_doy == 365;
+    }
+    post_day(S, X, mp, p, pd, fin_ym, fin_year_end);
   } else if (do_vertical) {
     pd.laf = X.laf[mp];
     pd.laf_prev = X.laf_prev[mp];
@@ -637,6 +786,22 @@ __global__ __launch_bounds__(256, kSnowChunk == 5 ? 3 : 1) void wg_cell_kernel(W
     normal_pair(seed, ref, uint32_t(fday), zp, zt);
     pf = float(double(pf) * wg_exp(X.prec_sigma * zp));
     tf = float(double(tf) + X.temp_sigma * zt);
+  }
+  // water use of the day: the desired surface-water use (routing.cpp:2193-2290)
+  // and the net abstraction from groundwater (routing.cpp:7440-7441, adapted by
+  // updateNetAbstractionGW in every groundwater branch when the last day left a
+  // remaining use; the adaptation reads only the last day's state, so it is done
+  // here for (semi-)arid cells too, whose groundwater step the chain runs)
+  double use_desired = 0., use_nag = 0.;
+  int64_t use_t = 0;
+  if (S.use_on) {
+    use_t = (int64_t((fday - S.use_day0) / 365) * 12 + day_info(doy).month) * n + p;
+    const int64_t us = S.use_stride;
+    double *U = X.use + mp;
+    use_nag = S.use_nug[use_t];
+    if (U[WG_USE_DAILY_REMAINING * us] != 0.) use_nag = update_net_abstraction_gw(U, us, S.use_frgi[p], use_nag);
+    use_desired = S.use_nus[use_t];
+    if (S.use_delayed == 1) use_desired += U[WG_USE_TOTAL_UNSAT * us];
   }
 
   // ------------------------------------------------------------------
@@ -987,7 +1152,8 @@ __global__ __launch_bounds__(256, kSnowChunk == 5 ? 3 : 1) void wg_cell_kernel(W
   } else {
     // humid cells (routing.cpp:1968-2039) and inland sinks (2112-2159)
     const double gw_prev = ci.x_gw;
-    const double netGWin = recharge_handoff * cellArea * (laf / 100.) / 1000000.;
+    double netGWin = recharge_handoff * cellArea * (laf / 100.) / 1000000.;
+    if (S.use_on) netGWin -= use_nag;  // routing.cpp:1990-1996, 2131-2137
     double gw = gw_prev * ci.s_e_gw + ci.s_inv_kgw * netGWin * ci.s_ome_gw;
     if (fabs(gw) <= kMinStorVol) gw = 0.;
     double gw_runoff = gw_prev - gw + netGWin;
@@ -1158,7 +1324,11 @@ __global__ __launch_bounds__(256, kSnowChunk == 5 ? 3 : 1) void wg_cell_kernel(W
       v[CF_GL_MAX] = lake_area * ci.s_lake_depth;
       v[CF_GL_LP] = (ow_prec * (lake_area / 1000000.));
       v[CF_GL_GWR] = gwr;
-      v[CF_GL_USE] = evap * (lake_area / 1000000.) + gwr * cellArea * (cf / 100.) / 1000000. + 0.;
+      // the lake's share of the use, half of it when the cell has a reservoir too
+      // (routing.cpp:2680-2695)
+      const double use_lake = (lk.flags & CFL_RES) ? 0.5 * use_desired : use_desired;
+      v[CF_GL_USE] = evap * (lake_area / 1000000.) + gwr * cellArea * (cf / 100.) / 1000000. +
+                     (S.use_on ? use_lake : 0.);
       v[CF_GL_PREV] = prev;
       v[CF_GL_S0E] = prev * e_sw;
     }
@@ -1204,9 +1374,11 @@ __global__ __launch_bounds__(256, kSnowChunk == 5 ? 3 : 1) void wg_cell_kernel(W
     v[CF_RES_CAP] = res ? ci.s_stor_cap : 0.;
     v[CF_RES_QMEAN] = res ? ci.s_mean_outflow : 0.;
     v[CF_RES_AREA] = res ? ci.s_reservoir_area : 0.;
-    v[CF_RES_PROVREL] = res ? ci.s_prov_rel : 0.;
+    v[CF_RES_PROVREL] = res ? (S.use_on ? S.use_provrel[use_t] : ci.s_prov_rel) : 0.;
     v[CF_RES_PAD] = 0.;
   }
+  v[CF_USE] = use_desired;
+  v[CF_A_NUG] = use_nag;
   if (lk.nu == 0) {
     // no upstream cell: G_riverInflow[n] is still 0 when the cell is routed
     const ChainOut o = chain_step(c, 0.0, day_info(doy));
@@ -1223,8 +1395,9 @@ __global__ __launch_bounds__(256, kSnowChunk == 5 ? 3 : 1) void wg_cell_kernel(W
     for (int k = 0; k < kPrefixPairs; k++) dst[int64_t(k) * n] = WgD2{v[2 * k], v[2 * k + 1]};
     if (lk.flags & (CFL_LAKE | CFL_RES)) {
 #pragma unroll
-      for (int k = kPrefixPairs; k < CF_COUNT / 2; k++) dst[int64_t(k) * n] = WgD2{v[2 * k], v[2 * k + 1]};
+      for (int k = kPrefixPairs; k < CF_USE / 2; k++) dst[int64_t(k) * n] = WgD2{v[2 * k], v[2 * k + 1]};
     }
+    if (S.use_on) dst[int64_t(CF_USE / 2) * n] = WgD2{v[CF_USE], v[CF_A_NUG]};
   }
 }
 
@@ -1302,11 +1475,20 @@ __device__ inline void prefetch_slot_wave(const WgStatic &S, const WgState &X, i
   const int off = suffix ? p * 16 : int(0x80000000u);
   WgD2 *row0 = X.crec + base * (WG_DEV_NREC / 2);
 #pragma unroll
-  for (int k = kPrefixPairs; k < CF_COUNT / 2; k++) {
+  for (int k = kPrefixPairs; k < CF_USE / 2; k++) {
     const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(row0 + k * n, (short)0, int(n * 16), 0x00020000);
     const WgU32x4 w = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
     q.a[2 * k] = __builtin_bit_cast(double, (unsigned long long)w.x | ((unsigned long long)w.y << 32));
     q.a[2 * k + 1] = __builtin_bit_cast(double, (unsigned long long)w.z | ((unsigned long long)w.w << 32));
+  }
+  // the water-use pair, for every cell when water use is on
+  if (S.use_on) {
+    const WgD2 u = src[int64_t(CF_USE / 2) * n];
+    q.a[CF_USE] = u.x;
+    q.a[CF_A_NUG] = u.y;
+  } else {
+    q.a[CF_USE] = 0.;
+    q.a[CF_A_NUG] = 0.;
   }
   q.inv_ksw = cs.inv_ksw;
   q.ome_sw = cs.ome_sw;
@@ -1346,14 +1528,22 @@ __device__ inline void prefetch_slot_batch(const WgStatic &S, const WgState &X, 
   }
   if (suffix) {
 #pragma unroll
-    for (int k = kPrefixPairs; k < CF_COUNT / 2; k++) {
+    for (int k = kPrefixPairs; k < CF_USE / 2; k++) {
       const WgD2 v = src[k * n];
       q.a[2 * k] = v.x;
       q.a[2 * k + 1] = v.y;
     }
   } else {
 #pragma unroll
-    for (int k = 2 * kPrefixPairs; k < CF_COUNT; k++) q.a[k] = 0.;
+    for (int k = 2 * kPrefixPairs; k < CF_USE; k++) q.a[k] = 0.;
+  }
+  if (S.use_on) {
+    const WgD2 u = src[int64_t(CF_USE / 2) * n];
+    q.a[CF_USE] = u.x;
+    q.a[CF_A_NUG] = u.y;
+  } else {
+    q.a[CF_USE] = 0.;
+    q.a[CF_A_NUG] = 0.;
   }
   q.inv_ksw = cs.inv_ksw;
   q.ome_sw = cs.ome_sw;
@@ -1435,6 +1625,7 @@ __device__ __forceinline__ void chain_cell(const WgStatic &S, const WgState &X, 
   store_or_drop(X.glo_wetl + base, n, run && (q.flags & CFL_GWET), p, o.glo_wetl);
   store_or_drop(X.gw + base, n, run && (q.flags & CFL_ARID), p, o.gw);
   if (X.pot_lgw) store_or_drop(X.pot_lgw + base, n, run && (q.flags & CFL_ARID), p, o.lgw);
+  if (X.use_rem) store_or_drop(X.use_rem + base, n, run, p, o.rem);
   store_or_drop(X.river + base, n, run, p, o.river);
   store_or_drop(X.tv + base, n, run, p, o.tv);
   if (run) s_tv[p - gs] = o.tv;
@@ -1618,6 +1809,7 @@ __device__ __forceinline__ void wave_batch(const WgStatic &S, const WgState &X, 
   store_or_drop(X.glo_wetl + base, n, run && (q.flags & CFL_GWET), p, o.glo_wetl);
   store_or_drop(X.gw + base, n, run && (q.flags & CFL_ARID), p, o.gw);
   if (X.pot_lgw) store_or_drop(X.pot_lgw + base, n, run && (q.flags & CFL_ARID), p, o.lgw);
+  if (X.use_rem) store_or_drop(X.use_rem + base, n, run, p, o.rem);
   store_or_drop(X.river + base, n, run, p, o.river);
   store_or_drop(X.tv + base, n, run, p, o.tv);
   const double qq = (run && (q.flags & CFL_LDD)) ? o.tv : 0.;

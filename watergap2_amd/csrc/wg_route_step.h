@@ -84,10 +84,15 @@ enum ChainField : int {
   // the mean demand of routing.cpp:360-403, computed once on the host), padding
   CF_RES_PREV, CF_RES_RED, CF_RES_CFA, CF_RES_KREL, CF_RES_CAP, CF_RES_QMEAN, CF_RES_AREA, CF_RES_PROVREL,
   CF_RES_PAD,
+  // water use (loaded only when it is on; 0 otherwise): the day's desired
+  // surface-water use (routing.cpp:2193-2290) and the net abstraction from the
+  // groundwater of (semi-)arid cells (routing.cpp:3324-3330)
+  CF_USE, CF_A_NUG,
   CF_COUNT
 };
 static_assert(CF_COUNT == WG_DEV_NREC, "routing record size");
 static_assert(CF_COUNT % 2 == 0, "routing record stored in pairs");
+static_assert(CF_USE % 2 == 0 && CF_A_NUG == CF_USE + 1 && CF_COUNT == CF_USE + 2, "water-use pair last");
 
 // chain flags (static per cell); bits 8-15 reservoir type, 16-23 start month
 enum {
@@ -104,12 +109,14 @@ struct ChainRec {
 
 // lgw: G_localGWRunoffIntoRiver after the step (the arid groundwater step's
 // runoff in (semi-)arid cells), a term of the potential cell runoff
+// rem: the surface-water use the river left (routing.cpp:3489-3512), which the
+// local lake is asked for at the end of the cell's routing (3590-3622)
 struct ChainOut {
-  double glo_lake, res, k_release, glo_wetl, gw, river, tv, lgw;
+  double glo_lake, res, k_release, glo_wetl, gw, river, tv, lgw, rem;
 };
 
 struct RiverOut {
-  double st, tv;
+  double st, tv, rem;
 };
 
 // River storage and transported volume, routing.cpp:3402-3514: riverInflow is the
@@ -117,35 +124,42 @@ struct RiverOut {
 // and the precipitation on the river.
 WG_RS_FN RiverOut river_step(double inflow, int flags, double lrir, double lgwrir, double rv_prec,
                                       double prev, double rv_evap, double rv_term, double s0e, double invk,
-                                      double ome) {
+                                      double ome, double remainingUse) {
   double riverInflow = inflow;
   if (flags & CFL_LDD) {
     riverInflow += lrir;
     riverInflow += lgwrir;
   }
   riverInflow += rv_prec;
-  const double evapRemUse = 0. + rv_evap;
+  const double evapRemUse = remainingUse + rv_evap;
   const double evapRemUseMax = riverInflow + rv_term;
   RiverOut r;
   if (evapRemUse > evapRemUseMax) {
     r.st = 0.;
     r.tv = riverInflow + prev - evapRemUseMax;
     if (r.tv < 0.) r.tv = 0.;
+    if (remainingUse > 0.)
+      r.rem = remainingUse - remainingUse * evapRemUseMax / evapRemUse;
+    else
+      r.rem = 0.;
   } else {
     r.st = s0e + invk * (riverInflow - evapRemUse) * ome;
     if (fabs(r.st) <= kMinStorVol) r.st = 0.;
     r.tv = riverInflow + prev - r.st - evapRemUse;
     if (r.tv < 0.) r.tv = 0.;
+    r.rem = 0.;
   }
   return r;
 }
 
 // Groundwater of a (semi-)arid cell with recharge below surface water bodies,
-// routing.cpp:3305-3386.  Returns the groundwater runoff; gw receives the storage.
+// routing.cpp:3305-3386, less the day's net abstraction nug (0 without water use).
+// Returns the groundwater runoff; gw receives the storage.
 WG_RS_FN double arid_groundwater(double gwr_swb, double area, double cf100, double recharge,
                                          double gw_prev, double s0e, double inv_kgw, double ome_gw,
-                                         double &gw_out) {
-  const double netGWin = gwr_swb * area * cf100 / 1000000. + recharge;
+                                         double nug, double &gw_out) {
+  double netGWin = gwr_swb * area * cf100 / 1000000. + recharge;
+  netGWin -= nug;
   double gw = s0e + inv_kgw * netGWin * ome_gw;
   if (fabs(gw) <= kMinStorVol) gw = 0.;
   double gw_runoff = gw_prev - gw + netGWin;
@@ -158,10 +172,12 @@ WG_RS_FN double arid_groundwater(double gwr_swb, double area, double cf100, doub
   return gw_runoff;
 }
 
-// routing.cpp:2623-3958 for one cell from its record (parity profile: no water
-// use, no station correction).  inflow_up is G_riverInflow[n] when the cell is
-// reached.  The reservoir branch (about one cell in two hundred) reads its own
-// inputs here.
+// routing.cpp:2623-3958 for one cell from its record (no station correction).
+// inflow_up is G_riverInflow[n] when the cell is reached.  The reservoir branch
+// (about one cell in two hundred) reads its own inputs here.  The water use
+// (CF_USE) is shared between a global lake and a reservoir, taken from each
+// (routing.cpp:2678-2716, 2866-2945), and what is left goes to the river; without
+// water use it is +0 throughout, which leaves every result unchanged.
 WG_RS_FN ChainOut chain_step(const ChainRec &c, double inflow_up, const DayInfo &d) {
   const int month = d.month;
   ChainOut o;
@@ -179,17 +195,23 @@ WG_RS_FN ChainOut chain_step(const ChainRec &c, double inflow_up, const DayInfo 
   o.k_release = 0.;
 
   inflow += inflow_up;  // routing.cpp:2623
+  double remainingUse = v[CF_USE], remainingUseGloLake = 0., remainingUseRes = 0.;
 
-  // global lake, routing.cpp:2630-2804
+  // global lake, routing.cpp:2630-2804; CF_GL_USE holds the lake's share of the use
   if (c.flags & CFL_LAKE) {
     const double prev = v[CF_GL_PREV], use = v[CF_GL_USE], maxStorage = v[CF_GL_MAX];
     const double totalInflow = inflow + v[CF_GL_LP];
     const double useMax = totalInflow + maxStorage + prev;
+    remainingUseGloLake = (c.flags & CFL_RES) ? 0.5 * remainingUse : remainingUse;
     double st;
     if (use > useMax) {
       st = (-1.) * maxStorage;
       outflow = 0.;
       gwr_glolak *= useMax / use;
+      if (remainingUseGloLake > 0.)
+        remainingUseGloLake -= remainingUseGloLake * useMax / use;
+      else
+        remainingUseGloLake = 0.;
     } else {
       st = v[CF_GL_S0E] + c.inv_ksw * (totalInflow - use) * c.ome_sw;
       outflow = totalInflow + prev - st - use;
@@ -201,6 +223,7 @@ WG_RS_FN ChainOut chain_step(const ChainRec &c, double inflow_up, const DayInfo 
         outflow = 0.;
         st = prev + totalInflow - use;
       }
+      remainingUseGloLake = 0.;
     }
     if (fabs(st) <= kMinStorVol) st = 0.;
     o.glo_lake = st;
@@ -232,6 +255,21 @@ WG_RS_FN ChainOut chain_step(const ChainRec &c, double inflow_up, const DayInfo 
     } else {
       st = prev + totalInflow - PETgwr;
     }
+    // the use down to 10 % of the capacity, a return flow (< 0) added
+    // (routing.cpp:2869-2874, 2922-2945); with +0 use the storage is unchanged
+    remainingUseRes = (c.flags & CFL_LAKE) ? 0.5 * remainingUse + remainingUseGloLake : remainingUse;
+    if (remainingUseRes < 0.) {
+      st -= remainingUseRes;
+      remainingUseRes = 0.;
+    } else if (st > (0.1 * cap)) {
+      if (remainingUseRes < (st - (cap * 0.1))) {
+        st -= remainingUseRes;
+        remainingUseRes = 0.;
+      } else {
+        remainingUseRes -= (st - (cap * 0.1));
+        st = cap * 0.1;
+      }
+    }
     if (fabs(st) <= kMinStorVol) st = 0.;
     double krel = v[CF_RES_KREL];
     if (month == flag_start_month(c.flags) - 1) {
@@ -244,8 +282,7 @@ WG_RS_FN ChainOut chain_step(const ChainRec &c, double inflow_up, const DayInfo 
     }
     const int rtype = flag_res_type(c.flags);
     if (rtype == 1) {
-      // irrigation reservoir: without water use the month's use term is +0, so
-      // the provisional release is a constant of the cell
+      // irrigation reservoir: the month's provisional release (routing.cpp:2970-2996)
       prov_rel = v[CF_RES_PROVREL];
     } else if (rtype == 2) {
       prov_rel = qmean;
@@ -273,6 +310,12 @@ WG_RS_FN ChainOut chain_step(const ChainRec &c, double inflow_up, const DayInfo 
     o.k_release = krel;
     inflow = outflow;
   }
+
+  // routing.cpp:3162-3171 (aggrNUsGloLakResOpt = 0)
+  if (c.flags & CFL_RES)
+    remainingUse = remainingUseRes;
+  else if (c.flags & CFL_LAKE)
+    remainingUse = remainingUseGloLake;
 
   // global wetland, routing.cpp:3178-3297
   if (c.flags & CFL_GWET) {
@@ -302,16 +345,17 @@ WG_RS_FN ChainOut chain_step(const ChainRec &c, double inflow_up, const DayInfo 
   if (arid_swb) {
     const double gwr_swb = v[CF_A_GWR_LL] + gwr_glolak + v[CF_A_GWR_LW] + gwr_glowet + gwr_res;
     lgwrir = arid_groundwater(gwr_swb, c.area, c.cf100, v[CF_A_RECH], v[CF_A_PREV], v[CF_A_S0E], c.inv_kgw,
-                              c.ome_gw, o.gw);
+                              c.ome_gw, v[CF_A_NUG], o.gw);
   }
 
   // river, routing.cpp:3402-3514
   const RiverOut r = river_step(inflow, c.flags, v[CF_LRIR], lgwrir, v[CF_RV_PREC], v[CF_RV_PREV], v[CF_RV_EVAP],
-                                v[CF_RV_TERM], v[CF_RV_S0E], v[CF_RV_INVK], v[CF_RV_OME]);
+                                v[CF_RV_TERM], v[CF_RV_S0E], v[CF_RV_INVK], v[CF_RV_OME], remainingUse);
   const double st = r.st, tv = r.tv;
   o.river = st;
   o.tv = tv;
   o.lgw = lgwrir;
+  o.rem = r.rem;
   return o;
 }
 
@@ -337,7 +381,7 @@ WG_RS_FN StepPre step_pre(const ChainRec &c) {
     const double gwr_glolak = (c.flags & CFL_LAKE) ? v[CF_GL_GWR] : 0.;
     const double gwr_swb = v[CF_A_GWR_LL] + gwr_glolak + v[CF_A_GWR_LW] + v[CF_GW_GWR] + 0.;
     r.lgwrir = arid_groundwater(gwr_swb, c.area, c.cf100, v[CF_A_RECH], v[CF_A_PREV], v[CF_A_S0E], c.inv_kgw,
-                                c.ome_gw, r.gw);
+                                c.ome_gw, v[CF_A_NUG], r.gw);
   }
   return r;
 }
@@ -394,7 +438,9 @@ WG_RS_FN ChainOut step_fast(const ChainRec &c, const StepPre &pre, double inflow
     const double rin_ldd = (rin + v[CF_LRIR]) + pre.lgwrir;
     rin = (c.flags & CFL_LDD) ? rin_ldd : rin;
     rin += v[CF_RV_PREC];
-    const double evapRemUse = 0. + v[CF_RV_EVAP];
+    // a global lake that keeps water leaves no use (routing.cpp:2752); no reservoir here
+    const double remainingUse = (c.flags & CFL_LAKE) ? 0. : v[CF_USE];
+    const double evapRemUse = remainingUse + v[CF_RV_EVAP];
     const double evapRemUseMax = rin + v[CF_RV_TERM];
     double tv_dry = rin + prev - evapRemUseMax;
     tv_dry = (tv_dry < 0.) ? 0. : tv_dry;
@@ -405,6 +451,8 @@ WG_RS_FN ChainOut step_fast(const ChainRec &c, const StepPre &pre, double inflow
     const bool dry = evapRemUse > evapRemUseMax;
     o.river = dry ? 0. : st;
     o.tv = dry ? tv_dry : tv;
+    const double rem_dry = (remainingUse > 0.) ? remainingUse - remainingUse * evapRemUseMax / evapRemUse : 0.;
+    o.rem = dry ? rem_dry : 0.;
   }
   return o;
 }
