@@ -1322,12 +1322,11 @@ namespace {
 // Launches the cell kernel with the handle's snow chunk.
 void launch_cell(wg_handle *h, const WgStatic &S, hipStream_t stream, dim3 grid, int k, int do_post,
                  int do_vertical, int write_diag, int slot) {
-  if (h->snow_chunk == 10)
-    hipLaunchKernelGGL(wg_cell_kernel<10>, grid, dim3(h->block_threads_cell), 0, stream, S, h->X, h->T,
-                       h->d_forcing, h->n_forcing_days, h->d_clock, k, do_post, do_vertical, write_diag, slot);
-  else
-    hipLaunchKernelGGL(wg_cell_kernel<5>, grid, dim3(h->block_threads_cell), 0, stream, S, h->X, h->T,
-                       h->d_forcing, h->n_forcing_days, h->d_clock, k, do_post, do_vertical, write_diag, slot);
+  // the kernels without water use carry no use code
+  auto *kern = h->snow_chunk == 10 ? (S.use_on ? wg_cell_kernel<10, true> : wg_cell_kernel<10, false>)
+                                   : (S.use_on ? wg_cell_kernel<5, true> : wg_cell_kernel<5, false>);
+  hipLaunchKernelGGL(kern, grid, dim3(h->block_threads_cell), 0, stream, S, h->X, h->T, h->d_forcing,
+                     h->n_forcing_days, h->d_clock, k, do_post, do_vertical, write_diag, slot);
 }
 
 // Enqueues one block of `days` days (captured or eager): of the whole grid on the
@@ -1361,27 +1360,23 @@ int enqueue_block(wg_handle *h, int days, hipEvent_t *evs, int part = -1) {
     if (evs) HIP_CHECK(h, hipEventRecord(evs[ev++], stream));
     if (h->chain_mode == 3) {
       // the wave walk runs 8 wavefronts with a 512-thread configuration, else 4
-      if (h->block_threads_chain >= 512)
-        hipLaunchKernelGGL(wg_chain_wave_kernel<8>, chgrid, dim3(512), wave_lds_bytes(h),
-                           stream, S, h->X, h->T, h->d_level_off, h->d_group_first, h->d_wave_wide,
-                           h->d_wave_item_off[1], h->d_wave_items[1], h->d_clock, k, slot);
-      else
-        hipLaunchKernelGGL(wg_chain_wave_kernel<4>, chgrid, dim3(256), wave_lds_bytes(h),
-                           stream, S, h->X, h->T, h->d_level_off, h->d_group_first, h->d_wave_wide,
-                           h->d_wave_item_off[0], h->d_wave_items[0], h->d_clock, k, slot);
+      const bool w8 = h->block_threads_chain >= 512;
+      auto *kern = w8 ? (S.use_on ? wg_chain_wave_kernel<8, true> : wg_chain_wave_kernel<8, false>)
+                      : (S.use_on ? wg_chain_wave_kernel<4, true> : wg_chain_wave_kernel<4, false>);
+      hipLaunchKernelGGL(kern, chgrid, dim3(w8 ? 512 : 256), wave_lds_bytes(h), stream, S, h->X, h->T,
+                         h->d_level_off, h->d_group_first, h->d_wave_wide, h->d_wave_item_off[w8 ? 1 : 0],
+                         h->d_wave_items[w8 ? 1 : 0], h->d_clock, k, slot);
     }
     else if (h->chain_mode == 1)
-      hipLaunchKernelGGL(wg_chain_flow_kernel, chgrid, dim3(h->block_threads_chain),
-                         size_t(h->max_group_cells) * sizeof(double), stream, S, h->X, h->T,
-                         h->d_level_off, h->d_group_first, h->d_clock, k, slot);
-    else if (h->chain_mode == 2)
-      hipLaunchKernelGGL(wg_chain_kernel<2>, chgrid, dim3(h->block_threads_chain),
-                         size_t(h->max_group_cells) * sizeof(double), stream, S, h->X, h->T,
-                         h->d_level_off, h->d_group_first, h->d_clock, k, slot);
-    else
-      hipLaunchKernelGGL(wg_chain_kernel<1>, chgrid, dim3(h->block_threads_chain),
-                         size_t(h->max_group_cells) * sizeof(double), stream, S, h->X, h->T,
-                         h->d_level_off, h->d_group_first, h->d_clock, k, slot);
+      hipLaunchKernelGGL(S.use_on ? wg_chain_flow_kernel<true> : wg_chain_flow_kernel<false>, chgrid,
+                         dim3(h->block_threads_chain), size_t(h->max_group_cells) * sizeof(double), stream, S,
+                         h->X, h->T, h->d_level_off, h->d_group_first, h->d_clock, k, slot);
+    else {
+      auto *kern = h->chain_mode == 2 ? (S.use_on ? wg_chain_kernel<2, true> : wg_chain_kernel<2, false>)
+                                      : (S.use_on ? wg_chain_kernel<1, true> : wg_chain_kernel<1, false>);
+      hipLaunchKernelGGL(kern, chgrid, dim3(h->block_threads_chain), size_t(h->max_group_cells) * sizeof(double),
+                         stream, S, h->X, h->T, h->d_level_off, h->d_group_first, h->d_clock, k, slot);
+    }
     HIP_CHECK(h, hipGetLastError());
     if (h->X.pot_acc) {
       hipLaunchKernelGGL(wg_pot_runoff_kernel, dim3((c1 - c0 + 255) / 256, M), dim3(256), 0, stream, S, h->X, c0,

@@ -313,6 +313,7 @@ struct PostDay {
 // kept and the month's irrigation withdrawal and consumptive use are recorded
 // (fin_ym: model year x 12 + month of the finished day); after day 365 the
 // year-end processing follows (fin_year_end).
+template <bool kUse>
 __device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p, PostDay &out, int fin_ym,
                          bool fin_year_end) {
   // Every input is loaded before the first store, so that the loads are in flight
@@ -334,7 +335,7 @@ __device__ void post_day(const WgStatic &S, const WgState &X, int64_t mp, int p,
   double red_river = X.red_river[mp];
   const double fswb_old = X.fswb_next[mp];
   const double laf = X.laf[mp];
-  if (S.use_on && contcell) {
+  if (kUse && contcell) {
     const int64_t us = S.use_stride;
     double *U = X.use + mp;
     double rem = X.use_rem[mp];
@@ -723,7 +724,9 @@ struct WgClock {
 // With chunks of 5 pairs the kernel is held to 168 registers, three wavefronts per
 // SIMD, for many members, where it is bandwidth-bound; chunks of 10 leave the
 // compiler free, for one member, where it is latency-bound.
-template <int kSnowChunk>
+// kUse: water use on (the handle's tables are set); the instantiation without it
+// is the model without water use, with no use code at all.
+template <int kSnowChunk, bool kUse>
 __global__ __launch_bounds__(256, kSnowChunk == 5 ? 3 : 1) void wg_cell_kernel(WgStatic S, WgState X, WgTables T,
                                                       const float *__restrict__ forcing, int64_t n_forcing_days,
                                                       const WgClock *__restrict__ clock, int k,
@@ -746,13 +749,13 @@ __global__ __launch_bounds__(256, kSnowChunk == 5 ? 3 : 1) void wg_cell_kernel(W
     // the finished day is day k - 1 of the block (the closing pass runs with k = days)
     int fin_ym = 0;
     bool fin_year_end = false;
-    if (S.use_on) {
+    if (kUse) {
       const int64_t fin = clock->day0 + k - 1;
       const int fin_doy = int((clock->doy0 - 1 + k - 1) % 365) + 1;
       fin_ym = int((fin - S.use_day0) / 365) * 12 + day_info(fin_doy).month;
       fin_year_end = fin_doy == 365;
     }
-    post_day(S, X, mp, p, pd, fin_ym, fin_year_end);
+    post_day<kUse>(S, X, mp, p, pd, fin_ym, fin_year_end);
   } else if (do_vertical) {
     pd.laf = X.laf[mp];
     pd.laf_prev = X.laf_prev[mp];
@@ -794,7 +797,7 @@ __global__ __launch_bounds__(256, kSnowChunk == 5 ? 3 : 1) void wg_cell_kernel(W
   // here for (semi-)arid cells too, whose groundwater step the chain runs)
   double use_desired = 0., use_nag = 0.;
   int64_t use_t = 0;
-  if (S.use_on) {
+  if (kUse) {
     use_t = (int64_t((fday - S.use_day0) / 365) * 12 + day_info(doy).month) * n + p;
     const int64_t us = S.use_stride;
     double *U = X.use + mp;
@@ -1153,7 +1156,7 @@ __global__ __launch_bounds__(256, kSnowChunk == 5 ? 3 : 1) void wg_cell_kernel(W
     // humid cells (routing.cpp:1968-2039) and inland sinks (2112-2159)
     const double gw_prev = ci.x_gw;
     double netGWin = recharge_handoff * cellArea * (laf / 100.) / 1000000.;
-    if (S.use_on) netGWin -= use_nag;  // routing.cpp:1990-1996, 2131-2137
+    if (kUse) netGWin -= use_nag;  // routing.cpp:1990-1996, 2131-2137
     double gw = gw_prev * ci.s_e_gw + ci.s_inv_kgw * netGWin * ci.s_ome_gw;
     if (fabs(gw) <= kMinStorVol) gw = 0.;
     double gw_runoff = gw_prev - gw + netGWin;
@@ -1328,7 +1331,7 @@ __global__ __launch_bounds__(256, kSnowChunk == 5 ? 3 : 1) void wg_cell_kernel(W
       // (routing.cpp:2680-2695)
       const double use_lake = (lk.flags & CFL_RES) ? 0.5 * use_desired : use_desired;
       v[CF_GL_USE] = evap * (lake_area / 1000000.) + gwr * cellArea * (cf / 100.) / 1000000. +
-                     (S.use_on ? use_lake : 0.);
+                     (kUse ? use_lake : 0.);
       v[CF_GL_PREV] = prev;
       v[CF_GL_S0E] = prev * e_sw;
     }
@@ -1374,14 +1377,14 @@ __global__ __launch_bounds__(256, kSnowChunk == 5 ? 3 : 1) void wg_cell_kernel(W
     v[CF_RES_CAP] = res ? ci.s_stor_cap : 0.;
     v[CF_RES_QMEAN] = res ? ci.s_mean_outflow : 0.;
     v[CF_RES_AREA] = res ? ci.s_reservoir_area : 0.;
-    v[CF_RES_PROVREL] = res ? (S.use_on ? S.use_provrel[use_t] : ci.s_prov_rel) : 0.;
+    v[CF_RES_PROVREL] = res ? (kUse ? S.use_provrel[use_t] : ci.s_prov_rel) : 0.;
     v[CF_RES_PAD] = 0.;
   }
   v[CF_USE] = use_desired;
   v[CF_A_NUG] = use_nag;
   if (lk.nu == 0) {
     // no upstream cell: G_riverInflow[n] is still 0 when the cell is routed
-    const ChainOut o = chain_step(c, 0.0, day_info(doy));
+    const ChainOut o = chain_step<kUse>(c, 0.0, day_info(doy));
     store_chain(X, mp, c.flags, o);
     if (lk.station >= 0)
       X.st_daily[(int64_t(m) * WG_DAILY_SLOTS + daily_slot) * S.n_stations + lk.station] = (ldd >= 0) ? o.tv : 0.;
@@ -1397,7 +1400,7 @@ __global__ __launch_bounds__(256, kSnowChunk == 5 ? 3 : 1) void wg_cell_kernel(W
 #pragma unroll
       for (int k = kPrefixPairs; k < CF_USE / 2; k++) dst[int64_t(k) * n] = WgD2{v[2 * k], v[2 * k + 1]};
     }
-    if (S.use_on) dst[int64_t(CF_USE / 2) * n] = WgD2{v[CF_USE], v[CF_A_NUG]};
+    if (kUse) dst[int64_t(CF_USE / 2) * n] = WgD2{v[CF_USE], v[CF_A_NUG]};
   }
 }
 
@@ -1459,6 +1462,7 @@ typedef unsigned int WgU32x4 __attribute__((ext_vector_type(4)));
 // which the hardware answers with zeros without a memory access: every lane issues
 // the same loads whatever its flags, so the count of outstanding loads, which the
 // compiler's waits rely on, does not depend on them.
+template <bool kUse>
 __device__ inline void prefetch_slot_wave(const WgStatic &S, const WgState &X, int64_t base, int p, int fl,
                                           ChainSlot &q) {
   const int64_t n = S.n;
@@ -1482,7 +1486,7 @@ __device__ inline void prefetch_slot_wave(const WgStatic &S, const WgState &X, i
     q.a[2 * k + 1] = __builtin_bit_cast(double, (unsigned long long)w.z | ((unsigned long long)w.w << 32));
   }
   // the water-use pair, for every cell when water use is on
-  if (S.use_on) {
+  if (kUse) {
     const WgD2 u = src[int64_t(CF_USE / 2) * n];
     q.a[CF_USE] = u.x;
     q.a[CF_A_NUG] = u.y;
@@ -1514,6 +1518,7 @@ __device__ inline void prefetch_slot_wave(const WgStatic &S, const WgState &X, i
 // cells come last in their level, so most batches load the prefix alone.  The
 // workgroup's wavefronts share one CU's vector memory pipeline, which bounds the
 // wide levels: every load instruction left out shortens them.
+template <bool kUse>
 __device__ inline void prefetch_slot_batch(const WgStatic &S, const WgState &X, int64_t base, int p, bool suffix,
                                            ChainSlot &q) {
   const int64_t n = S.n;
@@ -1537,7 +1542,7 @@ __device__ inline void prefetch_slot_batch(const WgStatic &S, const WgState &X, 
 #pragma unroll
     for (int k = 2 * kPrefixPairs; k < CF_USE; k++) q.a[k] = 0.;
   }
-  if (S.use_on) {
+  if (kUse) {
     const WgD2 u = src[int64_t(CF_USE / 2) * n];
     q.a[CF_USE] = u.x;
     q.a[CF_A_NUG] = u.y;
@@ -1607,6 +1612,7 @@ __device__ inline void store_or_drop(double *row, int n, bool keep, int i, doubl
 // step, LDS and global stores, the station's daily discharge.  `act` is false for
 // a lane without a cell on this level (its slot holds some other cell of the
 // group, its results are not stored).
+template <bool kUse>
 __device__ __forceinline__ void chain_cell(const WgStatic &S, const WgState &X, double *s_tv, const ChainSlot &q,
                                            bool act, int64_t base, int p, int gs, int m, const DayInfo &d,
                                            int daily_slot) {
@@ -1617,7 +1623,7 @@ __device__ __forceinline__ void chain_cell(const WgStatic &S, const WgState &X, 
     if (u < q.nu) up += s_tv[(q.up[u >> 1] >> ((u & 1) * 16)) & 0xffff];
   ChainRec c;
   slot_record(q, c);
-  const ChainOut o = chain_step(c, up, d);
+  const ChainOut o = chain_step<kUse>(c, up, d);
   const int n = S.n;
   store_or_drop(X.glo_lake + base, n, run && (q.flags & CFL_LAKE), p, o.glo_lake);
   store_or_drop(X.res + base, n, run && (q.flags & CFL_RES), p, o.res);
@@ -1625,7 +1631,7 @@ __device__ __forceinline__ void chain_cell(const WgStatic &S, const WgState &X, 
   store_or_drop(X.glo_wetl + base, n, run && (q.flags & CFL_GWET), p, o.glo_wetl);
   store_or_drop(X.gw + base, n, run && (q.flags & CFL_ARID), p, o.gw);
   if (X.pot_lgw) store_or_drop(X.pot_lgw + base, n, run && (q.flags & CFL_ARID), p, o.lgw);
-  if (X.use_rem) store_or_drop(X.use_rem + base, n, run, p, o.rem);
+  if (kUse) store_or_drop(X.use_rem + base, n, run, p, o.rem);
   store_or_drop(X.river + base, n, run, p, o.river);
   store_or_drop(X.tv + base, n, run, p, o.tv);
   if (run) s_tv[p - gs] = o.tv;
@@ -1647,7 +1653,7 @@ __device__ inline int slot_cell(const int32_t *__restrict__ lo, int lf, int l1, 
 // before any computation, so its latency overlaps the levels in between.  The
 // slots rotate by name, never copied, so no wait on the new loads is forced.
 // Cells beyond the block size (levels wider than the block) load directly.
-template <int kAhead>
+template <int kAhead, bool kUse>
 __device__ __forceinline__ void chain_level(const WgStatic &S, const WgState &X, double *s_tv,
                                             const int32_t *__restrict__ lo, int l, int l1, int64_t base, int gs,
                                             int m, const DayInfo &d, int daily_slot, const ChainSlot &use,
@@ -1656,11 +1662,11 @@ __device__ __forceinline__ void chain_level(const WgStatic &S, const WgState &X,
   const int beg = lo[l], end = lo[l + 1];
   prefetch_slot(S, X, base, slot_cell(lo, l + kAhead, l1, gs), fill);
   const bool act = beg + tid < end;
-  chain_cell(S, X, s_tv, use, act, base, act ? beg + tid : gs, gs, m, d, daily_slot);
+  chain_cell<kUse>(S, X, s_tv, use, act, base, act ? beg + tid : gs, gs, m, d, daily_slot);
   for (int p = beg + tid + nt; p < end; p += nt) {
     ChainSlot q;
     prefetch_slot(S, X, base, p, q);
-    chain_cell(S, X, s_tv, q, true, base, p, gs, m, d, daily_slot);
+    chain_cell<kUse>(S, X, s_tv, q, true, base, p, gs, m, d, daily_slot);
   }
   lds_barrier();
 }
@@ -1670,7 +1676,7 @@ __device__ __forceinline__ void chain_level(const WgStatic &S, const WgState &X,
 // cell kernel's output); the static record holds group-local upstream indices.
 // Each thread's first cell of a level is loaded kAhead levels early (1 or 2), so
 // that the load latency overlaps the levels in between; the slots rotate by name.
-template <int kAhead>
+template <int kAhead, bool kUse>
 __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, WgTables T,
                                                        const int32_t *__restrict__ group_level_off,
                                                        const int32_t *__restrict__ group_first_level,
@@ -1714,14 +1720,14 @@ __global__ __launch_bounds__(256) void wg_chain_kernel(WgStatic S, WgState X, Wg
   lds_barrier();
   if (kAhead == 1) {
     for (int l = l0 + 1; l < l1; l += 2) {
-      chain_level<1>(S, X, s_tv, lo, l, l1, base, gs, m, d, daily_slot, a, b);
-      if (l + 1 < l1) chain_level<1>(S, X, s_tv, lo, l + 1, l1, base, gs, m, d, daily_slot, b, a);
+      chain_level<1, kUse>(S, X, s_tv, lo, l, l1, base, gs, m, d, daily_slot, a, b);
+      if (l + 1 < l1) chain_level<1, kUse>(S, X, s_tv, lo, l + 1, l1, base, gs, m, d, daily_slot, b, a);
     }
   } else {
     for (int l = l0 + 1; l < l1; l += 3) {
-      chain_level<2>(S, X, s_tv, lo, l, l1, base, gs, m, d, daily_slot, a, c);
-      if (l + 1 < l1) chain_level<2>(S, X, s_tv, lo, l + 1, l1, base, gs, m, d, daily_slot, b, a);
-      if (l + 2 < l1) chain_level<2>(S, X, s_tv, lo, l + 2, l1, base, gs, m, d, daily_slot, c, b);
+      chain_level<2, kUse>(S, X, s_tv, lo, l, l1, base, gs, m, d, daily_slot, a, c);
+      if (l + 1 < l1) chain_level<2, kUse>(S, X, s_tv, lo, l + 1, l1, base, gs, m, d, daily_slot, b, a);
+      if (l + 2 < l1) chain_level<2, kUse>(S, X, s_tv, lo, l + 2, l1, base, gs, m, d, daily_slot, c, b);
     }
   }
   (void)T;
@@ -1751,7 +1757,7 @@ __device__ inline void wave_lds_order() {
 // batches' lakeres flags: the next batch's suffix is loaded, and this batch's
 // lake and reservoir stores issued, only when it is 1.  Otherwise a batch of the
 // narrow rest: per-cell record suffix through s_fl, every store issued.
-template <bool kWide>
+template <bool kWide, bool kUse>
 __device__ __forceinline__ void wave_batch(const WgStatic &S, const WgState &X, double *s_tv, int b0, int bend,
                                            int pnext, int ge, int gs, int64_t base, int m, const DayInfo &d,
                                            int daily_slot, const uint8_t *s_fl, int next_lr, int cur_lr,
@@ -1760,9 +1766,9 @@ __device__ __forceinline__ void wave_batch(const WgStatic &S, const WgState &X, 
   {
     const int pf = pnext + lane < ge ? pnext + lane : gs;
     if (kWide)
-      prefetch_slot_batch(S, X, base, pf, next_lr != 0, fill);
+      prefetch_slot_batch<kUse>(S, X, base, pf, next_lr != 0, fill);
     else
-      prefetch_slot_wave(S, X, base, pf, s_fl[pf - gs], fill);
+      prefetch_slot_wave<kUse>(S, X, base, pf, s_fl[pf - gs], fill);
   }
   const int p = b0 + lane;
   const bool have = p < bend;
@@ -1774,7 +1780,7 @@ __device__ __forceinline__ void wave_batch(const WgStatic &S, const WgState &X, 
     const int lv_b = __builtin_amdgcn_readlane(q.level, last);
     ChainRec c;
     slot_record(q, c);
-    const StepPre pre = step_pre(c);
+    const StepPre pre = step_pre<kUse>(c);
     for (int l = lv_a; l <= lv_b; l++) {
       if (have && q.level == l) {
         // upstream transported volumes in routing order (routing.cpp:3955-3958);
@@ -1792,8 +1798,8 @@ __device__ __forceinline__ void wave_batch(const WgStatic &S, const WgState &X, 
           for (int u = 2; u < 8; u++) up += (u < q.nu) ? t[u] : 0.;
         }
         bool slow;
-        o = step_fast(c, pre, up, slow);
-        if (slow) o = chain_step(c, up, d);
+        o = step_fast<kUse>(c, pre, up, slow);
+        if (slow) o = chain_step<kUse>(c, up, d);
         run = (q.flags & CFL_CONT) != 0;
         s_tv[p - gs] = run ? o.tv : 0.;
       }
@@ -1809,7 +1815,7 @@ __device__ __forceinline__ void wave_batch(const WgStatic &S, const WgState &X, 
   store_or_drop(X.glo_wetl + base, n, run && (q.flags & CFL_GWET), p, o.glo_wetl);
   store_or_drop(X.gw + base, n, run && (q.flags & CFL_ARID), p, o.gw);
   if (X.pot_lgw) store_or_drop(X.pot_lgw + base, n, run && (q.flags & CFL_ARID), p, o.lgw);
-  if (X.use_rem) store_or_drop(X.use_rem + base, n, run, p, o.rem);
+  if (kUse) store_or_drop(X.use_rem + base, n, run, p, o.rem);
   store_or_drop(X.river + base, n, run, p, o.river);
   store_or_drop(X.tv + base, n, run, p, o.tv);
   const double qq = (run && (q.flags & CFL_LDD)) ? o.tv : 0.;
@@ -1817,7 +1823,7 @@ __device__ __forceinline__ void wave_batch(const WgStatic &S, const WgState &X, 
                 have && q.station >= 0, q.station, qq);
 }
 
-template <int kWaves>
+template <int kWaves, bool kUse>
 __global__ __launch_bounds__(64 * kWaves) void wg_chain_wave_kernel(WgStatic S, WgState X, WgTables T,
                                                             const int32_t *__restrict__ group_level_off,
                                                             const int32_t *__restrict__ group_first_level,
@@ -1880,7 +1886,7 @@ __global__ __launch_bounds__(64 * kWaves) void wg_chain_wave_kernel(WgStatic S, 
   WgWaveItem ia = item(it0);
   if (la > 0) {
     const int pf = ia.start + lane < ge ? ia.start + lane : gs;
-    prefetch_slot_batch(S, X, base, pf, ia.lakeres != 0, s0);
+    prefetch_slot_batch<kUse>(S, X, base, pf, ia.lakeres != 0, s0);
   }
   __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   lds_barrier();
@@ -1892,13 +1898,13 @@ __global__ __launch_bounds__(64 * kWaves) void wg_chain_wave_kernel(WgStatic S, 
         lds_barrier();
         lev++;
       }
-      wave_batch<true>(S, X, s_tv, ia.start, ia.end, ib.start, ge, gs, base, m, d, daily_slot, s_fl, ib.lakeres,
+      wave_batch<true, kUse>(S, X, s_tv, ia.start, ia.end, ib.start, ge, gs, base, m, d, daily_slot, s_fl, ib.lakeres,
                  ia.lakeres, s0, s1);
       while (lev < ib.level) {
         lds_barrier();
         lev++;
       }
-      wave_batch<true>(S, X, s_tv, ib.start, ib.end, ic.start, ge, gs, base, m, d, daily_slot, s_fl, ic.lakeres,
+      wave_batch<true, kUse>(S, X, s_tv, ib.start, ib.end, ic.start, ge, gs, base, m, d, daily_slot, s_fl, ic.lakeres,
                  ib.lakeres, s1, s0);
       ia = ic;
     }
@@ -1912,13 +1918,13 @@ __global__ __launch_bounds__(64 * kWaves) void wg_chain_wave_kernel(WgStatic S, 
   const int nb = (ge - pb + 127) / 128 * 2;
   {
     const int pa = pb + lane < ge ? pb + lane : gs;
-    prefetch_slot_wave(S, X, base, pa, s_fl[pa - gs], s0);
+    prefetch_slot_wave<kUse>(S, X, base, pa, s_fl[pa - gs], s0);
   }
   __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   for (int b = 0; b < nb; b += 2) {
     const int b0 = pb + b * 64;
-    wave_batch<false>(S, X, s_tv, b0, ge, b0 + 64, ge, gs, base, m, d, daily_slot, s_fl, 1, 1, s0, s1);
-    wave_batch<false>(S, X, s_tv, b0 + 64, ge, b0 + 128, ge, gs, base, m, d, daily_slot, s_fl, 1, 1, s1, s0);
+    wave_batch<false, kUse>(S, X, s_tv, b0, ge, b0 + 64, ge, gs, base, m, d, daily_slot, s_fl, 1, 1, s0, s1);
+    wave_batch<false, kUse>(S, X, s_tv, b0 + 64, ge, b0 + 128, ge, gs, base, m, d, daily_slot, s_fl, 1, 1, s1, s0);
   }
   (void)T;
 }
@@ -1945,6 +1951,7 @@ __device__ inline bool flow_next(const int32_t *s_lo, int l1_rel, int nt, int &l
   return l < l1_rel;
 }
 
+template <bool kUse>
 __global__ __launch_bounds__(256) void wg_chain_flow_kernel(WgStatic S, WgState X, WgTables T,
                                                             const int32_t *__restrict__ group_level_off,
                                                             const int32_t *__restrict__ group_first_level,
@@ -2011,7 +2018,7 @@ __global__ __launch_bounds__(256) void wg_chain_flow_kernel(WgStatic S, WgState 
       const int64_t mp = base + p;
       ChainRec c;
       slot_record(cur, c);
-      const ChainOut o = chain_step(c, up, d);
+      const ChainOut o = chain_step<kUse>(c, up, d);
       vtv[p - gs] = __double_as_longlong(o.tv);
       store_chain(X, mp, cur.flags, o);
       qq = (cur.flags & CFL_LDD) ? o.tv : 0.;

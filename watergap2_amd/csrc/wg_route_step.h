@@ -178,6 +178,8 @@ WG_RS_FN double arid_groundwater(double gwr_swb, double area, double cf100, doub
 // (CF_USE) is shared between a global lake and a reservoir, taken from each
 // (routing.cpp:2678-2716, 2866-2945), and what is left goes to the river; without
 // water use it is +0 throughout, which leaves every result unchanged.
+// kUse false: water use off, compiled with the use terms at +0.
+template <bool kUse = true>
 WG_RS_FN ChainOut chain_step(const ChainRec &c, double inflow_up, const DayInfo &d) {
   const int month = d.month;
   ChainOut o;
@@ -195,7 +197,7 @@ WG_RS_FN ChainOut chain_step(const ChainRec &c, double inflow_up, const DayInfo 
   o.k_release = 0.;
 
   inflow += inflow_up;  // routing.cpp:2623
-  double remainingUse = v[CF_USE], remainingUseGloLake = 0., remainingUseRes = 0.;
+  double remainingUse = kUse ? v[CF_USE] : 0., remainingUseGloLake = 0., remainingUseRes = 0.;
 
   // global lake, routing.cpp:2630-2804; CF_GL_USE holds the lake's share of the use
   if (c.flags & CFL_LAKE) {
@@ -345,7 +347,7 @@ WG_RS_FN ChainOut chain_step(const ChainRec &c, double inflow_up, const DayInfo 
   if (arid_swb) {
     const double gwr_swb = v[CF_A_GWR_LL] + gwr_glolak + v[CF_A_GWR_LW] + gwr_glowet + gwr_res;
     lgwrir = arid_groundwater(gwr_swb, c.area, c.cf100, v[CF_A_RECH], v[CF_A_PREV], v[CF_A_S0E], c.inv_kgw,
-                              c.ome_gw, v[CF_A_NUG], o.gw);
+                              c.ome_gw, kUse ? v[CF_A_NUG] : 0., o.gw);
   }
 
   // river, routing.cpp:3402-3514
@@ -373,6 +375,7 @@ struct StepPre {
   double lgwrir, gw;
 };
 
+template <bool kUse = true>
 WG_RS_FN StepPre step_pre(const ChainRec &c) {
   const double *v = c.v;
   StepPre r{v[CF_LGWRIR], v[CF_A_PREV]};
@@ -381,11 +384,12 @@ WG_RS_FN StepPre step_pre(const ChainRec &c) {
     const double gwr_glolak = (c.flags & CFL_LAKE) ? v[CF_GL_GWR] : 0.;
     const double gwr_swb = v[CF_A_GWR_LL] + gwr_glolak + v[CF_A_GWR_LW] + v[CF_GW_GWR] + 0.;
     r.lgwrir = arid_groundwater(gwr_swb, c.area, c.cf100, v[CF_A_RECH], v[CF_A_PREV], v[CF_A_S0E], c.inv_kgw,
-                                c.ome_gw, v[CF_A_NUG], r.gw);
+                                c.ome_gw, kUse ? v[CF_A_NUG] : 0., r.gw);
   }
   return r;
 }
 
+template <bool kUse = true>
 WG_RS_FN ChainOut step_fast(const ChainRec &c, const StepPre &pre, double inflow_up, bool &slow) {
   const double *v = c.v;
   ChainOut o;
@@ -439,7 +443,7 @@ WG_RS_FN ChainOut step_fast(const ChainRec &c, const StepPre &pre, double inflow
     rin = (c.flags & CFL_LDD) ? rin_ldd : rin;
     rin += v[CF_RV_PREC];
     // a global lake that keeps water leaves no use (routing.cpp:2752); no reservoir here
-    const double remainingUse = (c.flags & CFL_LAKE) ? 0. : v[CF_USE];
+    const double remainingUse = (kUse && !(c.flags & CFL_LAKE)) ? v[CF_USE] : 0.;
     const double evapRemUse = remainingUse + v[CF_RV_EVAP];
     const double evapRemUseMax = rin + v[CF_RV_TERM];
     double tv_dry = rin + prev - evapRemUseMax;
