@@ -25,6 +25,8 @@ import math
 import os
 
 import numpy as np
+
+import test_water_use_oracle as wu
 import pytest
 
 from oracle import wg_oracle
@@ -836,6 +838,20 @@ MUTANTS = [
     ("threshold elevation ignored",
      [("          temp_elev = dailyTempC - ((thresh_elev - elev[0]) * T_GRADNT);", "          ;")],
      check_threshold_elevation),
+    # water use (the checks live with the other water-use pins)
+    ("return-flow factor without the groundwater fraction",
+     [("      factor = 1 - (1 - frgi) * (1 - eff);", "      factor = 1 - frgi * (1 - eff);")],
+     wu.check_net_abstraction_gw),
+    ("reduced return flow not given back",
+     [("    returnflowChange = (reintroducedRatio * USE(WG_USE_REDUCED_RETURN));",
+       "    returnflowChange = 0.;")], wu.check_net_abstraction_gw),
+    ("year end adds last year's unsatisfied use",
+     [("      USE(WG_USE_TOTAL_UNSAT) -= USE(WG_USE_UNSAT_PREV_YEAR);",
+       "      USE(WG_USE_TOTAL_UNSAT) += USE(WG_USE_UNSAT_PREV_YEAR);")], wu.check_year_end),
+    ("reservoir use added, not taken",
+     [("            m->res[i] -= remainingUseRes;\n            remainingUseRes = 0.;\n          } else {",
+       "            m->res[i] += remainingUseRes;\n            remainingUseRes = 0.;\n          } else {")],
+     wu.check_reservoir_use),
 ]
 
 

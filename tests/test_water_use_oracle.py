@@ -12,6 +12,8 @@ routing.cpp:966-972), with the NETABS multipliers at 1.
 import numpy as np
 import pytest
 
+from oracle import wg_oracle
+
 DAYS = np.array([31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31], dtype=np.float64)
 
 
@@ -53,3 +55,158 @@ def test_zero_use_equals_the_model_without_water_use(wg, oracle):
             b.water_use_year_end()
     assert _states_equal(a.get_state(), b.get_state()) == []
     assert not np.any(b.water_use_state())
+
+
+def _humid_cells(w, k):
+    a = np.asarray(w.aindex)
+    ok = (a == 0) & (np.asarray(w.ldd) >= 0) & (np.asarray(w.contfreq) > 0)
+    return np.flatnonzero(ok)[:k]
+
+
+# (DRU, WU, CU, frgi, UI, UO, RRF) -> (UI, UO, RRF, change of the abstraction) after
+# updateNetAbstractionGW, worked by hand (routing.cpp:5516-5570); km3
+NAG_CASES = [
+    # unsatisfied use below the irrigation share: eff 0.5, frgi 0.25, factor
+    # 1 - 0.75 x 0.5 = 0.625, WUsiNew (2.5e-3 - 1.5e-3) / 0.625 = 1.6e-3, return
+    # flow change 0.25 x 0.5 x (1.6e-3 - 4e-3) = -3e-4
+    ((1.5e-3, 4e-3, 2e-3, 0.25, 0, 0, 0), (1.5e-3, 0, -3e-4, 3e-4)),
+    # above it: WUsiNew 0, 2.5e-3 from irrigation, 2.5e-3 from other sectors,
+    # return flow change 0.125 x (0 - 4e-3)
+    ((5e-3, 4e-3, 2e-3, 0.25, 0, 0, 0), (2.5e-3, 2.5e-3, -5e-4, 5e-4)),
+    # no irrigation withdrawal: all from other sectors, abstraction unchanged
+    ((2e-3, 0, 0, 0.5, 0, 1e-3, 0), (0, 3e-3, 0, 0)),
+    # delayed satisfaction covered by the other sectors' deficit
+    ((-1e-3, 4e-3, 2e-3, 0.5, 0, 3e-3, 0), (0, 2e-3, 0, 0)),
+    # ... beyond it: half the reduced return flow given back
+    ((-3e-3, 4e-3, 2e-3, 0.5, 4e-3, 1e-3, -2e-3), (2e-3, 0, -1e-3, -1e-3)),
+    # ratio below -1 clamped: all given back
+    ((-3e-3, 4e-3, 2e-3, 0.5, 1e-3, 0, -2e-3), (0, 0, 0, -2e-3)),
+]
+
+
+def test_net_abstraction_gw_known_answers(wg):
+    check_net_abstraction_gw(wg)
+
+
+def check_net_abstraction_gw(wg, variant="wg"):
+    """One day from set states: the unsatisfied-use sums after updateNetAbstractionGW
+    match the hand-worked values, and the groundwater storage equals that of a run
+    whose net abstraction from groundwater is the hand-worked adapted one."""
+    w = _world(wg)
+    n = w.n
+    cells = _humid_cells(w, len(NAG_CASES))
+    assert len(cells) == len(NAG_CASES)
+    f = w.forcing(seed=2, day0=0, n_days=1)
+    nug = np.full(n, 1e-3)
+    frgi = np.zeros(n)
+    z = np.zeros((n, 12))
+    st = np.zeros((wg._abi.WG_USE_NFIELDS, n))
+    nug_b = nug.copy()
+    F = {name: j for j, name in enumerate(wg._abi.USE_FIELDS)}
+    for c, ((dru, wu, cu, fr, ui, uo, rrf), (_, _, _, delta)) in zip(cells, NAG_CASES):
+        st[F["daily_remaining"], c], st[F["withdrawal_irrig"], c], st[F["consumptive_irrig"], c] = dru, wu, cu
+        st[F["unsat_irrig"], c], st[F["unsat_other"], c], st[F["reduced_return"], c] = ui, uo, rrf
+        frgi[c] = fr
+        nug_b[c] = nug[c] + delta
+    a, b = wg_oracle.Oracle(w, variant), wg_oracle.Oracle(w, variant)
+    for o, g, s in ((a, nug, st), (b, nug_b, np.zeros_like(st))):
+        o.set_water_use(frgi, delayed=0)
+        o.water_use_year(z, _monthly(g), z, z)
+        o.set_water_use_state(s)
+        o.step_day(f[0, 0], f[0, 1], f[0, 2], f[0, 3], 1)
+    ua = a.water_use_state()
+    for c, (_, (ui, uo, rrf, _)) in zip(cells, NAG_CASES):
+        np.testing.assert_allclose([ua[F["unsat_irrig"], c], ua[F["unsat_other"], c], ua[F["reduced_return"], c]],
+                                   [ui, uo, rrf], rtol=1e-12, atol=1e-18)
+    ga, gb = a.get_state()["groundwater"][cells], b.get_state()["groundwater"][cells]
+    np.testing.assert_allclose(ga, gb, rtol=1e-12, atol=1e-18)
+    # and the abstraction matters: without the adaptation the storages differ
+    assert np.abs(ga[[0, 1, 4, 5]] - oracle_gw_plain(variant, w, f, nug, frgi, cells)).min() > 1e-6
+
+
+def oracle_gw_plain(variant, w, f, nug, frgi, cells):
+    o = wg_oracle.Oracle(w, variant)
+    z = np.zeros((w.n, 12))
+    o.set_water_use(frgi, delayed=0)
+    o.water_use_year(z, _monthly(nug), z, z)
+    o.step_day(f[0, 0], f[0, 1], f[0, 2], f[0, 3], 1)
+    return o.get_state()["groundwater"][cells][[0, 1, 4, 5]]
+
+
+# (TU, UI, UO, RRF, TU_py, UI_py, UO_py, RRF_py) -> the same after the year end,
+# worked by hand (routing.cpp:5257-5296)
+YEAR_END_CASES = [
+    ((1, 2, 3, -1, 2, 0, 0, 0), (0, 0, 0, 0, 1, 2, 3, -1)),               # less unsatisfied than last year
+    ((3, 2, 1, -1, 1, 0.5, 0.25, -0.5), (2, 1.5, 0.75, -0.5, 1, 0.5, 0.25, -0.5)),
+    ((3, 1, 0.5, -1, 1, 3, 0, 0), (2, 0, 0, 0, 1, 3, 0, 0)),              # sum negative: cleared
+    ((3, 3, 0, -1, 1, 0, 1, 0), (2, 2, 0, -1, 1, 0, 1, 0)),               # other sectors negative
+    ((3, 0, 3, -2, 1, 1, 0, 0), (2, 0, 2, 0, 1, 1, 0, 0)),                # irrigation negative
+    ((3, 2, 0, 1, 1, 0, 0, 0), (2, 0, 0, 0, 1, 0, 0, 0)),                 # return flow positive
+]
+
+
+def test_year_end_known_answers(wg):
+    check_year_end(wg)
+
+
+def check_year_end(wg, variant="wg"):
+    w = _world(wg)
+    n = w.n
+    o = wg_oracle.Oracle(w, variant)
+    o.set_water_use(np.zeros(n), delayed=1)
+    F = {name: j for j, name in enumerate(wg._abi.USE_FIELDS)}
+    keys = ["total_unsat", "unsat_irrig", "unsat_other", "reduced_return", "unsat_prev_year",
+            "unsat_irrig_prev_year", "unsat_other_prev_year", "reduced_return_prev_year"]
+    st = np.zeros((wg._abi.WG_USE_NFIELDS, n))
+    for c, (before, _) in enumerate(YEAR_END_CASES):
+        for k, v in zip(keys, before):
+            st[F[k], c] = v
+    o.set_water_use_state(st)
+    o.water_use_year_end()
+    got = o.water_use_state()
+    for c, (_, after) in enumerate(YEAR_END_CASES):
+        assert [got[F[k], c] for k in keys] == list(after), c
+
+
+def test_reservoir_gives_up_the_use_it_can_satisfy(wg):
+    check_reservoir_use(wg)
+
+
+def check_reservoir_use(wg, variant="wg"):
+    """A type-2 reservoir half full: one day's use U taken from it leaves the
+    storage U lower than the same day without use, the release (a rule of the
+    mean outflow and the release coefficient) and everything downstream
+    unchanged, and no unsatisfied use."""
+    w = _world(wg)
+    n = w.n
+    res = np.flatnonzero((np.asarray(w.reservoir_area) > 0) & (np.asarray(w.res_type) == 2) &
+                         (np.asarray(w.lake_area) == 0))
+    assert len(res) > 0
+    f = w.forcing(seed=2, day0=0, n_days=1)
+    cap = np.asarray(w.stor_cap, dtype=np.float64)
+    z = np.zeros((n, 12))
+
+    def run(u):
+        o = wg_oracle.Oracle(w, variant)
+        s = o.get_state()
+        s["reservoir"][res] = 0.5 * cap[res]
+        o.set_state(s)
+        o.set_water_use(np.zeros(n), delayed=0)
+        o.water_use_year(_monthly(u), z, z, z)
+        # day 2: not the first of a month, when the release coefficient of a
+        # reservoir whose year starts then is set from its storage (routing.cpp:2956-2963)
+        o.step_day(f[0, 0], f[0, 1], f[0, 2], f[0, 3], 2)
+        return o
+
+    # reservoirs that do not spill over their capacity on this day (a spill would
+    # absorb part of the use)
+    plain = run(np.zeros(n))
+    res = res[plain.get_state()["reservoir"][res] < 0.9 * cap[res]]
+    assert len(res) > 0
+    use = np.zeros(n)
+    use[res] = 0.01 * cap[res]
+    runs = [plain, run(use)]
+    sb, sa = runs[0].get_state(), runs[1].get_state()
+    np.testing.assert_allclose(sb["reservoir"][res] - sa["reservoir"][res], use[res], rtol=1e-9)
+    assert np.array_equal(runs[0].day_outputs()["discharge"], runs[1].day_outputs()["discharge"])
+    assert not np.any(runs[1].water_use_state()[0][res])
