@@ -21,6 +21,9 @@ Besides the JSON line's value, the run reports
     summary is present under profiles/;
   * deep: the same measurement on a global grid of 352 routing levels (60 ocean
     outlets), where the routing chain is three times as deep;
+  * water_use: the global grid with water use (subtract_use 2, use_alloc 0,
+    delayed satisfaction) from seeded monthly net-use tables, with the same
+    loop without water use timed beside it (--water-use runs this leg alone);
   * ensemble: 256 perturbed members of the global grid for one year;
   * calibration: basins calibrated per hour at the 1 % criterion with the
     reference's gamma search (120 synthetic basins, 30 years), and 64 candidates
@@ -430,6 +433,47 @@ def deep_run(wg, args):
             "chain_kernel_achieved_GBs": round(chain_b / (leg["chain_ms"] * 1e-3) / 1e9, 1)}
 
 
+def water_use_run(wg, world, forcing, n_years=1):
+    """The day loop with water use (subtract_use 2, use_alloc 0, delayed
+    satisfaction) on the global grid: seeded monthly net-use tables, with return
+    flows and a tenth of the cells using twenty times more, one warm-up year
+    (graph capture), then n_years timed; the same loop without water use beside it."""
+    n = world.n
+    rng = np.random.default_rng(11)
+    days = np.array([31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31], dtype=np.float64)
+    scale = np.asarray(world.cell_area, dtype=np.float64) * 2e-6
+    years = 1 + n_years
+    nus = rng.uniform(-0.3, 1.5, (years, n, 12)) * scale[None, :, None]
+    nus[:, rng.random(n) < 0.1, :] *= 20.0
+    nug = rng.uniform(0.0, 0.5, (years, n, 12)) * scale[None, :, None]
+    wu = rng.uniform(0.0, 2.0, (years, n, 12)) * scale[None, :, None]
+    cu = wu * rng.uniform(0.2, 0.9, (years, n, 12))
+    m3 = 1e9 * days[None, None, :]
+    out = {}
+    for label, opts in (("with_use", {"subtract_use": 2}), ("without_use", None)):
+        model = wg.WaterBalanceModel(world, options=opts)
+        model.set_stations(world.basin_outlet)
+        if opts:
+            model.set_water_use(nus * m3, nug * m3, wu * m3, cu * m3, rng.uniform(0.0, 1.0, n), delayed=1)
+        model.load_forcing(forcing[: 365 * years])
+        model.step_days(0, 1, 365)
+        model.synchronize()
+        t0 = time.perf_counter()
+        for y in range(1, years):
+            model.step_days(365 * y, 1, 365)
+        model.synchronize()
+        out[label] = time.perf_counter() - t0
+        model.close()
+    d = 365 * n_years
+    return {"metric": "cell-days/s at global 0.5deg (FP64), water use without allocation",
+            "value": round(n * d / out["with_use"], 1), "unit": "cell-days/s",
+            "us_per_day": round(out["with_use"] / d * 1e6, 2),
+            "us_per_day_without_use": round(out["without_use"] / d * 1e6, 2),
+            "config": {"workload": f"global 0.5deg synthetic land mask, {n} cells, seed 1, {n_years} year(s) of "
+                                   "daily steps, seeded monthly net-use tables, delayed satisfaction",
+                       "cells": n}}
+
+
 def traffic_from_profiles(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprof counter summary
     (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration), or None."""
@@ -466,6 +510,9 @@ def main():
     ap.add_argument("--sur-rounds", default="3,3,2,3",
                     help="surrogate's later rounds: correction rounds, proposals per basin in each, "
                          "bracket rounds, points per basin in each")
+    ap.add_argument("--water-use", action="store_true",
+                    help="only the water-use leg (global grid with seeded net-use tables)")
+    ap.add_argument("--wu-years", type=int, default=1, help="timed years of the water-use leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch configuration (profiling runs)")
     ap.add_argument("--ensemble-members", type=int, default=256,
@@ -511,6 +558,12 @@ def main():
         return
     if args.surrogate:
         print(json.dumps(surrogate_run(args)))
+        return
+    if args.water_use:
+        import watergap2_amd as wg
+        w = wg.World(wg.global_spec(seed=1))
+        print(json.dumps(water_use_run(wg, w, w.forcing(seed=1, day0=0, n_days=365 * (1 + args.wu_years)),
+                                       args.wu_years)))
         return
 
     if args.gpus != 1:
@@ -591,6 +644,12 @@ def main():
         print(f"# deep network: {deep['value']:.4g} cell-days/s, {deep['config']['routing_levels']} levels, "
               f"chain {deep['wg_chain_kernel_us']} us/day", file=sys.stderr, flush=True)
 
+    use = None
+    if not args.no_extras and args.wu_years > 0 and forcing.shape[0] >= 365 * (1 + args.wu_years):
+        use = water_use_run(wg, world, forcing, args.wu_years)
+        print(f"# water use: {use['value']:.4g} cell-days/s, {use['us_per_day']} us/day "
+              f"({use['us_per_day_without_use']} without)", file=sys.stderr, flush=True)
+
     ens = None
     if not args.no_extras and args.ensemble_members > 0:
         ens = ensemble_run(wg, world, forcing if forcing.shape[0] >= 365 else world.forcing(seed=1, day0=0, n_days=365),
@@ -635,6 +694,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "deep": deep,
+        "water_use": use,
         "ensemble": ens,
         "calibration": cal,
         "surrogate": sur,
