@@ -122,3 +122,25 @@ def test_refused_calls(wg, regional_edge_world):
         m.step_days(30, 31, 5)
     m.set_water_use(z, z, z, z, np.zeros(w.n), year0_day=20)
     m.step_days(30, 31, 5)
+
+
+def test_water_use_global_grid_two_partitions(wg, oracle, global_world):
+    """The global grid (67,420 cells) with water use, its deepest routing groups
+    run as a second partition beside the others: 380 days across a year end,
+    every cell and every water-use field against the oracle."""
+    w = global_world
+    n_days = 380
+    print(f"case: water use, global grid, two partitions, {n_days} days", flush=True)
+    tabs, frgi = use_tables(w, seed=13)
+    f = w.forcing(seed=1, day0=0, n_days=n_days)
+    m = wg.WaterBalanceModel(w, options={"subtract_use": 2})
+    assert len(m.partitions()) == 2
+    m.set_water_use(*tabs, frgi, year0_day=0, delayed=1)
+    m.load_forcing(f)
+    m.step_days(0, 1, 200)
+    m.step_days(200, 201, n_days - 200)
+    o = oracle_run(oracle, w, f, tabs, frgi, 1, n_days)
+    gs, os_ = m.get_state(), o.get_state()
+    diff = [key for key in os_ if key != "started" and not np.array_equal(np.asarray(gs[key]), os_[key])]
+    assert diff == [], diff
+    assert np.array_equal(m.water_use_state(), o.water_use_state())
