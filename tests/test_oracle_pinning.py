@@ -852,6 +852,13 @@ MUTANTS = [
      [("            m->res[i] -= remainingUseRes;\n            remainingUseRes = 0.;\n          } else {",
        "            m->res[i] += remainingUseRes;\n            remainingUseRes = 0.;\n          } else {")],
      wu.check_reservoir_use),
+    ("river not asked for the use",
+     [("  double evapRemUse = remainingUse + riverEvapo;", "  double evapRemUse = 0. + riverEvapo;")],
+     wu.check_exhausting_use),
+    ("local lake drawn to 0, not -Smax",
+     [("        remainingUse -= (maxStorage + m->loc_lake[i]);\n        m->loc_lake[i] = (-1.) * maxStorage;",
+       "        remainingUse -= (maxStorage + m->loc_lake[i]);\n        m->loc_lake[i] = 0.;")],
+     wu.check_exhausting_use),
 ]
 
 

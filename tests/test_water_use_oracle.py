@@ -210,3 +210,38 @@ def check_reservoir_use(wg, variant="wg"):
     np.testing.assert_allclose(sb["reservoir"][res] - sa["reservoir"][res], use[res], rtol=1e-9)
     assert np.array_equal(runs[0].day_outputs()["discharge"], runs[1].day_outputs()["discharge"])
     assert not np.any(runs[1].water_use_state()[0][res])
+
+
+def test_exhausting_use_empties_river_then_local_lake(wg):
+    check_exhausting_use(wg)
+
+
+def check_exhausting_use(wg, variant="wg"):
+    """A use far beyond what any cell holds (1,000 and 2,000 km3/day): every river
+    without a global lake or reservoir ends the day empty, its outflow the same for
+    both uses (routing.cpp:3483-3492: the evaporation and the use share what the
+    river can give, the transported volume is what is left of the inflow); then
+    the local lake is drawn to its floor -Smax (routing.cpp:3596-3604), where its
+    area reduction factor is exactly 0, the same for both uses."""
+    w = _world(wg)
+    n = w.n
+    f = w.forcing(seed=2, day0=0, n_days=1)
+    cells = np.flatnonzero((np.asarray(w.lake_area) == 0) & (np.asarray(w.reservoir_area) == 0) &
+                           (np.asarray(w.ldd) >= 0) & (np.asarray(w.loc_lake) > 0))
+    assert len(cells) > 1000
+    z = np.zeros((n, 12))
+    runs = []
+    for u in (1000.0, 2000.0):
+        o = wg_oracle.Oracle(w, variant)
+        o.set_water_use(np.zeros(n), delayed=0)
+        o.water_use_year(_monthly(np.full(n, u)), z, z, z)
+        o.step_day(f[0, 0], f[0, 1], f[0, 2], f[0, 3], 2)
+        runs.append((o.get_state(), o.day_outputs()["discharge"], o.water_use_state()[0]))
+    (s1, q1, tu1), (s2, q2, tu2) = runs
+    assert not np.any(s1["river"][cells]) and not np.any(s2["river"][cells])
+    assert np.array_equal(q1[cells], q2[cells])
+    assert (s1["loc_lake"][cells] < 0).all()
+    assert np.array_equal(s1["loc_lake"][cells], s2["loc_lake"][cells])
+    assert not np.any(s1["red_loc_lake"][cells]) and not np.any(s2["red_loc_lake"][cells])
+    # what was not satisfied is kept: the larger use leaves more unsatisfied
+    assert (tu2[cells] > tu1[cells]).all()
