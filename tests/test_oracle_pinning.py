@@ -859,6 +859,10 @@ MUTANTS = [
      [("        remainingUse -= (maxStorage + m->loc_lake[i]);\n        m->loc_lake[i] = (-1.) * maxStorage;",
        "        remainingUse -= (maxStorage + m->loc_lake[i]);\n        m->loc_lake[i] = 0.;")],
      wu.check_exhausting_use),
+    ("use served before the river's evaporation",
+     [("      remainingUse -= remainingUse * evapRemUseMax / evapRemUse;",
+       "      remainingUse -= (evapRemUseMax < remainingUse ? evapRemUseMax : remainingUse);")],
+     wu.check_river_share),
 ]
 
 

@@ -245,3 +245,52 @@ def check_exhausting_use(wg, variant="wg"):
     assert not np.any(s1["red_loc_lake"][cells]) and not np.any(s2["red_loc_lake"][cells])
     # what was not satisfied is kept: the larger use leaves more unsatisfied
     assert (tu2[cells] > tu1[cells]).all()
+
+
+def test_river_shares_its_water_between_evaporation_and_use(wg):
+    check_river_share(wg)
+
+
+def check_river_share(wg, variant="wg"):
+    """Cells with a river that cannot meet evaporation E plus use U give both the
+    same fraction of what they can give, Max (routing.cpp:3483-3496): the use taken
+    is U Max / (U + E), so 1 / taken = 1 / Max + (E / Max) / U.  Three uses per
+    cell (no local lake, so the unsatisfied use is what the river left): the line
+    through two predicts the third, and its slope gives E > 0 (a use served
+    before the evaporation would give a slope of 0)."""
+    w = _world(wg)
+    n = w.n
+    f = w.forcing(seed=2, day0=0, n_days=1)
+    w.loc_lake[:] = 0.0  # the river's leftover is then the unsatisfied use
+    # headwater cells: no upstream water, so what the river can give does not
+    # change with the use of other cells
+    down = np.asarray(w.downstream)
+    has_up = np.zeros(n, bool)
+    has_up[down[down > 0] - 1] = True
+    cells = np.flatnonzero((np.asarray(w.lake_area) == 0) & (np.asarray(w.reservoir_area) == 0) &
+                           (np.asarray(w.ldd) >= 0) & ~has_up)
+    z = np.zeros((n, 12))
+
+    def run(u):
+        o = wg_oracle.Oracle(w, variant)
+        o.set_water_use(np.zeros(n), delayed=0)
+        o.water_use_year(_monthly(u), z, z, z)
+        o.step_day(f[0, 0], f[0, 1], f[0, 2], f[0, 3], 2)
+        return o
+
+    base = run(np.zeros(n))
+    s0 = base.get_state()
+    # uses of 2, 4 and 8 times what the river held and passed on without use
+    cap = s0["river"] + base.day_outputs()["discharge"]
+    us = [k * np.maximum(cap, 1e-9) for k in (2.0, 4.0, 8.0)]
+    taken = [u - run(u).water_use_state()[0] for u in us]
+    sel = cells[(taken[0][cells] > 1e-12)]
+    assert len(sel) > 100
+    x = [1.0 / u[sel] for u in us]
+    y = [1.0 / t[sel] for t in taken]
+    slope = (y[1] - y[0]) / (x[1] - x[0])
+    icpt = y[0] - slope * x[0]
+    np.testing.assert_allclose(icpt + slope * x[2], y[2], rtol=1e-8)
+    # E >= 0 everywhere (0 where the river has no evaporation that day), > 0 in most cells
+    e = slope / icpt
+    assert (e > -1e-15).all() and (e > 1e-12).mean() > 0.5
