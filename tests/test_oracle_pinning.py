@@ -863,6 +863,10 @@ MUTANTS = [
      [("      remainingUse -= remainingUse * evapRemUseMax / evapRemUse;",
        "      remainingUse -= (evapRemUseMax < remainingUse ? evapRemUseMax : remainingUse);")],
      wu.check_river_share),
+    ("global lake without its use",
+     [("    use = evap * ((double)st->lake_area[i] / 1000000.) + gwr_glolak * cellArea * (cf / 100.) / 1000000. +\n          remainingUseGloLake;",
+       "    use = evap * ((double)st->lake_area[i] / 1000000.) + gwr_glolak * cellArea * (cf / 100.) / 1000000. +\n          0.;")],
+     wu.check_global_lake_use),
 ]
 
 

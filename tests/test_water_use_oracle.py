@@ -294,3 +294,51 @@ def check_river_share(wg, variant="wg"):
     # E >= 0 everywhere (0 where the river has no evaporation that day), > 0 in most cells
     e = slope / icpt
     assert (e > -1e-15).all() and (e > 1e-12).mean() > 0.5
+
+
+def test_global_lake_takes_its_use(wg):
+    check_global_lake_use(wg)
+
+
+def check_global_lake_use(wg, variant="wg"):
+    """A global lake (no reservoir in the cell) below its maximum that can meet its
+    use U: the day's storage drops by U (1 - e^-k) / k, the linear-reservoir solution
+    dS/dt = I - PET - U - k S over one day (routing.cpp:2723-2728), or by U exactly
+    where the outflow falls to 0 and the storage balance has no outflow term
+    (2740-2751); nothing is left unsatisfied.  One lake at a time, so no other
+    cell's water changes."""
+    w = _world(wg)
+    n = w.n
+    f = w.forcing(seed=2, day0=0, n_days=1)
+    lakes = np.flatnonzero((np.asarray(w.lake_area) > 0) & (np.asarray(w.reservoir_area) == 0))
+    k_sw = np.asarray(w.params)[wg._abi.PARAM_INDEX["P_SWOUTF_C"]]
+    z = np.zeros((n, 12))
+
+    # the model starts its lakes full (setLakeWetlToMaximum); start them half full
+    gmax = wg_oracle.Oracle(w, variant).get_state()["glo_lake"].copy()
+
+    def run(u):
+        o = wg_oracle.Oracle(w, variant)
+        s = o.get_state()
+        s["glo_lake"][lakes] = 0.5 * gmax[lakes]
+        o.set_state(s)
+        o.set_water_use(np.zeros(n), delayed=0)
+        o.water_use_year(_monthly(u), z, z, z)
+        o.step_day(f[0, 0], f[0, 1], f[0, 2], f[0, 3], 2)
+        return o
+
+    base = run(np.zeros(n)).get_state()
+    checked = 0
+    for c in lakes:
+        if not base["glo_lake"][c] < 0.9 * gmax[c]:  # at its maximum the spill absorbs the use
+            continue
+        u = np.zeros(n)
+        u[c] = 1e-3 * max(abs(base["glo_lake"][c]), 1e-3)
+        o = run(u)
+        d = base["glo_lake"][c] - o.get_state()["glo_lake"][c]
+        k = k_sw[c]
+        ode = u[c] * (1. - np.exp(-k)) / k
+        assert np.isclose(d, ode, rtol=1e-7) or np.isclose(d, u[c], rtol=1e-7), (c, d, ode, u[c])
+        assert o.water_use_state()[0][c] == 0
+        checked += 1
+    assert checked >= 5
